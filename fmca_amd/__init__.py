@@ -1,0 +1,196 @@
+"""MI355X-native H2-matrix kernel action with FMCA's interface.
+
+Python face of the C ABI (``include/fmca_amd.h``), mirroring FMCA's pybind11
+module (py/FMCA.cpp:448-468):
+
+    K = fmca_amd.CovarianceKernel("MATERN32", 0.5)
+    H = fmca_amd.H2Matrix(K, Pr, Pc, 3, 0.8)      # Pr: (d, m), Pc: (d, n) float64
+    y = H.action(x)                              # x: (n, nrhs) float64 -> (m, nrhs)
+
+plus the C++-side name ``H2kernelActor(kernel, Peval, P, deg, eta).action(rhs)``
+(FMCA/src/H2Matrix/H2kernelActor.h:21-38) and device-resident forms that take
+torch tensors.  Interpolation is tensor-product Chebyshev
+(FMCA/src/Interpolator/TensorProductInterpolator.h).
+
+The compiled extension is required: importing this package without it raises
+ImportError (there is no CPU fallback in the product path).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+try:
+    from . import _fmca_amd_py as _C
+except ImportError as e:  # pragma: no cover - exercised only on a broken install
+    raise ImportError(
+        "fmca_amd: the compiled extension is missing; run `python -m fmca_amd.build`"
+    ) from e
+
+FP64 = _C.FP64
+FP32 = _C.FP32
+ROWS = _C.ROWS
+COLS = _C.COLS
+
+KERNEL_TYPES = (
+    "EXPONENTIAL", "MATERN32", "MATERN52", "MATERN72", "MATERN92", "GAUSSIAN",
+    "INVMULTIQUADRIC", "MULTIQUADRIC", "BIHARMONIC2D", "BIHARMONIC3D", "TRIHARMONIC3D",
+    "TPS2D",
+)
+
+
+class CovarianceKernel:
+    """FMCA::CovarianceKernel(ktype, l=1, c=1, nu=1) (CovarianceKernel.h:50-121).
+
+    Parameters are held by value (FMCA's copy keeps reading the original
+    object's lengthscale through captured ``this``, CovarianceKernel.h:57-116).
+    """
+
+    def __init__(self, ktype: str = "EXPONENTIAL", l: float = 1.0, c: float = 1.0,
+                 nu: float = 1.0):
+        kt = str(ktype).upper()
+        if kt not in KERNEL_TYPES:
+            raise ValueError(f"desired kernel not implemented: {ktype!r}")
+        self._ktype = kt
+        self.l = float(l)
+        self.c = float(c)
+        self.nu = float(nu)
+
+    def kernelType(self) -> str:
+        return self._ktype
+
+    def __repr__(self):
+        return f"CovarianceKernel({self._ktype!r}, l={self.l}, c={self.c}, nu={self.nu})"
+
+
+def _points(P, name):
+    P = np.asarray(P)
+    if P.dtype != np.float64:
+        raise TypeError(f"{name} must be a float64 array")
+    if P.ndim != 2:
+        raise ValueError(f"{name} must be a (dim, npts) array")
+    return np.asfortranarray(P)
+
+
+class H2kernelActor:
+    """H2kernelActor(kernel, Peval, P, mpole_deg, eta) -> .action(rhs).
+
+    Peval: (dim, m) targets, P: (dim, n) sources.  ``action`` takes an
+    (n, nrhs) float64 array and returns the (m, nrhs) product in natural order.
+    """
+
+    def __init__(self, kernel: CovarianceKernel, Peval, P, mpole_deg: int = 3,
+                 eta: float = 0.8):
+        Peval = _points(Peval, "Peval")
+        P = _points(P, "P")
+        self.kernel = kernel
+        self.deg = int(mpole_deg)
+        self.eta = float(eta)
+        self._a = _C.Actor(kernel.kernelType(), kernel.l, kernel.c, kernel.nu, Peval, P,
+                           self.deg, self.eta)
+        self.m = self._a.m
+        self.n = self._a.n
+        self.dim = self._a.dim
+
+    # -- host-memory forms (FMCA semantics) ---------------------------------
+    def action(self, rhs, precision: int = FP64) -> np.ndarray:
+        return self._a.action(rhs, precision, False)
+
+    def action_cluster_order(self, rhs, precision: int = FP64) -> np.ndarray:
+        """H2MatrixBase::action: rhs in column-tree order, result in row-tree order."""
+        return self._a.action(rhs, precision, True)
+
+    # -- device-memory forms (torch tensors on the GPU) ----------------------
+    def action_device(self, rhs, out=None, precision: int = FP64, cluster_order: bool = False,
+                      stream=None):
+        """rhs: float64 CUDA tensor (n,) or (n, nrhs); enqueued on the current stream."""
+        import torch
+        x = _as_device_matrix(rhs, self.n)
+        nrhs = x.shape[1]
+        if out is None:
+            out = torch.empty((nrhs, self.m), dtype=torch.float64, device=x.device).t()
+        _check_out(out, self.m, nrhs)
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        self._a.action_device(x.data_ptr(), x.stride(1) if nrhs > 1 else self.n, nrhs,
+                              out.data_ptr(), out.stride(1) if nrhs > 1 else self.m,
+                              precision, cluster_order, s.cuda_stream)
+        return out if rhs.dim() == 2 else out[:, 0]
+
+    def action_transpose_device(self, rhs, out=None, precision: int = FP64, stream=None):
+        """K^T rhs: rhs (m,) or (m, nrhs) -> (n, nrhs), natural order."""
+        import torch
+        x = _as_device_matrix(rhs, self.m)
+        nrhs = x.shape[1]
+        if out is None:
+            out = torch.empty((nrhs, self.n), dtype=torch.float64, device=x.device).t()
+        _check_out(out, self.n, nrhs)
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        self._a.action_transpose_device(x.data_ptr(), x.stride(1) if nrhs > 1 else self.m, nrhs,
+                                        out.data_ptr(), out.stride(1) if nrhs > 1 else self.n,
+                                        precision, s.cuda_stream)
+        return out if rhs.dim() == 2 else out[:, 0]
+
+    # -- plan queries ---------------------------------------------------------
+    def upload(self):
+        self._a.upload()
+
+    def indices(self, which: int) -> np.ndarray:
+        return self._a.indices(which)
+
+    def rindices(self) -> np.ndarray:
+        return self._a.indices(ROWS)
+
+    def cindices(self) -> np.ndarray:
+        return self._a.indices(COLS)
+
+    def statistics(self) -> np.ndarray:
+        return self._a.statistics()
+
+    def info(self) -> dict:
+        return self._a.info()
+
+    def clusters(self, which: int) -> dict:
+        d = self._a.clusters(which)
+        bb = d.pop("bb")
+        d["bbmin"], d["bbmax"], d["bbsize"] = bb[:, 0, :], bb[:, 1, :], bb[:, 2, :]
+        return d
+
+    def blocks(self):
+        r, c, lr = self._a.blocks()
+        return r, c, lr.astype(bool)
+
+
+class H2Matrix(H2kernelActor):
+    """FMCA.H2Matrix(CovarianceKernel, Pr, Pc, p=3, eta=0.8) (py/FMCA.cpp:209-250).
+
+    ``action(rhs)`` permutes into the column tree, applies H2MatrixBase::action
+    and permutes back, i.e. it returns K(Pr, Pc) @ rhs in natural order.
+    """
+
+    def __init__(self, cov: CovarianceKernel, Pr, Pc, p: int = 3, eta: float = 0.8):
+        super().__init__(cov, Pr, Pc, p, eta)
+
+
+def _as_device_matrix(t, rows):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError("expected a float64 CUDA tensor")
+    x = t if t.dim() == 2 else t.reshape(-1, 1)
+    if x.shape[0] != rows:
+        raise ValueError(f"expected {rows} rows, got {x.shape[0]}")
+    if x.shape[1] > 1 and (x.stride(0) != 1 or x.stride(1) < rows):
+        x = x.t().contiguous().t()  # column-major copy
+    elif x.shape[1] == 1 and not x.is_contiguous():
+        x = x.contiguous()
+    return x
+
+
+def _check_out(out, rows, nrhs):
+    import torch
+    if out.dtype != torch.float64 or not out.is_cuda or tuple(out.shape) != (rows, nrhs):
+        raise ValueError("out must be a float64 CUDA tensor of shape (rows, nrhs)")
+    if nrhs > 1 and (out.stride(0) != 1 or out.stride(1) < rows):
+        raise ValueError("out must be column-major")
+
+
+__all__ = ["CovarianceKernel", "H2Matrix", "H2kernelActor", "FP64", "FP32", "ROWS", "COLS",
+           "KERNEL_TYPES"]

@@ -1,0 +1,537 @@
+// C-ABI implementation: the actor owns the host plan and its HBM copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fmca_amd.h"
+#include "device.hpp"
+#include "plan.hpp"
+
+using namespace fmca_amd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+fmca_status fail(fmca_status st, const std::string &msg) {
+  g_last_error = msg;
+  return st;
+}
+
+fmca_status from_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return FMCA_OK;
+  const fmca_status st = e == hipErrorOutOfMemory ? FMCA_ERR_OUT_OF_MEMORY
+                         : (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+                             ? FMCA_ERR_NO_DEVICE
+                             : FMCA_ERR_DEVICE;
+  return fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                            \
+  do {                                                           \
+    hipError_t _e = (expr);                                      \
+    if (_e != hipSuccess) return from_hip(_e, #expr);            \
+  } while (0)
+
+// A device allocation owned by the actor.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(b, 256));
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <typename T>
+  hipError_t upload(const std::vector<T> &v) {
+    hipError_t e = alloc(v.size() * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// Integer arrays of one tree on the device (shared by both precisions).
+struct DevTreeInts {
+  DevBuf perm, begin, size, son0, son1, parent, leaves;
+};
+
+template <typename T>
+struct DevTreeReals {
+  DevBuf pts, bbmin, bbsize, transfer;
+};
+
+struct DevPattern {
+  DevBuf far_ptr, far_src, far_targets, near_ptr, near_src;
+  DevPatternView view(const Pattern &p) const {
+    DevPatternView v;
+    v.nfar_targets = (int)p.far_targets.size();
+    v.far_ptr = far_ptr.as<int32_t>();
+    v.far_src = far_src.as<int32_t>();
+    v.far_targets = far_targets.as<int32_t>();
+    v.nnear_leaves = (int)p.near_ptr.size() - 1;
+    v.near_ptr = near_ptr.as<int32_t>();
+    v.near_src = near_src.as<int32_t>();
+    return v;
+  }
+  hipError_t upload(const Pattern &p) {
+    hipError_t e;
+    if ((e = far_ptr.upload(p.far_ptr)) != hipSuccess) return e;
+    if ((e = far_src.upload(p.far_src)) != hipSuccess) return e;
+    if ((e = far_targets.upload(p.far_targets)) != hipSuccess) return e;
+    if ((e = near_ptr.upload(p.near_ptr)) != hipSuccess) return e;
+    return near_src.upload(p.near_src);
+  }
+};
+
+template <typename T>
+struct Workspace {
+  DevBuf xs, mom, tlhs;
+};
+
+}  // namespace
+
+struct fmca_h2_actor {
+  int dim = 0, deg = 0;
+  int64_t m = 0, n = 0;
+  double eta = 0;
+  int kt = 0;
+  double l = 1, c = 1, nu = 1;
+  Interpolation ip;
+  Tree rows_tree, cols_tree;
+  bool shared = false;  // Peval and P are the same point set: one tree
+  Pattern pat, pat_t;
+  bool have_pat_t = false;
+  std::vector<double> trans_rows, trans_cols;
+  std::vector<double> pts_rows, pts_cols;  // dim-major, cluster order
+  // device state
+  int device = -1;
+  bool uploaded = false;
+  DevTreeInts d_rows_i, d_cols_i;
+  DevTreeReals<double> d_rows_64, d_cols_64;
+  DevTreeReals<float> d_rows_32, d_cols_32;
+  DevPattern d_pat, d_pat_t;
+  Workspace<double> ws64;
+  Workspace<float> ws32;
+  DevBuf stage_in, stage_out;
+  hipStream_t stream = nullptr;
+  std::mutex mtx;
+
+  const Tree &rows() const { return rows_tree; }
+  const Tree &cols() const { return shared ? rows_tree : cols_tree; }
+
+  ~fmca_h2_actor() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+std::vector<double> cluster_order_points(const Tree &T, const double *P) {
+  const int d = T.dim;
+  std::vector<double> out((size_t)d * T.npts);
+  for (int i = 0; i < d; ++i)
+    for (int64_t j = 0; j < T.npts; ++j)
+      out[(size_t)i * T.npts + j] = P[i + (size_t)d * T.perm[j]];
+  return out;
+}
+
+template <typename T>
+std::vector<T> convert(const std::vector<double> &v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+hipError_t upload_ints(DevTreeInts &d, const Tree &T) {
+  hipError_t e;
+  std::vector<int32_t> b(T.begin.begin(), T.begin.end()), s(T.size.begin(), T.size.end());
+  if ((e = d.perm.upload(T.perm)) != hipSuccess) return e;
+  if ((e = d.begin.upload(b)) != hipSuccess) return e;
+  if ((e = d.size.upload(s)) != hipSuccess) return e;
+  if ((e = d.son0.upload(T.son0)) != hipSuccess) return e;
+  if ((e = d.son1.upload(T.son1)) != hipSuccess) return e;
+  if ((e = d.parent.upload(T.parent)) != hipSuccess) return e;
+  return d.leaves.upload(T.leaves);
+}
+
+template <typename T>
+hipError_t upload_reals(DevTreeReals<T> &d, const Tree &tr, const std::vector<double> &pts,
+                        const std::vector<double> &transfer) {
+  hipError_t e;
+  if ((e = d.pts.upload(convert<T>(pts))) != hipSuccess) return e;
+  if ((e = d.bbmin.upload(convert<T>(tr.bbmin))) != hipSuccess) return e;
+  if ((e = d.bbsize.upload(convert<T>(tr.bbsize))) != hipSuccess) return e;
+  return d.transfer.upload(convert<T>(transfer));
+}
+
+template <typename T>
+DevTreeView<T> make_view(const Tree &tr, const DevTreeInts &di, const DevTreeReals<T> &dr) {
+  DevTreeView<T> v;
+  v.dim = tr.dim;
+  v.ncl = tr.ncl;
+  v.nleaves = (int)tr.leaves.size();
+  v.depth = tr.depth;
+  v.npts = tr.npts;
+  v.perm = di.perm.as<uint32_t>();
+  v.pts = dr.pts.template as<T>();
+  v.bbmin = dr.bbmin.template as<T>();
+  v.bbsize = dr.bbsize.template as<T>();
+  v.begin = di.begin.as<int32_t>();
+  v.size = di.size.as<int32_t>();
+  v.son0 = di.son0.as<int32_t>();
+  v.son1 = di.son1.as<int32_t>();
+  v.parent = di.parent.as<int32_t>();
+  v.leaves = di.leaves.as<int32_t>();
+  v.transfer = dr.transfer.template as<T>();
+  return v;
+}
+
+fmca_status ensure_device(fmca_h2_actor *a) {
+  if (a->uploaded) {
+    HIP_TRY(hipSetDevice(a->device));
+    return FMCA_OK;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(FMCA_ERR_NO_DEVICE, "no HIP device available");
+  HIP_TRY(hipGetDevice(&a->device));
+  HIP_TRY(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+  HIP_TRY(upload_ints(a->d_rows_i, a->rows()));
+  HIP_TRY(upload_reals<double>(a->d_rows_64, a->rows(), a->pts_rows, a->trans_rows));
+  HIP_TRY(upload_reals<float>(a->d_rows_32, a->rows(), a->pts_rows, a->trans_rows));
+  if (!a->shared) {
+    HIP_TRY(upload_ints(a->d_cols_i, a->cols()));
+    HIP_TRY(upload_reals<double>(a->d_cols_64, a->cols(), a->pts_cols, a->trans_cols));
+    HIP_TRY(upload_reals<float>(a->d_cols_32, a->cols(), a->pts_cols, a->trans_cols));
+  }
+  HIP_TRY(a->d_pat.upload(a->pat));
+  a->uploaded = true;
+  return FMCA_OK;
+}
+
+template <typename T>
+InterpParams<T> interp_params(const Interpolation &ip) {
+  InterpParams<T> p;
+  p.P1 = ip.P1;
+  p.q = ip.q;
+  for (int k = 0; k < kMaxP1; ++k) {
+    p.xi[k] = k < ip.P1 ? (T)ip.xi[k] : T(0);
+    p.w[k] = k < ip.P1 ? (T)ip.w[k] : T(0);
+  }
+  return p;
+}
+
+template <typename T>
+KernelParams<T> kernel_params(const fmca_h2_actor *a) {
+  KernelParams<T> k;
+  k.kt = a->kt;
+  k.l = (T)a->l;
+  k.c = (T)a->c;
+  k.nu = (T)a->nu;
+  return k;
+}
+
+// Enqueue one action on `stream`.  transpose swaps the roles of the trees.
+template <typename T>
+fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r_rows,
+                    const DevTreeReals<T> &r_cols, const double *d_rhs, int64_t ld_rhs,
+                    int64_t nrhs, double *d_out, int64_t ld_out, bool natural, bool transpose,
+                    hipStream_t stream) {
+  const Tree &tr_rows = a->rows();
+  const Tree &tr_cols = a->cols();
+  const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
+  DevTreeView<T> v_rows = make_view<T>(tr_rows, a->d_rows_i, r_rows);
+  DevTreeView<T> v_cols = make_view<T>(tr_cols, i_cols, r_cols);
+  const Pattern *pat = &a->pat;
+  const DevPattern *dpat = &a->d_pat;
+  if (transpose) {
+    std::swap(v_rows, v_cols);
+    if (!a->shared) {
+      pat = &a->pat_t;
+      dpat = &a->d_pat_t;
+    }
+  }
+  const Tree &t_tgt = transpose ? tr_cols : tr_rows;
+  const Tree &t_src = transpose ? tr_rows : tr_cols;
+  HostLevels l_tgt{t_tgt.depth, t_tgt.level_ptr.data()};
+  HostLevels l_src{t_src.depth, t_src.level_ptr.data()};
+  const int q = a->ip.q;
+  HIP_TRY(ws.xs.alloc((size_t)t_src.npts * nrhs * sizeof(T)));
+  HIP_TRY(ws.mom.alloc((size_t)t_src.ncl * q * nrhs * sizeof(T)));
+  HIP_TRY(ws.tlhs.alloc((size_t)t_tgt.ncl * q * nrhs * sizeof(T)));
+  ActionBuffers<T> buf{ws.xs.template as<T>(), ws.mom.template as<T>(), ws.tlhs.template as<T>()};
+  hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
+                                  interp_params<T>(a->ip), kernel_params<T>(a), d_rhs, ld_rhs,
+                                  (int)nrhs, d_out, ld_out, natural, buf, stream);
+  return from_hip(e, "launch_action");
+}
+
+fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
+                        double *d_out, int64_t ld_out, int precision, bool natural,
+                        bool transpose, hipStream_t stream) {
+  if (transpose && !a->shared && !a->have_pat_t) {
+    compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
+    HIP_TRY(a->d_pat_t.upload(a->pat_t));
+    a->have_pat_t = true;
+  }
+  if (precision == FMCA_PRECISION_FP64)
+    return enqueue<double>(a, a->ws64, a->d_rows_64, a->shared ? a->d_rows_64 : a->d_cols_64,
+                           d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream);
+  if (precision == FMCA_PRECISION_FP32)
+    return enqueue<float>(a, a->ws32, a->d_rows_32, a->shared ? a->d_rows_32 : a->d_cols_32,
+                          d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream);
+  return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+}
+
+fmca_status host_action(fmca_h2_actor *a, const double *rhs, int64_t ld_rhs, int64_t nrhs,
+                        double *out, int64_t ld_out, int precision, bool natural) {
+  if (!a || (!rhs && nrhs) || (!out && nrhs) || nrhs < 0)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or negative nrhs");
+  if (ld_rhs < a->n || ld_out < a->m) return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (nrhs == 0) return FMCA_OK;
+  std::lock_guard<std::mutex> lock(a->mtx);
+  fmca_status st = ensure_device(a);
+  if (st != FMCA_OK) return st;
+  const size_t in_bytes = (size_t)a->n * nrhs * sizeof(double);
+  const size_t out_bytes = (size_t)a->m * nrhs * sizeof(double);
+  HIP_TRY(a->stage_in.alloc(in_bytes));
+  HIP_TRY(a->stage_out.alloc(out_bytes));
+  HIP_TRY(hipMemcpy2DAsync(a->stage_in.p, a->n * sizeof(double), rhs, ld_rhs * sizeof(double),
+                           a->n * sizeof(double), nrhs, hipMemcpyHostToDevice, a->stream));
+  st = enqueue_any(a, a->stage_in.as<double>(), a->n, nrhs, a->stage_out.as<double>(), a->m,
+                   precision, natural, false, a->stream);
+  if (st != FMCA_OK) return st;
+  HIP_TRY(hipMemcpy2DAsync(out, ld_out * sizeof(double), a->stage_out.p, a->m * sizeof(double),
+                           a->m * sizeof(double), nrhs, hipMemcpyDeviceToHost, a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  return FMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fmca_status_string(fmca_status s) {
+  switch (s) {
+    case FMCA_OK:
+      return "ok";
+    case FMCA_ERR_INVALID_ARGUMENT:
+      return "invalid argument";
+    case FMCA_ERR_UNKNOWN_KERNEL:
+      return "unknown kernel type";
+    case FMCA_ERR_NOT_SUPPORTED:
+      return "not supported";
+    case FMCA_ERR_NO_DEVICE:
+      return "no HIP device";
+    case FMCA_ERR_OUT_OF_MEMORY:
+      return "out of device memory";
+    case FMCA_ERR_DEVICE:
+      return "device error";
+    case FMCA_ERR_NOT_CONVERGED:
+      return "not converged";
+    case FMCA_ERR_NONFINITE:
+      return "non-finite value";
+  }
+  return "unknown status";
+}
+
+const char *fmca_last_error(void) { return g_last_error.c_str(); }
+
+fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
+                           const double *Peval, int64_t n, const double *P, int deg, double eta,
+                           fmca_h2_actor **actor) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  *actor = nullptr;
+  if (!kernel || !kernel->ktype) return fail(FMCA_ERR_INVALID_ARGUMENT, "kernel is null");
+  const int kt = parse_kernel_type(kernel->ktype);
+  if (kt < 0) return fail(FMCA_ERR_UNKNOWN_KERNEL, std::string("kernel ") + kernel->ktype);
+  if (dim < 1 || dim > kMaxDim)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "dimension must be 1, 2 or 3");
+  if (deg < 0 || deg + 1 > kMaxP1) return fail(FMCA_ERR_NOT_SUPPORTED, "degree must be 0..9");
+  if (m < 1 || n < 1 || !P || !Peval) return fail(FMCA_ERR_INVALID_ARGUMENT, "empty point set");
+  if (n > (int64_t)INT32_MAX || m > (int64_t)INT32_MAX)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "more than 2^31-1 points");
+  if (!(eta >= 0) || !std::isfinite(eta)) return fail(FMCA_ERR_INVALID_ARGUMENT, "eta");
+  if (!(kernel->l > 0) || !std::isfinite(kernel->l))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "lengthscale must be positive");
+  for (int64_t k = 0; k < (int64_t)dim * n; ++k)
+    if (!std::isfinite(P[k])) return fail(FMCA_ERR_NONFINITE, "non-finite source coordinate");
+  for (int64_t k = 0; k < (int64_t)dim * m; ++k)
+    if (!std::isfinite(Peval[k])) return fail(FMCA_ERR_NONFINITE, "non-finite target coordinate");
+  std::unique_ptr<fmca_h2_actor> a;
+  try {
+    a.reset(new fmca_h2_actor());
+    a->dim = dim;
+    a->deg = deg;
+    a->m = m;
+    a->n = n;
+    a->eta = eta;
+    a->kt = kt;
+    a->l = kernel->l;
+    a->c = kernel->c;
+    a->nu = kernel->nu;
+    a->ip.init(dim, deg);
+    // H2ClusterTree::init (H2ClusterTree.h:65-74): minimum cluster size q
+    a->shared = (P == Peval && m == n) ||
+                (m == n && std::memcmp(P, Peval, sizeof(double) * dim * n) == 0);
+    build_tree(a->rows_tree, dim, m, Peval, a->ip.q);
+    if (!a->shared) build_tree(a->cols_tree, dim, n, P, a->ip.q);
+    compute_pattern(a->pat, a->rows(), a->cols(), eta, a->ip.q);
+    compute_transfers(a->trans_rows, a->rows(), a->ip);
+    a->pts_rows = cluster_order_points(a->rows(), Peval);
+    if (!a->shared) {
+      compute_transfers(a->trans_cols, a->cols(), a->ip);
+      a->pts_cols = cluster_order_points(a->cols(), P);
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(FMCA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+  }
+  *actor = a.release();
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_destroy(fmca_h2_actor *actor) {
+  if (!actor) return FMCA_OK;
+  if (actor->uploaded) (void)hipSetDevice(actor->device);
+  delete actor;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_upload(fmca_h2_actor *actor) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  return ensure_device(actor);
+}
+
+fmca_status fmca_h2_action(fmca_h2_actor *actor, const double *rhs, int64_t ld_rhs, int64_t nrhs,
+                           double *out, int64_t ld_out, int precision) {
+  return host_action(actor, rhs, ld_rhs, nrhs, out, ld_out, precision, true);
+}
+
+fmca_status fmca_h2_action_cluster_order(fmca_h2_actor *actor, const double *rhs, int64_t ld_rhs,
+                                         int64_t nrhs, double *out, int64_t ld_out,
+                                         int precision) {
+  return host_action(actor, rhs, ld_rhs, nrhs, out, ld_out, precision, false);
+}
+
+fmca_status fmca_h2_action_device(fmca_h2_actor *actor, const double *d_rhs, int64_t ld_rhs,
+                                  int64_t nrhs, double *d_out, int64_t ld_out, int precision,
+                                  int cluster_order, void *stream) {
+  if (!actor || nrhs < 0 || (nrhs && (!d_rhs || !d_out)))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or negative nrhs");
+  if (ld_rhs < actor->n || ld_out < actor->m)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (nrhs == 0) return FMCA_OK;
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  fmca_status st = ensure_device(actor);
+  if (st != FMCA_OK) return st;
+  return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, cluster_order == 0,
+                     false, (hipStream_t)stream);
+}
+
+fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *d_rhs,
+                                            int64_t ld_rhs, int64_t nrhs, double *d_out,
+                                            int64_t ld_out, int precision, void *stream) {
+  if (!actor || nrhs < 0 || (nrhs && (!d_rhs || !d_out)))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or negative nrhs");
+  if (ld_rhs < actor->m || ld_out < actor->n)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (nrhs == 0) return FMCA_OK;
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  fmca_status st = ensure_device(actor);
+  if (st != FMCA_OK) return st;
+  return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, true, true,
+                     (hipStream_t)stream);
+}
+
+fmca_status fmca_h2_indices(const fmca_h2_actor *actor, int which, uint32_t *out) {
+  if (!actor || !out) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  const Tree &T = which == FMCA_ROWS ? actor->rows() : actor->cols();
+  std::copy(T.perm.begin(), T.perm.end(), out);
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_statistics(const fmca_h2_actor *actor, double out[6]) {
+  if (!actor || !out) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  const Statistics s = statistics(actor->pat, actor->rows(), actor->cols(), actor->ip.q);
+  out[0] = s.rows;
+  out[1] = s.cols;
+  out[2] = s.low_rank_blocks;
+  out[3] = s.dense_blocks;
+  out[4] = s.nz_per_row;
+  out[5] = s.storage_gb;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_info(const fmca_h2_actor *actor, double out[13]) {
+  if (!actor || !out) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  out[0] = actor->dim;
+  out[1] = (double)actor->m;
+  out[2] = (double)actor->n;
+  out[3] = actor->ip.q;
+  out[4] = actor->rows().ncl;
+  out[5] = actor->cols().ncl;
+  out[6] = (double)actor->pat.n_far;
+  out[7] = (double)actor->pat.n_near;
+  out[8] = actor->pat.near_entries;
+  out[9] = actor->pat.far_entries;
+  out[10] = actor->rows().depth;
+  out[11] = actor->cols().depth;
+  out[12] = actor->shared ? 1 : 0;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_clusters(const fmca_h2_actor *actor, int which, int64_t *begin, int64_t *size,
+                             int32_t *level, int32_t *sons, double *bb) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  const Tree &T = which == FMCA_ROWS ? actor->rows() : actor->cols();
+  const int d = T.dim;
+  for (int c = 0; c < T.ncl; ++c) {
+    if (begin) begin[c] = T.begin[c];
+    if (size) size[c] = T.size[c];
+    if (level) level[c] = T.level[c];
+    if (sons) {
+      sons[2 * c] = T.son0[c];
+      sons[2 * c + 1] = T.son1[c];
+    }
+    if (bb)
+      for (int i = 0; i < d; ++i) {
+        bb[(size_t)3 * d * c + i] = T.bbmin[(size_t)c * d + i];
+        bb[(size_t)3 * d * c + d + i] = T.bbmax[(size_t)c * d + i];
+        bb[(size_t)3 * d * c + 2 * d + i] = T.bbsize[(size_t)c * d + i];
+      }
+  }
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *col,
+                           int8_t *low_rank) {
+  if (!actor || !row || !col || !low_rank) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  const Pattern &p = actor->pat;
+  std::copy(p.block_row.begin(), p.block_row.end(), row);
+  std::copy(p.block_col.begin(), p.block_col.end(), col);
+  std::copy(p.block_lr.begin(), p.block_lr.end(), low_rank);
+  return FMCA_OK;
+}
+
+}  // extern "C"

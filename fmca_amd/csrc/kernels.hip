@@ -1,0 +1,702 @@
+// HIP kernels of the matrix-free H2 action for gfx950 (MI355X).
+//
+// Pipeline of one action y = K(rows, cols) x (FMCA H2kernelActor::action,
+// FMCA/src/H2Matrix/H2kernelActor.h:38-80), all on one stream:
+//   1. permute_in        rhs -> cluster order, point-major (rhs fastest)
+//   2. upward_leaf       leaf moments  m_t = V_t x_t, V generated from the points
+//   3. upward_transfer   per level, m_t = sum_sons (T0 (x) T1 (x) T2) m_son
+//   4. coupling          per target cluster, l_t = sum_far K(xi_t, xi_s) m_s
+//   5. downward_transfer per level, l_son += (T0 (x) T1 (x) T2)^T l_t
+//   6. leaf_output       per target leaf, y = V_t^T l_t + sum_near K(P_t, P_s) x_s,
+//                        written straight to the caller's layout
+// Kernel entries are generated in registers and never stored.  Every output
+// element is owned by exactly one workgroup (gather by target), so no atomics
+// are needed and results are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace fmca_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Scalar helpers.  FP64 uses the ROCm device math library; FP32 uses the
+// hardware transcendental instructions (v_exp_f32 / v_sqrt_f32).
+template <typename T>
+struct Math;
+
+template <>
+struct Math<double> {
+  static __device__ __forceinline__ double exp_(double x) { return exp(x); }
+  static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double log_(double x) { return log(x); }
+  static __device__ __forceinline__ double eps() { return 2.220446049250313e-16; }
+};
+
+template <>
+struct Math<float> {
+  static __device__ __forceinline__ float exp_(float x) {
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float log_(float x) { return logf(x); }
+  static __device__ __forceinline__ float eps() { return 1.1920928955078125e-07f; }
+};
+
+// Kernel value from the squared distance (CovarianceKernel.h:56-116).  KT < 0
+// selects the type at run time (uniform branch) for the less common kernels.
+template <int KT, typename T>
+__device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
+  using M = Math<T>;
+  const int kt = KT >= 0 ? KT : kp.kt;
+  const T l = kp.l;
+  switch (kt) {
+    case KT_EXPONENTIAL:
+      return M::exp_(-M::sqrt_(r2) / l);
+    case KT_MATERN32: {
+      const T a = T(1.7320508075688772935) * M::sqrt_(r2) / l;
+      return (T(1) + a) * M::exp_(-a);
+    }
+    case KT_MATERN52: {
+      const T a = T(2.2360679774997896964) * M::sqrt_(r2) / l;
+      return (T(1) + a + a * a / T(3)) * M::exp_(-a);
+    }
+    case KT_MATERN72: {
+      const T r = M::sqrt_(r2), x = r / l;
+      return (T(1) + T(2.6457513110645905905) * x + T(14. / 5) * x * x +
+              T(7. * 2.6457513110645905905 / 15.) * x * x * x) *
+             M::exp_(-T(2.6457513110645905905) * x);
+    }
+    case KT_MATERN92: {
+      const T r = M::sqrt_(r2), x = r / l;
+      return (T(1) + T(3) * x + T(27. / 7.) * x * x + T(18. / 7.) * x * x * x +
+              T(27. / 35.) * x * x * x * x) *
+             M::exp_(-T(3) * x);
+    }
+    case KT_GAUSSIAN:
+      return M::exp_(T(-0.5) * r2 / (l * l));
+    case KT_INVMULTIQUADRIC:
+      return T(1) / M::sqrt_(r2 / (l * l) + kp.c * kp.c);
+    case KT_MULTIQUADRIC:
+      return M::sqrt_(r2 / (l * l) + kp.c * kp.c);
+    case KT_BIHARMONIC2D: {
+      const T r = M::sqrt_(r2);
+      return r < M::eps() ? T(0) : M::log_(r / l) * (r / l) * (r / l);
+    }
+    case KT_BIHARMONIC3D:
+      return M::sqrt_(r2) / l;
+    case KT_TRIHARMONIC3D: {
+      const T x = M::sqrt_(r2) / l;
+      return x * x * x;
+    }
+    case KT_TPS2D: {
+      const T r = M::sqrt_(r2);
+      return r > T(0) ? r * r * M::log_(r) : T(0);
+    }
+  }
+  return T(0);
+}
+
+// Barycentric Lagrange factors l_a(u), a < P1, written with a stride
+// (TensorProductInterpolator.h:64-81; a factor within eps of its node is 1).
+template <typename T>
+__device__ __forceinline__ void lagrange_1d(T u, const InterpParams<T> &ip, T *out, int stride) {
+  T s = 0;
+  for (int j = 0; j < ip.P1; ++j) s += ip.w[j] / (u - ip.xi[j]);
+  for (int a = 0; a < ip.P1; ++a) {
+    const T d = u - ip.xi[a];
+    out[a * stride] = fabs(d) > Math<T>::eps() ? ip.w[a] / d / s : T(1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1. rhs (column-major, natural order) -> xs (cluster order, point-major)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_in(const double *__restrict__ rhs,
+                                                       int64_t ld, int nrhs, int64_t n,
+                                                       const uint32_t *__restrict__ perm,
+                                                       T *__restrict__ xs) {
+  const int64_t total = n * nrhs;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / nrhs;
+    const int r = (int)(e - j * nrhs);
+    const int64_t src = perm ? (int64_t)perm[j] : j;
+    xs[e] = (T)rhs[src + r * ld];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. leaf moments: mom[t][alpha][r] = sum_p V_t[alpha][p] xs[p][r]
+// One workgroup per leaf; the Lagrange factors of 64 points at a time are
+// staged in LDS and the tensor-product basis value is formed on the fly.
+template <typename T, int D, int NR>
+__global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, InterpParams<T> ip,
+                                                        const T *__restrict__ xs, int nrhs,
+                                                        T *__restrict__ mom) {
+  extern __shared__ unsigned char smem_raw[];
+  T *F = reinterpret_cast<T *>(smem_raw);  // [D][P1][64]
+  T *X = F + D * ip.P1 * 64;               // [64][NR]
+  const int c = tr.leaves[blockIdx.x];
+  const int b0 = tr.begin[c], sz = tr.size[c];
+  const int P1 = ip.P1, q = ip.q;
+  T mn[D], inv[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    mn[i] = tr.bbmin[c * D + i];
+    inv[i] = tr.bbsize[c * D + i];
+  }
+  for (int rc = 0; rc < nrhs; rc += NR) {
+    const int nr = min(NR, nrhs - rc);
+    for (int abase = 0; abase < q; abase += kBlock) {
+      const int alpha = abase + threadIdx.x;
+      int a[D];
+      {
+        int rem = alpha;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          a[i] = rem % P1;
+          rem /= P1;
+        }
+      }
+      T acc[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) acc[r] = 0;
+      for (int p0 = 0; p0 < sz; p0 += 64) {
+        const int np = min(64, sz - p0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < D * 64; e += kBlock) {
+          const int i = e / 64, jj = e % 64;
+          if (jj < np) {
+            const T x = tr.pts[(int64_t)i * tr.npts + b0 + p0 + jj];
+            lagrange_1d<T>((x - mn[i]) / inv[i], ip, F + (i * P1) * 64 + jj, 64);
+          }
+        }
+        for (int e = threadIdx.x; e < 64 * NR; e += kBlock) {
+          const int jj = e / NR, r = e % NR;
+          X[e] = (jj < np && r < nr) ? xs[(int64_t)(b0 + p0 + jj) * nrhs + rc + r] : T(0);
+        }
+        __syncthreads();
+        if (alpha < q) {
+          for (int jj = 0; jj < np; ++jj) {
+            T v = F[(0 * P1 + a[0]) * 64 + jj];
+#pragma unroll
+            for (int i = 1; i < D; ++i) v *= F[(i * P1 + a[i]) * 64 + jj];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += v * X[jj * NR + r];
+          }
+        }
+      }
+      if (alpha < q)
+        for (int r = 0; r < nr; ++r) mom[((int64_t)c * q + alpha) * nrhs + rc + r] = acc[r];
+    }
+  }
+}
+
+// Mode-i product with a P1 x P1 matrix on a [q][nr] tensor held in LDS:
+// out[lin with a_i] = sum_b M[a_i][b] in[lin with b]   (transpose: M[b][a_i]).
+template <typename T>
+__device__ __forceinline__ void mode_product(const T *__restrict__ in, T *__restrict__ out,
+                                             const T *__restrict__ M, int P1, int q, int nr,
+                                             int stride, bool transpose) {
+  for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
+    const int lin = e / nr, r = e - lin * nr;
+    const int ai = (lin / stride) % P1;
+    const int base = lin - ai * stride;
+    T s = 0;
+    for (int b = 0; b < P1; ++b) {
+      const T m = transpose ? M[b * P1 + ai] : M[ai * P1 + b];
+      s += m * in[(base + b * stride) * nr + r];
+    }
+    out[e] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. upward transfer for the clusters [cbeg, cend) of one level:
+//    mom[t] = sum_{s in sons(t)} (T_s0 (x) ... ) mom[s]
+template <typename T, int D>
+__global__ __launch_bounds__(kBlock) void k_upward_transfer(DevTreeView<T> tr, InterpParams<T> ip,
+                                                            int cbeg, int nrhs, int nrc,
+                                                            T *__restrict__ mom) {
+  extern __shared__ unsigned char smem_raw[];
+  const int P1 = ip.P1, q = ip.q;
+  const int c = cbeg + blockIdx.x;
+  if (tr.son0[c] < 0) return;
+  T *A = reinterpret_cast<T *>(smem_raw);
+  T *B = A + q * nrc;
+  T *O = B + q * nrc;
+  T *Tm = O + q * nrc;  // D x P1 x P1
+  for (int rc = 0; rc < nrhs; rc += nrc) {
+    const int nr = min(nrc, nrhs - rc);
+    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) O[e] = 0;
+    for (int si = 0; si < 2; ++si) {
+      const int s = si ? tr.son1[c] : tr.son0[c];
+      __syncthreads();
+      for (int e = threadIdx.x; e < D * P1 * P1; e += blockDim.x)
+        Tm[e] = tr.transfer[(int64_t)s * D * P1 * P1 + e];
+      for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
+        const int lin = e / nr, r = e - lin * nr;
+        A[e] = mom[((int64_t)s * q + lin) * nrhs + rc + r];
+      }
+      __syncthreads();
+      T *cur = A, *nxt = B;
+      int stride = 1;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        mode_product<T>(cur, nxt, Tm + i * P1 * P1, P1, q, nr, stride, false);
+        __syncthreads();
+        T *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        stride *= P1;
+      }
+      for (int e = threadIdx.x; e < q * nr; e += blockDim.x) O[e] += cur[e];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
+      const int lin = e / nr, r = e - lin * nr;
+      mom[((int64_t)c * q + lin) * nrhs + rc + r] = O[e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. coupling: tlhs[t] = sum_{s in far(t)} K(xi_t, xi_s) mom[s]
+// (unsymmetricNystromEvaluator.h:42-58 with invV = I, H2kernelActor.h:64-66).
+// One workgroup per target cluster; thread <-> target interpolation node.
+template <typename T, int D, int KT, int NR>
+__global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTreeView<T> cols,
+                                                     DevPatternView pat, InterpParams<T> ip,
+                                                     KernelParams<T> kp, int nrhs,
+                                                     const T *__restrict__ mom,
+                                                     T *__restrict__ tlhs) {
+  extern __shared__ unsigned char smem_raw[];
+  const int P1 = ip.P1, q = ip.q;
+  T *Y = reinterpret_cast<T *>(smem_raw);  // [D][P1] source nodes
+  T *Mv = Y + D * kMaxP1;                  // [q][NR] source moments
+  const int t = blockIdx.x;
+  const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
+  for (int rc = 0; rc < nrhs; rc += NR) {
+    const int nr = min(NR, nrhs - rc);
+    for (int abase = 0; abase < q; abase += blockDim.x) {
+      const int alpha = abase + threadIdx.x;
+      T x[D];
+      {
+        int rem = alpha < q ? alpha : 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          const int ai = rem % P1;
+          rem /= P1;
+          x[i] = ip.xi[ai] * rows.bbsize[t * D + i] + rows.bbmin[t * D + i];
+        }
+      }
+      T acc[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) acc[r] = 0;
+      for (int f = f0; f < f1; ++f) {
+        const int s = pat.far_src[f];
+        __syncthreads();
+        for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
+          const int i = e / P1, b = e % P1;
+          Y[i * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + i] + cols.bbmin[s * D + i];
+        }
+        for (int e = threadIdx.x; e < q * NR; e += blockDim.x) {
+          const int beta = e / NR, r = e % NR;
+          Mv[e] = r < nr ? mom[((int64_t)s * q + beta) * nrhs + rc + r] : T(0);
+        }
+        __syncthreads();
+        if (alpha < q) {
+          int b[D];
+#pragma unroll
+          for (int i = 0; i < D; ++i) b[i] = 0;
+          for (int beta = 0; beta < q; ++beta) {
+            T r2 = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+              const T dd = x[i] - Y[i * kMaxP1 + b[i]];
+              r2 += dd * dd;
+            }
+            const T kv = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
+            // advance the tensor index (uniform across the wave)
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+              if (++b[i] < P1) break;
+              b[i] = 0;
+            }
+          }
+        }
+      }
+      if (alpha < q)
+        for (int r = 0; r < nr; ++r) tlhs[((int64_t)t * q + alpha) * nrhs + rc + r] = acc[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. downward transfer for the clusters [cbeg, cend) of one level >= 1:
+//    tlhs[c] += (T_c0 (x) ...)^T tlhs[parent(c)]
+template <typename T, int D>
+__global__ __launch_bounds__(kBlock) void k_downward_transfer(DevTreeView<T> tr,
+                                                              InterpParams<T> ip, int cbeg,
+                                                              int nrhs, int nrc,
+                                                              T *__restrict__ tlhs) {
+  extern __shared__ unsigned char smem_raw[];
+  const int P1 = ip.P1, q = ip.q;
+  const int c = cbeg + blockIdx.x;
+  const int p = tr.parent[c];
+  T *A = reinterpret_cast<T *>(smem_raw);
+  T *B = A + q * nrc;
+  T *Tm = B + q * nrc;
+  for (int e = threadIdx.x; e < D * P1 * P1; e += blockDim.x)
+    Tm[e] = tr.transfer[(int64_t)c * D * P1 * P1 + e];
+  for (int rc = 0; rc < nrhs; rc += nrc) {
+    const int nr = min(nrc, nrhs - rc);
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
+      const int lin = e / nr, r = e - lin * nr;
+      A[e] = tlhs[((int64_t)p * q + lin) * nrhs + rc + r];
+    }
+    __syncthreads();
+    T *cur = A, *nxt = B;
+    int stride = 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      mode_product<T>(cur, nxt, Tm + i * P1 * P1, P1, q, nr, stride, true);
+      __syncthreads();
+      T *tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+      stride *= P1;
+    }
+    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
+      const int lin = e / nr, r = e - lin * nr;
+      tlhs[((int64_t)c * q + lin) * nrhs + rc + r] += cur[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. leaf output: y_t = V_t^T tlhs[t] + sum_{s in near(t)} K(P_t, P_s) xs_s,
+// written to out (natural order through perm, or cluster order).
+// One workgroup per target leaf, thread <-> target point.  LDS holds the
+// thread's Lagrange factors plus one union region used first for tiles of
+// far-field coefficients, then for tiles of source points.
+constexpr int kAlphaTile = 128;
+constexpr int kSrcTile = 64;
+
+template <typename T, int NR>
+__host__ __device__ constexpr size_t leaf_output_union_elems(int D) {
+  return (size_t)kAlphaTile * NR > (size_t)D * kSrcTile + (size_t)kSrcTile * NR
+             ? (size_t)kAlphaTile * NR
+             : (size_t)D * kSrcTile + (size_t)kSrcTile * NR;
+}
+
+template <typename T, int D, int KT, int NR>
+__global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, DevTreeView<T> cols,
+                                                        DevPatternView pat, InterpParams<T> ip,
+                                                        KernelParams<T> kp, int nrhs,
+                                                        const T *__restrict__ xs,
+                                                        const T *__restrict__ tlhs,
+                                                        double *__restrict__ out, int64_t ldo,
+                                                        int natural) {
+  extern __shared__ unsigned char smem_raw[];
+  const int P1 = ip.P1, q = ip.q;
+  T *F = reinterpret_cast<T *>(smem_raw);  // [D][P1][kBlock] Lagrange factors
+  T *U = F + D * P1 * kBlock;              // union region
+  T *Mv = U;                               // [kAlphaTile][NR] far-field coefficients
+  T *Ys = U;                               // [D][kSrcTile] source points
+  T *Xs = U + D * kSrcTile;                // [kSrcTile][NR] source rhs
+  const int li = blockIdx.x;
+  const int c = rows.leaves[li];
+  const int b0 = rows.begin[c], sz = rows.size[c];
+  const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
+  for (int rc = 0; rc < nrhs; rc += NR) {
+    const int nr = min(NR, nrhs - rc);
+    for (int pbase = 0; pbase < sz; pbase += kBlock) {
+      const int i = pbase + threadIdx.x;
+      const bool act = i < sz;
+      T x[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        x[d] = act ? rows.pts[(int64_t)d * rows.npts + b0 + i] : T(0);
+      T acc[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) acc[r] = 0;
+      // far field through the leaf basis (backward_transform_impl.h:30-31)
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          lagrange_1d<T>((x[d] - rows.bbmin[c * D + d]) / rows.bbsize[c * D + d], ip,
+                         F + (d * P1) * kBlock + threadIdx.x, kBlock);
+      }
+      int a[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) a[d] = 0;
+      for (int at = 0; at < q; at += kAlphaTile) {
+        const int na = min(kAlphaTile, q - at);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kAlphaTile * NR; e += blockDim.x) {
+          const int al = e / NR, r = e % NR;
+          Mv[e] = (al < na && r < nr) ? tlhs[((int64_t)c * q + at + al) * nrhs + rc + r] : T(0);
+        }
+        __syncthreads();
+        if (act) {
+          for (int al = 0; al < na; ++al) {
+            T v = F[(0 * P1 + a[0]) * kBlock + threadIdx.x];
+#pragma unroll
+            for (int d = 1; d < D; ++d) v *= F[(d * P1 + a[d]) * kBlock + threadIdx.x];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += v * Mv[al * NR + r];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              if (++a[d] < P1) break;
+              a[d] = 0;
+            }
+          }
+        }
+      }
+      // near field (unsymmetricNystromEvaluator.h:63-73, H2kernelActor.h:68-69)
+      for (int nb = n0; nb < n1; ++nb) {
+        const int s = pat.near_src[nb];
+        const int sb = cols.begin[s], ss = cols.size[s];
+        for (int j0 = 0; j0 < ss; j0 += kSrcTile) {
+          const int nj = min(kSrcTile, ss - j0);
+          __syncthreads();
+          for (int e = threadIdx.x; e < D * kSrcTile; e += blockDim.x) {
+            const int d = e / kSrcTile, jj = e % kSrcTile;
+            Ys[e] = jj < nj ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj] : T(0);
+          }
+          for (int e = threadIdx.x; e < kSrcTile * NR; e += blockDim.x) {
+            const int jj = e / NR, r = e % NR;
+            Xs[e] = (jj < nj && r < nr) ? xs[(int64_t)(sb + j0 + jj) * nrhs + rc + r] : T(0);
+          }
+          __syncthreads();
+          if (act) {
+            for (int jj = 0; jj < nj; ++jj) {
+              T r2 = 0;
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const T dd = x[d] - Ys[d * kSrcTile + jj];
+                r2 += dd * dd;
+              }
+              const T kv = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+              for (int r = 0; r < NR; ++r) acc[r] += kv * Xs[jj * NR + r];
+            }
+          }
+        }
+      }
+      if (act) {
+        const int64_t row = natural ? (int64_t)rows.perm[b0 + i] : (int64_t)(b0 + i);
+        for (int r = 0; r < nr; ++r) out[row + (int64_t)(rc + r) * ldo] = (double)acc[r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int NR>
+size_t output_lds(int D, int P1) {
+  return ((size_t)D * P1 * kBlock + leaf_output_union_elems<T, NR>(D)) * sizeof(T);
+}
+
+template <typename T, int NR>
+size_t coupling_lds(int D, int q) {
+  return ((size_t)D * kMaxP1 + (size_t)q * NR) * sizeof(T);
+}
+
+template <typename T, int D, int KT, int NR>
+hipError_t launch_far_near(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                           const DevPatternView &pat, const InterpParams<T> &ip,
+                           const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf,
+                           double *out, int64_t ldo, bool natural, hipStream_t stream) {
+  const int q = ip.q;
+  const int threads = q <= 64 ? 64 : (q <= 128 ? 128 : kBlock);
+  const size_t lds = coupling_lds<T, NR>(D, q);
+  hipLaunchKernelGGL((k_coupling<T, D, KT, NR>), dim3(rows.ncl), dim3(threads), lds, stream, rows,
+                     cols, pat, ip, kp, nrhs, buf.mom, buf.tlhs);
+  return hipGetLastError();
+}
+
+template <typename T, int D, int KT, int NR>
+hipError_t launch_output(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                         const DevPatternView &pat, const InterpParams<T> &ip,
+                         const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
+                         int64_t ldo, bool natural, hipStream_t stream) {
+  const size_t lds = output_lds<T, NR>(D, ip.P1);
+  hipLaunchKernelGGL((k_leaf_output<T, D, KT, NR>), dim3(rows.nleaves), dim3(kBlock), lds,
+                     stream, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.tlhs, out, ldo,
+                     natural ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <typename T, int D, int NR>
+hipError_t dispatch_kt(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                       const DevPatternView &pat, const InterpParams<T> &ip,
+                       const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
+                       int64_t ldo, bool natural, hipStream_t stream) {
+#define FMCA_KT_CASE(K)                                                                 \
+  case K:                                                                               \
+    return output ? launch_output<T, D, K, NR>(rows, cols, pat, ip, kp, nrhs, buf, out, \
+                                               ldo, natural, stream)                    \
+                  : launch_far_near<T, D, K, NR>(rows, cols, pat, ip, kp, nrhs, buf, out, \
+                                                 ldo, natural, stream);
+  switch (kp.kt) {
+    FMCA_KT_CASE(KT_EXPONENTIAL)
+    FMCA_KT_CASE(KT_MATERN32)
+    FMCA_KT_CASE(KT_MATERN52)
+    FMCA_KT_CASE(KT_GAUSSIAN)
+    default:
+      return output ? launch_output<T, D, -1, NR>(rows, cols, pat, ip, kp, nrhs, buf, out, ldo,
+                                                  natural, stream)
+                    : launch_far_near<T, D, -1, NR>(rows, cols, pat, ip, kp, nrhs, buf, out,
+                                                    ldo, natural, stream);
+  }
+#undef FMCA_KT_CASE
+}
+
+constexpr size_t kLdsLimit = 64 * 1024;
+
+// Largest rhs tile (32, 8 or 1) whose LDS footprint fits one workgroup.
+template <typename T>
+int pick_nr(bool output, int D, int P1, int q, int nrhs) {
+  auto fits = [&](int nr) {
+    const size_t b = output ? (nr == 32   ? output_lds<T, 32>(D, P1)
+                               : nr == 8 ? output_lds<T, 8>(D, P1)
+                                         : output_lds<T, 1>(D, P1))
+                            : (nr == 32   ? coupling_lds<T, 32>(D, q)
+                               : nr == 8 ? coupling_lds<T, 8>(D, q)
+                                         : coupling_lds<T, 1>(D, q));
+    return b <= kLdsLimit;
+  };
+  if (nrhs > 8 && fits(32)) return 32;
+  if (nrhs > 1 && fits(8)) return 8;
+  return 1;
+}
+
+template <typename T, int D>
+hipError_t dispatch_nr(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                       const DevPatternView &pat, const InterpParams<T> &ip,
+                       const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
+                       int64_t ldo, bool natural, hipStream_t stream) {
+  const int nr = pick_nr<T>(output, D, ip.P1, ip.q, nrhs);
+  if (output ? output_lds<T, 1>(D, ip.P1) > kLdsLimit : coupling_lds<T, 1>(D, ip.q) > kLdsLimit)
+    return hipErrorInvalidConfiguration;
+  if (nr == 1)
+    return dispatch_kt<T, D, 1>(output, rows, cols, pat, ip, kp, nrhs, buf, out, ldo, natural,
+                                stream);
+  if (nr == 8)
+    return dispatch_kt<T, D, 8>(output, rows, cols, pat, ip, kp, nrhs, buf, out, ldo, natural,
+                                stream);
+  return dispatch_kt<T, D, 32>(output, rows, cols, pat, ip, kp, nrhs, buf, out, ldo, natural,
+                               stream);
+}
+
+template <typename T, int D>
+hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                    const HostLevels &lrows, const HostLevels &lcols, const DevPatternView &pat,
+                    const InterpParams<T> &ip, const KernelParams<T> &kp, const double *rhs,
+                    int64_t ld_rhs, int nrhs, double *out, int64_t ld_out, bool natural,
+                    ActionBuffers<T> buf, hipStream_t stream) {
+  const int q = ip.q, P1 = ip.P1;
+  hipError_t err;
+  // 1. permute into cluster order
+  {
+    const int64_t total = cols.npts * nrhs;
+    const int grid = (int)std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL((k_permute_in<T>), dim3(grid), dim3(kBlock), 0, stream, rhs, ld_rhs, nrhs,
+                       cols.npts, natural ? cols.perm : nullptr, buf.xs);
+  }
+  // 2. leaf moments
+  {
+    if (nrhs == 1) {
+      const size_t lds = ((size_t)D * P1 * 64 + 64 * 1) * sizeof(T);
+      hipLaunchKernelGGL((k_upward_leaf<T, D, 1>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
+                         cols, ip, buf.xs, nrhs, buf.mom);
+    } else {
+      const size_t lds = ((size_t)D * P1 * 64 + 64 * 8) * sizeof(T);
+      hipLaunchKernelGGL((k_upward_leaf<T, D, 8>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
+                         cols, ip, buf.xs, nrhs, buf.mom);
+    }
+  }
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  // 3. upward transfers, deepest inner level first
+  const size_t budget = 64 * 1024 / sizeof(T) - (size_t)D * P1 * P1;
+  const int nrc_up = (int)std::max<size_t>(1, std::min<size_t>(nrhs, budget / (3 * (size_t)q)));
+  const int nrc_dn = (int)std::max<size_t>(1, std::min<size_t>(nrhs, budget / (2 * (size_t)q)));
+  for (int L = lcols.depth - 1; L >= 0; --L) {
+    const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
+    const size_t lds = ((size_t)3 * q * nrc_up + (size_t)D * P1 * P1) * sizeof(T);
+    hipLaunchKernelGGL((k_upward_transfer<T, D>), dim3(nc), dim3(kBlock), lds, stream, cols, ip,
+                       cb, nrhs, nrc_up, buf.mom);
+  }
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  // 4. coupling (every target cluster is written, empty far lists give zeros)
+  if ((err = dispatch_nr<T, D>(false, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
+                               stream)) != hipSuccess)
+    return err;
+  // 5. downward transfers, top level first
+  for (int L = 1; L <= lrows.depth; ++L) {
+    const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
+    const size_t lds = ((size_t)2 * q * nrc_dn + (size_t)D * P1 * P1) * sizeof(T);
+    hipLaunchKernelGGL((k_downward_transfer<T, D>), dim3(nc), dim3(kBlock), lds, stream, rows, ip,
+                       cb, nrhs, nrc_dn, buf.tlhs);
+  }
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  // 6. leaf output: far field through V^T plus the near field
+  return dispatch_nr<T, D>(true, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
+                           stream);
+}
+
+}  // namespace
+
+}  // namespace fmca_amd
+
+namespace fmca_amd {
+
+template <typename T>
+hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                         const HostLevels &lrows, const HostLevels &lcols,
+                         const DevPatternView &pat, const InterpParams<T> &ip,
+                         const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
+                         double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
+                         hipStream_t stream) {
+  switch (rows.dim) {
+    case 1:
+      return action_d<T, 1>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
+                            ld_out, natural_order, buf, stream);
+    case 2:
+      return action_d<T, 2>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
+                            ld_out, natural_order, buf, stream);
+    case 3:
+      return action_d<T, 3>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
+                            ld_out, natural_order, buf, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+template hipError_t launch_action<double>(const DevTreeView<double> &, const DevTreeView<double> &,
+                                          const HostLevels &, const HostLevels &,
+                                          const DevPatternView &, const InterpParams<double> &,
+                                          const KernelParams<double> &, const double *, int64_t,
+                                          int, double *, int64_t, bool, ActionBuffers<double>,
+                                          hipStream_t);
+template hipError_t launch_action<float>(const DevTreeView<float> &, const DevTreeView<float> &,
+                                         const HostLevels &, const HostLevels &,
+                                         const DevPatternView &, const InterpParams<float> &,
+                                         const KernelParams<float> &, const double *, int64_t, int,
+                                         double *, int64_t, bool, ActionBuffers<float>,
+                                         hipStream_t);
+
+}  // namespace fmca_amd

@@ -1,0 +1,423 @@
+// Host-side planner: cluster tree, block pattern, Chebyshev data, transfers.
+// Level-synchronous and OpenMP-parallel; produces FMCA's tree and blocks.
+#include "plan.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <utility>
+
+namespace fmca_amd {
+
+namespace {
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+constexpr double kPi = 3.14159265358979323846264338327950288;
+const char *const kNames[KT_COUNT] = {
+    "EXPONENTIAL",     "MATERN32",     "MATERN52",     "MATERN72",
+    "MATERN92",        "GAUSSIAN",     "INVMULTIQUADRIC", "MULTIQUADRIC",
+    "BIHARMONIC2D",    "BIHARMONIC3D", "TRIHARMONIC3D",   "TPS2D"};
+}  // namespace
+
+int parse_kernel_type(const char *name) {
+  if (!name) return -1;
+  std::string s(name);
+  for (auto &c : s) c = (char)std::toupper((unsigned char)c);
+  for (int i = 0; i < KT_COUNT; ++i)
+    if (s == kNames[i]) return i;
+  return -1;
+}
+
+const char *kernel_type_name(int kt) {
+  return (kt >= 0 && kt < KT_COUNT) ? kNames[kt] : "UNKNOWN";
+}
+
+// ---------------------------------------------------------------------------
+void Interpolation::init(int d, int p) {
+  dim = d;
+  deg = p;
+  P1 = p + 1;
+  q = 1;
+  for (int i = 0; i < d; ++i) q *= P1;
+  const double weight = 2. * kPi / (2. * double(p) + 2.);
+  xi.resize(P1);
+  w.resize(P1);
+  for (int k = 0; k < P1; ++k) {
+    xi[k] = 0.5 * (std::cos(weight * (double(k) + 0.5)) + 1);
+    w[k] = std::sin(weight * (double(k) + 0.5));
+    if (k & 1) w[k] = -w[k];
+  }
+  // graded-lexicographic order of {0..p}^d: by l1 norm, then lexicographic
+  // with the first entry most significant (MultiIndexSet.h:87-100).
+  std::vector<std::vector<int>> idx;
+  idx.reserve(q);
+  for (int lin = 0; lin < q; ++lin) {
+    std::vector<int> a(d);
+    int r = lin;
+    for (int i = 0; i < d; ++i) {
+      a[i] = r % P1;
+      r /= P1;
+    }
+    idx.push_back(a);
+  }
+  std::vector<int> order(q);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    int sx = 0, sy = 0;
+    for (int v : idx[x]) sx += v;
+    for (int v : idx[y]) sy += v;
+    if (sx != sy) return sx < sy;
+    return idx[x] < idx[y];
+  });
+  fmca_order.assign(order.begin(), order.end());
+}
+
+void Interpolation::lagrange_1d(double u, double *out) const {
+  double s = 0;
+  for (int j = 0; j < P1; ++j) s += w[j] / (u - xi[j]);
+  for (int a = 0; a < P1; ++a) {
+    const double d = u - xi[a];
+    out[a] = std::abs(d) > kEps ? w[a] / d / s : 1.;
+  }
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+struct WorkNode {
+  int64_t begin = 0, size = 0;
+  int32_t parent = -1, level = 0, son0 = -1, son1 = -1;
+  double bb[3 * kMaxDim];  // [min | max | size], each dim entries
+};
+}  // namespace
+
+void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize) {
+  T = Tree();
+  T.dim = dim;
+  T.npts = n;
+  T.perm.resize(n);
+  std::iota(T.perm.begin(), T.perm.end(), 0u);
+  std::vector<WorkNode> nodes(1);
+  WorkNode &root = nodes[0];
+  root.begin = 0;
+  root.size = n;
+  // eps-enlarged root box (ClusterTreeInitializer_ClusterTree.h:44-56)
+  for (int i = 0; i < dim; ++i) {
+    double mn = P[i], mx = P[i];
+#pragma omp parallel for reduction(min : mn) reduction(max : mx) schedule(static)
+    for (int64_t j = 1; j < n; ++j) {
+      const double v = P[i + (size_t)dim * j];
+      mn = v < mn ? v : mn;
+      mx = v > mx ? v : mx;
+    }
+    mn -= kEps * std::abs(mn);
+    mx += kEps * std::abs(mx);
+    root.bb[i] = mn;
+    root.bb[i + dim] = mx;
+    root.bb[i + 2 * dim] = mx - mn;
+  }
+  const int64_t thr = min_csize >= 1 ? (2 * min_csize - 1) : 1;
+  std::vector<int32_t> level_ptr{0};
+  int64_t lb = 0, le = 1;
+  int level = 0;
+  while (lb < le) {
+    const int64_t cnt = le - lb;
+    std::vector<char> split(cnt, 0);
+    std::vector<WorkNode> sons(2 * cnt);
+    // Split every node of this level: sort its index range by the longest
+    // edge of its split box and halve it (ClusterTreeSplitter.h:198-219).
+    // Sorting (key, index) pairs by key performs the same comparisons as
+    // sorting indices through the coordinate, so std::sort yields the same
+    // permutation, ties included.
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t k = 0; k < cnt; ++k) {
+      const WorkNode &nd = nodes[lb + k];
+      if (nd.size <= thr) continue;
+      split[k] = 1;
+      int longest = 0;
+      for (int i = 1; i < dim; ++i)
+        if (nd.bb[i + 2 * dim] > nd.bb[longest + 2 * dim]) longest = i;
+      std::vector<std::pair<double, uint32_t>> kv(nd.size);
+      uint32_t *id = T.perm.data() + nd.begin;
+      for (int64_t j = 0; j < nd.size; ++j) kv[j] = {P[longest + (size_t)dim * id[j]], id[j]};
+      std::sort(kv.begin(), kv.end(),
+                [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+                  return a.first < b.first;
+                });
+      for (int64_t j = 0; j < nd.size; ++j) id[j] = kv[j].second;
+      WorkNode &c1 = sons[2 * k];
+      WorkNode &c2 = sons[2 * k + 1];
+      std::memcpy(c1.bb, nd.bb, sizeof(nd.bb));
+      std::memcpy(c2.bb, nd.bb, sizeof(nd.bb));
+      c1.begin = nd.begin;
+      c1.size = nd.size / 2;
+      c2.begin = nd.begin + c1.size;
+      c2.size = nd.size - c1.size;
+      c1.bb[longest + dim] = kv[c1.size - 1].first;
+      c1.bb[longest + 2 * dim] = c1.bb[longest + dim] - c1.bb[longest];
+      c2.bb[longest] = kv[c1.size].first;
+      c2.bb[longest + 2 * dim] = c2.bb[longest + dim] - c2.bb[longest];
+      c1.level = c2.level = level + 1;
+      c1.parent = c2.parent = (int32_t)(lb + k);
+    }
+    for (int64_t k = 0; k < cnt; ++k)
+      if (split[k]) {
+        nodes[lb + k].son0 = (int32_t)nodes.size();
+        nodes.push_back(sons[2 * k]);
+        nodes[lb + k].son1 = (int32_t)nodes.size();
+        nodes.push_back(sons[2 * k + 1]);
+      }
+    level_ptr.push_back((int32_t)le);
+    lb = le;
+    le = (int64_t)nodes.size();
+    ++level;
+  }
+  const int ncl = (int)nodes.size();
+  T.ncl = ncl;
+  T.depth = (int)level_ptr.size() - 2;
+  T.level_ptr = level_ptr;
+  T.begin.resize(ncl);
+  T.size.resize(ncl);
+  T.level.resize(ncl);
+  T.son0.resize(ncl);
+  T.son1.resize(ncl);
+  T.parent.resize(ncl);
+  T.bbmin.assign((size_t)ncl * dim, 0.);
+  T.bbmax.assign((size_t)ncl * dim, 0.);
+  T.bbsize.assign((size_t)ncl * dim, 0.);
+  for (int c = 0; c < ncl; ++c) {
+    T.begin[c] = nodes[c].begin;
+    T.size[c] = nodes[c].size;
+    T.level[c] = nodes[c].level;
+    T.son0[c] = nodes[c].son0;
+    T.son1[c] = nodes[c].son1;
+    T.parent[c] = nodes[c].parent;
+    if (nodes[c].son0 < 0) T.leaves.push_back(c);
+  }
+  // Shrink to tight boxes, deepest level first (ClusterTreeInitializer_ClusterTree.h:88-132).
+  for (int L = T.depth; L >= 0; --L) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int c = T.level_ptr[L]; c < T.level_ptr[L + 1]; ++c) {
+      double mn[kMaxDim], mx[kMaxDim];
+      if (T.size[c]) {
+        const double *p0 = P + (size_t)dim * T.perm[T.begin[c]];
+        for (int i = 0; i < dim; ++i) mn[i] = mx[i] = p0[i];
+        if (T.son0[c] >= 0) {
+          for (int s : {T.son0[c], T.son1[c]})
+            if (T.size[s])
+              for (int i = 0; i < dim; ++i) {
+                mn[i] = std::min(mn[i], T.bbmin[(size_t)s * dim + i]);
+                mx[i] = std::max(mx[i], T.bbmax[(size_t)s * dim + i]);
+              }
+        } else {
+          for (int64_t j = 0; j < T.size[c]; ++j) {
+            const double *p = P + (size_t)dim * T.perm[T.begin[c] + j];
+            for (int i = 0; i < dim; ++i) {
+              mn[i] = std::min(p[i], mn[i]);
+              mx[i] = std::max(p[i], mx[i]);
+            }
+          }
+        }
+      } else {
+        for (int i = 0; i < dim; ++i) {
+          mn[i] = std::numeric_limits<double>::infinity();
+          mx[i] = -std::numeric_limits<double>::infinity();
+        }
+      }
+      for (int i = 0; i < dim; ++i) {
+        double sz = mx[i] - mn[i];
+        if (mx[i] - mn[i] < 2e2 * kEps) {
+          mx[i] += 1e2 * kEps;
+          mn[i] -= 1e2 * kEps;
+          sz = mx[i] - mn[i];
+        }
+        T.bbmin[(size_t)c * dim + i] = mn[i];
+        T.bbmax[(size_t)c * dim + i] = mx[i];
+        T.bbsize[(size_t)c * dim + i] = sz;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+double radius(const Tree &T, int c) {
+  double s = 0;
+  for (int i = 0; i < T.dim; ++i) {
+    const double v = T.bbsize[(size_t)c * T.dim + i];
+    s += v * v;
+  }
+  return 0.5 * std::sqrt(s);
+}
+
+// CompareCluster::compare (Admissibility.h:95-115) with computeDistance (:19-29).
+// 0 = refine, 1 = low rank, 2 = dense.
+int admissibility(const Tree &R, int t, const Tree &C, int s, double eta) {
+  const int d = R.dim;
+  double s2 = 0;
+  for (int i = 0; i < d; ++i) {
+    const double v = 0.5 * (R.bbmin[(size_t)t * d + i] - C.bbmin[(size_t)s * d + i] +
+                            R.bbmax[(size_t)t * d + i] - C.bbmax[(size_t)s * d + i]);
+    s2 += v * v;
+  }
+  const double rr = radius(R, t), rc = radius(C, s);
+  double dist = std::sqrt(s2) - rr - rc;
+  dist = dist > 0 ? dist : 0;
+  const double rad = rr > rc ? rr : rc;
+  if (rad > eta * dist) return (R.is_leaf(t) || C.is_leaf(s)) ? 2 : 0;
+  return 1;
+}
+}  // namespace
+
+void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int q) {
+  pat = Pattern();
+  // Level-synchronous expansion of the block tree; the set of leaf blocks is
+  // the one of FMCA's depth-first construction (H2MatrixBase.h:160-191).
+  std::vector<std::pair<int32_t, int32_t>> frontier{{0, 0}};
+  const int nth = omp_get_max_threads();
+  while (!frontier.empty()) {
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> next(nth);
+    std::vector<std::vector<int32_t>> brow(nth), bcol(nth);
+    std::vector<std::vector<int8_t>> blr(nth);
+    const int64_t F = (int64_t)frontier.size();
+#pragma omp parallel num_threads(nth)
+    {
+      const int tid = omp_get_thread_num();
+      const int nt = omp_get_num_threads();
+      const int64_t lo = F * tid / nt, hi = F * (tid + 1) / nt;
+      for (int64_t k = lo; k < hi; ++k) {
+        const int t = frontier[k].first, s = frontier[k].second;
+        const int adm = admissibility(R, t, C, s, eta);
+        if (adm == 0) {
+          for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i)
+              next[tid].push_back({i ? R.son1[t] : R.son0[t], j ? C.son1[s] : C.son0[s]});
+        } else {
+          int8_t lr = 0;
+          if (adm == 1) {
+            // FMCA compares 32-bit unsigned products; the wrap is kept.
+            const uint32_t lhs = uint32_t(q) * uint32_t(q);
+            const uint32_t rhs = uint32_t(R.size[t]) * uint32_t(C.size[s]);
+            lr = lhs < rhs;
+          }
+          brow[tid].push_back(t);
+          bcol[tid].push_back(s);
+          blr[tid].push_back(lr);
+        }
+      }
+    }
+    frontier.clear();
+    for (int t = 0; t < nth; ++t) {
+      frontier.insert(frontier.end(), next[t].begin(), next[t].end());
+      pat.block_row.insert(pat.block_row.end(), brow[t].begin(), brow[t].end());
+      pat.block_col.insert(pat.block_col.end(), bcol[t].begin(), bcol[t].end());
+      pat.block_lr.insert(pat.block_lr.end(), blr[t].begin(), blr[t].end());
+    }
+  }
+  const int64_t nb = (int64_t)pat.block_row.size();
+  // far field CSR by target cluster
+  pat.far_ptr.assign(R.ncl + 1, 0);
+  for (int64_t b = 0; b < nb; ++b)
+    if (pat.block_lr[b]) {
+      ++pat.far_ptr[pat.block_row[b] + 1];
+      ++pat.n_far;
+    }
+  for (int c = 0; c < R.ncl; ++c) pat.far_ptr[c + 1] += pat.far_ptr[c];
+  pat.far_src.assign(pat.far_ptr[R.ncl], 0);
+  {
+    std::vector<int32_t> fill(pat.far_ptr.begin(), pat.far_ptr.end() - 1);
+    for (int64_t b = 0; b < nb; ++b)
+      if (pat.block_lr[b]) pat.far_src[fill[pat.block_row[b]]++] = pat.block_col[b];
+  }
+  for (int c = 0; c < R.ncl; ++c) {
+    std::sort(pat.far_src.begin() + pat.far_ptr[c], pat.far_src.begin() + pat.far_ptr[c + 1]);
+    if (pat.far_ptr[c + 1] > pat.far_ptr[c]) pat.far_targets.push_back(c);
+  }
+  pat.far_entries = double(pat.n_far) * q * q;
+  // near field CSR by target leaf; a dense block whose row cluster is not a
+  // leaf is split over the leaves below it (their point ranges partition it).
+  const int nl = (int)R.leaves.size();
+  std::vector<int32_t> leaf_index(R.ncl, -1);
+  for (int k = 0; k < nl; ++k) leaf_index[R.leaves[k]] = k;
+  std::vector<int32_t> by_begin(R.leaves);
+  std::sort(by_begin.begin(), by_begin.end(),
+            [&](int a, int b) { return R.begin[a] < R.begin[b]; });
+  std::vector<int64_t> begins(nl);
+  for (int k = 0; k < nl; ++k) begins[k] = R.begin[by_begin[k]];
+  auto leaves_under = [&](int t, int &lo, int &hi) {
+    lo = int(std::lower_bound(begins.begin(), begins.end(), R.begin[t]) - begins.begin());
+    hi = int(std::lower_bound(begins.begin(), begins.end(), R.begin[t] + R.size[t]) -
+             begins.begin());
+  };
+  pat.near_ptr.assign(nl + 1, 0);
+  for (int64_t b = 0; b < nb; ++b)
+    if (!pat.block_lr[b]) {
+      ++pat.n_near;
+      pat.near_entries += double(R.size[pat.block_row[b]]) * C.size[pat.block_col[b]];
+      int lo, hi;
+      leaves_under(pat.block_row[b], lo, hi);
+      for (int k = lo; k < hi; ++k) ++pat.near_ptr[leaf_index[by_begin[k]] + 1];
+    }
+  for (int k = 0; k < nl; ++k) pat.near_ptr[k + 1] += pat.near_ptr[k];
+  pat.near_src.assign(pat.near_ptr[nl], 0);
+  {
+    std::vector<int32_t> fill(pat.near_ptr.begin(), pat.near_ptr.end() - 1);
+    for (int64_t b = 0; b < nb; ++b)
+      if (!pat.block_lr[b]) {
+        int lo, hi;
+        leaves_under(pat.block_row[b], lo, hi);
+        for (int k = lo; k < hi; ++k)
+          pat.near_src[fill[leaf_index[by_begin[k]]]++] = pat.block_col[b];
+      }
+  }
+  for (int k = 0; k < nl; ++k)
+    std::sort(pat.near_src.begin() + pat.near_ptr[k], pat.near_src.begin() + pat.near_ptr[k + 1],
+              [&](int a, int b) { return C.begin[a] < C.begin[b]; });
+}
+
+// ---------------------------------------------------------------------------
+void compute_transfers(std::vector<double> &T, const Tree &tree, const Interpolation &ip) {
+  const int d = tree.dim, P1 = ip.P1;
+  T.assign((size_t)tree.ncl * d * P1 * P1, 0.);
+#pragma omp parallel for schedule(static)
+  for (int c = 1; c < tree.ncl; ++c) {
+    const int p = tree.parent[c];
+    double vals[kMaxP1];
+    for (int i = 0; i < d; ++i) {
+      const double sc = tree.bbsize[(size_t)c * d + i];
+      const double sp = tree.bbsize[(size_t)p * d + i];
+      const double off = (tree.bbmin[(size_t)c * d + i] - tree.bbmin[(size_t)p * d + i]) / sp;
+      double *Ti = T.data() + ((size_t)c * d + i) * P1 * P1;
+      for (int b = 0; b < P1; ++b) {
+        if (tree.size[c] == 0) continue;  // FMCA leaves E = 0 for empty sons
+        const double x = ip.xi[b] * sc / sp + off;
+        ip.lagrange_1d(x, vals);
+        for (int a = 0; a < P1; ++a) Ti[a * P1 + b] = vals[a];
+      }
+    }
+  }
+}
+
+Statistics statistics(const Pattern &pat, const Tree &R, const Tree &C, int q) {
+  Statistics s{};
+  double mem = 0;
+  for (size_t b = 0; b < pat.block_row.size(); ++b) {
+    if (pat.block_lr[b]) {
+      s.low_rank_blocks += 1;
+      mem += double(q) * q;
+    } else {
+      s.dense_blocks += 1;
+      mem += double(R.size[pat.block_row[b]]) * C.size[pat.block_col[b]];
+    }
+  }
+  s.rows = double(R.npts);
+  s.cols = double(C.npts);
+  s.nz_per_row = std::round(mem / double(R.npts));
+  s.storage_gb = mem * sizeof(double) / 1e9;
+  return s;
+}
+
+}  // namespace fmca_amd

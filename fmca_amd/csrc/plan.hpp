@@ -1,0 +1,102 @@
+// Host-side planner for the MI355X H2-matrix kernel action.
+//
+// Builds FMCA's cluster tree, block pattern and tensor-product Chebyshev data,
+// flattened into structure-of-arrays that are uploaded to HBM once.  Cluster
+// ids are FMCA's level-order block ids.  Nothing here touches the GPU, so the
+// planner can be exercised on a host without one.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fmca_amd {
+
+// CovarianceKernel types (FMCA/src/CovarianceKernel/CovarianceKernel.h:56-116).
+enum KernelType : int {
+  KT_EXPONENTIAL = 0,
+  KT_MATERN32 = 1,
+  KT_MATERN52 = 2,
+  KT_MATERN72 = 3,
+  KT_MATERN92 = 4,
+  KT_GAUSSIAN = 5,
+  KT_INVMULTIQUADRIC = 6,
+  KT_MULTIQUADRIC = 7,
+  KT_BIHARMONIC2D = 8,
+  KT_BIHARMONIC3D = 9,
+  KT_TRIHARMONIC3D = 10,
+  KT_TPS2D = 11,
+  KT_COUNT = 12,
+};
+
+// Case-insensitive name -> KernelType, -1 if FMCA does not define it.
+int parse_kernel_type(const char *name);
+const char *kernel_type_name(int kt);
+
+constexpr int kMaxDim = 3;
+constexpr int kMaxP1 = 10;  // polynomial degree <= 9
+
+// Tensor-product Chebyshev interpolation on [0,1]^d
+// (FMCA/src/Interpolator/TensorProductInterpolator.h:39-81).  Moments are held
+// in tensor order lin = a_0 + P1*a_1 + P1^2*a_2; fmca_order maps FMCA's
+// graded-lexicographic position k (MultiIndexSet.h:87-111) to lin.
+struct Interpolation {
+  int dim = 0, deg = 0, P1 = 0, q = 0;
+  std::vector<double> xi, w;
+  std::vector<int32_t> fmca_order;
+  void init(int dim, int deg);
+  // l_a(u), a = 0..deg, with FMCA's rule that a factor within eps of its node
+  // is 1 (TensorProductInterpolator.h:71-78).
+  void lagrange_1d(double u, double *out) const;
+};
+
+// A cluster tree in FMCA's level order.  Clusters of one level are contiguous.
+struct Tree {
+  int dim = 0;
+  int64_t npts = 0;
+  int ncl = 0;
+  int depth = 0;                     // deepest level
+  std::vector<uint32_t> perm;        // cluster order position -> point index
+  std::vector<int64_t> begin, size;  // per cluster
+  std::vector<int32_t> level, son0, son1, parent;
+  std::vector<int32_t> level_ptr;    // clusters of level L: [level_ptr[L], level_ptr[L+1])
+  std::vector<double> bbmin, bbmax, bbsize;  // ncl x dim
+  std::vector<int32_t> leaves;       // leaf cluster ids, ascending
+  bool is_leaf(int c) const { return son0[c] < 0; }
+};
+
+// ClusterTree with CardinalityBisection and shrink-to-fit boxes
+// (FMCA/src/Clustering/ClusterTreeInitializer_ClusterTree.h:24-132,
+//  FMCA/src/Clustering/ClusterTreeSplitter.h:198-219).
+void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize);
+
+// FMCA's leaf blocks (H2MatrixBase::computePattern, H2MatrixBase.h:160-191),
+// regrouped by target for a gather-style GPU schedule.
+struct Pattern {
+  // every leaf block of FMCA's block tree (row id, column id, low-rank flag)
+  std::vector<int32_t> block_row, block_col;
+  std::vector<int8_t> block_lr;
+  // far field: CSR over target clusters -> source clusters
+  std::vector<int32_t> far_ptr, far_src;
+  std::vector<int32_t> far_targets;  // target clusters with a non-empty far list
+  // near field: CSR over target leaves (index into Tree::leaves) -> source clusters
+  std::vector<int32_t> near_ptr, near_src;
+  double near_entries = 0, far_entries = 0;
+  int64_t n_far = 0, n_near = 0;
+};
+
+void compute_pattern(Pattern &pat, const Tree &rows, const Tree &cols, double eta, int q);
+
+// Transfer factors: E_c = T_c,0 (x) T_c,1 (x) T_c,2 for every non-root cluster c
+// (FMCA/src/H2Matrix/compute_cluster_bases_impl.h:36-51), stored as
+// T[((c*dim + i)*P1 + a)*P1 + b] = l_a(x_i(b)) with x_i(b) the son's node b in
+// the father's reference frame.
+void compute_transfers(std::vector<double> &T, const Tree &tree, const Interpolation &ip);
+
+struct Statistics {
+  double rows, cols, low_rank_blocks, dense_blocks, nz_per_row, storage_gb;
+};
+// H2MatrixBase::statistics (H2MatrixBase.h:70-105)
+Statistics statistics(const Pattern &pat, const Tree &rows, const Tree &cols, int q);
+
+}  // namespace fmca_amd

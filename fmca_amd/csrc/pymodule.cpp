@@ -1,0 +1,181 @@
+// pybind11 module over the C ABI (the Python face of py/FMCA.cpp:209-250,462-468).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "fmca_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(fmca_status s) {
+  if (s == FMCA_OK) return;
+  const std::string msg = std::string(fmca_status_string(s)) + ": " + fmca_last_error();
+  if (s == FMCA_ERR_INVALID_ARGUMENT || s == FMCA_ERR_UNKNOWN_KERNEL || s == FMCA_ERR_NONFINITE)
+    throw py::value_error(msg);
+  if (s == FMCA_ERR_NOT_SUPPORTED) throw py::value_error(msg);
+  throw std::runtime_error(msg);
+}
+
+using farr = py::array_t<double, py::array::f_style | py::array::forcecast>;
+
+// float64 only, as FMCA's noconvert() arguments (py/FMCA.cpp:467)
+farr as_f64(const py::array &a, const char *what) {
+  if (!a.dtype().is(py::dtype::of<double>()))
+    throw py::type_error(std::string(what) + " must be a float64 array");
+  return farr::ensure(a);
+}
+
+struct Actor {
+  fmca_h2_actor *h = nullptr;
+  int dim = 0;
+  int64_t m = 0, n = 0;
+  Actor() = default;
+  Actor(const Actor &) = delete;
+  ~Actor() {
+    if (h) fmca_h2_destroy(h);
+  }
+};
+
+std::unique_ptr<Actor> create(const std::string &ktype, double l, double c, double nu,
+                              const py::array &Peval_in, const py::array &P_in, int deg,
+                              double eta) {
+  farr Peval = as_f64(Peval_in, "Peval"), P = as_f64(P_in, "P");
+  if (Peval.ndim() != 2 || P.ndim() != 2 || Peval.shape(0) != P.shape(0))
+    throw py::value_error("points must be (dim, n) arrays of equal dim");
+  auto a = std::make_unique<Actor>();
+  a->dim = (int)P.shape(0);
+  a->m = Peval.shape(1);
+  a->n = P.shape(1);
+  fmca_kernel_desc k{ktype.c_str(), l, c, nu};
+  {
+    py::gil_scoped_release nogil;
+    check(fmca_h2_create(&k, a->dim, a->m, Peval.data(), a->n, P.data(), deg, eta, &a->h));
+  }
+  return a;
+}
+
+py::array action(Actor &a, const py::array &rhs_in, int precision, bool cluster_order) {
+  farr rhs = as_f64(rhs_in, "rhs");
+  if (rhs.ndim() < 1 || rhs.ndim() > 2 || rhs.shape(0) != a.n)
+    throw py::value_error("rhs must have n rows");
+  const int64_t nrhs = rhs.ndim() == 2 ? rhs.shape(1) : 1;
+  farr out({(py::ssize_t)a.m, (py::ssize_t)nrhs});
+  double *po = out.mutable_data();
+  const double *pr = rhs.data();
+  {
+    py::gil_scoped_release nogil;
+    if (cluster_order)
+      check(fmca_h2_action_cluster_order(a.h, pr, a.n, nrhs, po, a.m, precision));
+    else
+      check(fmca_h2_action(a.h, pr, a.n, nrhs, po, a.m, precision));
+  }
+  return out;
+}
+
+py::dict info(const Actor &a) {
+  double v[13];
+  check(fmca_h2_info(a.h, v));
+  py::dict d;
+  const char *keys[13] = {"dim",        "m",          "n",           "q",
+                          "row_clusters", "col_clusters", "far_blocks", "near_blocks",
+                          "near_entries", "far_entries",  "row_depth",  "col_depth",
+                          "shared_tree"};
+  for (int k = 0; k < 13; ++k) d[keys[k]] = v[k];
+  return d;
+}
+
+py::dict clusters(const Actor &a, int which) {
+  double v[13];
+  check(fmca_h2_info(a.h, v));
+  const py::ssize_t nc = (py::ssize_t)(which == FMCA_ROWS ? v[4] : v[5]);
+  py::array_t<int64_t> begin(nc), size(nc);
+  py::array_t<int32_t> level(nc), sons({nc, (py::ssize_t)2});
+  py::array_t<double> bb({nc, (py::ssize_t)3, (py::ssize_t)a.dim});
+  check(fmca_h2_clusters(a.h, which, begin.mutable_data(), size.mutable_data(),
+                         level.mutable_data(), sons.mutable_data(), bb.mutable_data()));
+  py::dict d;
+  d["begin"] = begin;
+  d["size"] = size;
+  d["level"] = level;
+  d["sons"] = sons;
+  d["bb"] = bb;
+  return d;
+}
+
+py::tuple blocks(const Actor &a) {
+  double v[13];
+  check(fmca_h2_info(a.h, v));
+  const py::ssize_t nb = (py::ssize_t)(v[6] + v[7]);
+  py::array_t<int32_t> row(nb), col(nb);
+  py::array_t<int8_t> lr(nb);
+  check(fmca_h2_blocks(a.h, row.mutable_data(), col.mutable_data(), lr.mutable_data()));
+  return py::make_tuple(row, col, lr);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_fmca_amd_py, m) {
+  m.doc() = "MI355X-native H2-matrix kernel action (C ABI bindings)";
+  m.attr("FP64") = (int)FMCA_PRECISION_FP64;
+  m.attr("FP32") = (int)FMCA_PRECISION_FP32;
+  m.attr("ROWS") = (int)FMCA_ROWS;
+  m.attr("COLS") = (int)FMCA_COLS;
+  py::class_<Actor>(m, "Actor")
+      .def(py::init(&create), py::arg("ktype"), py::arg("l"), py::arg("c"), py::arg("nu"),
+           py::arg("Peval"), py::arg("P"), py::arg("deg"), py::arg("eta"))
+      .def_readonly("dim", &Actor::dim)
+      .def_readonly("m", &Actor::m)
+      .def_readonly("n", &Actor::n)
+      .def("upload",
+           [](Actor &a) {
+             py::gil_scoped_release nogil;
+             check(fmca_h2_upload(a.h));
+           })
+      .def("action", &action, py::arg("rhs"), py::arg("precision") = 0,
+           py::arg("cluster_order") = false)
+      .def(
+          "action_device",
+          [](Actor &a, uintptr_t rhs, int64_t ld_rhs, int64_t nrhs, uintptr_t out, int64_t ld_out,
+             int precision, bool cluster_order, uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            check(fmca_h2_action_device(a.h, reinterpret_cast<const double *>(rhs), ld_rhs, nrhs,
+                                        reinterpret_cast<double *>(out), ld_out, precision,
+                                        cluster_order ? 1 : 0, reinterpret_cast<void *>(stream)));
+          },
+          py::arg("rhs"), py::arg("ld_rhs"), py::arg("nrhs"), py::arg("out"), py::arg("ld_out"),
+          py::arg("precision") = 0, py::arg("cluster_order") = false, py::arg("stream") = 0)
+      .def(
+          "action_transpose_device",
+          [](Actor &a, uintptr_t rhs, int64_t ld_rhs, int64_t nrhs, uintptr_t out, int64_t ld_out,
+             int precision, uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            check(fmca_h2_action_transpose_device(
+                a.h, reinterpret_cast<const double *>(rhs), ld_rhs, nrhs,
+                reinterpret_cast<double *>(out), ld_out, precision,
+                reinterpret_cast<void *>(stream)));
+          },
+          py::arg("rhs"), py::arg("ld_rhs"), py::arg("nrhs"), py::arg("out"), py::arg("ld_out"),
+          py::arg("precision") = 0, py::arg("stream") = 0)
+      .def("indices",
+           [](const Actor &a, int which) {
+             py::array_t<uint32_t> out(which == FMCA_ROWS ? a.m : a.n);
+             check(fmca_h2_indices(a.h, which, out.mutable_data()));
+             return out;
+           })
+      .def("statistics",
+           [](const Actor &a) {
+             py::array_t<double> out({(py::ssize_t)6, (py::ssize_t)1});
+             check(fmca_h2_statistics(a.h, out.mutable_data()));
+             return out;
+           })
+      .def("info", &info)
+      .def("clusters", &clusters)
+      .def("blocks", &blocks);
+  m.def("status_string", [](int s) { return std::string(fmca_status_string((fmca_status)s)); });
+}

@@ -1,0 +1,134 @@
+/*
+ * fmca_amd.h -- C ABI of the MI355X-native H2-matrix kernel action.
+ *
+ * Drop-in for FMCA's matrix-free H2 action (muchip/fmca).  Plain pointers and
+ * sizes only; every matrix is column-major FP64 as an Eigen::MatrixXd is.
+ * Points are dim x n (one point per column), right-hand sides n x nrhs.
+ *
+ * Host pointers are borrowed for the duration of a call only (FMCA's actor
+ * keeps references to P/Peval, FMCA/src/Moments/NystromMoments.h:41; this
+ * library copies what it needs).  Device memory is owned by the actor handle.
+ * All functions return a status code instead of FMCA's assert()s.
+ */
+#ifndef FMCA_AMD_H_
+#define FMCA_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fmca_h2_actor fmca_h2_actor; /* opaque */
+
+typedef enum {
+  FMCA_OK = 0,
+  FMCA_ERR_INVALID_ARGUMENT = 1,
+  FMCA_ERR_UNKNOWN_KERNEL = 2, /* FMCA: assert "desired kernel not implemented" */
+  FMCA_ERR_NOT_SUPPORTED = 3,  /* dim > 3, degree > 9, ... */
+  FMCA_ERR_NO_DEVICE = 4,
+  FMCA_ERR_OUT_OF_MEMORY = 5,
+  FMCA_ERR_DEVICE = 6,
+  FMCA_ERR_NOT_CONVERGED = 7,
+  FMCA_ERR_NONFINITE = 8
+} fmca_status;
+
+/* Arithmetic of the coupling and near-field kernels. */
+enum {
+  FMCA_PRECISION_FP64 = 0, /* plain form: every phase in FP64 */
+  FMCA_PRECISION_FP32 = 1  /* precision form: FP32 kernel entries and contractions */
+};
+
+enum { FMCA_ROWS = 0, FMCA_COLS = 1 }; /* row tree = Peval (targets), column tree = P */
+
+/* Kernel description: FMCA::CovarianceKernel(ktype, l, c, nu)
+ * (FMCA/src/CovarianceKernel/CovarianceKernel.h:50-52).  ktype is one of
+ * EXPONENTIAL, MATERN32, MATERN52, MATERN72, MATERN92, GAUSSIAN,
+ * INVMULTIQUADRIC, MULTIQUADRIC, BIHARMONIC2D, BIHARMONIC3D, TRIHARMONIC3D,
+ * TPS2D (case-insensitive). */
+typedef struct {
+  const char *ktype;
+  double l, c, nu;
+} fmca_kernel_desc;
+
+/* Replaces H2kernelActor(kernel, Peval, P, mpole_deg, eta)
+ * (FMCA/src/H2Matrix/H2kernelActor.h:21-36): tensor-product Chebyshev
+ * moments of degree `deg`, ClusterTree with CardinalityBisection, pattern
+ * with CompareCluster at `eta`.  Peval: dim x m (targets), P: dim x n
+ * (sources).  The plan is built on the host; device buffers are created on
+ * the first action (or fmca_h2_upload), on the current HIP device. */
+fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
+                           const double *Peval, int64_t n, const double *P, int deg,
+                           double eta, fmca_h2_actor **actor);
+
+/* Destroys the actor and frees its device memory. */
+fmca_status fmca_h2_destroy(fmca_h2_actor *actor);
+
+/* Uploads the plan to the current device now instead of on the first action. */
+fmca_status fmca_h2_upload(fmca_h2_actor *actor);
+
+/* Replaces Matrix H2kernelActor::action(const Matrix &rhs) const
+ * (H2kernelActor.h:38-80): out = K(Peval, P) * rhs in natural order.
+ * rhs: n x nrhs (leading dimension ld_rhs >= n), out: m x nrhs (ld_out >= m),
+ * both host memory.  Synchronous. */
+fmca_status fmca_h2_action(fmca_h2_actor *actor, const double *rhs, int64_t ld_rhs,
+                           int64_t nrhs, double *out, int64_t ld_out, int precision);
+
+/* Replaces H2MatrixBase::action(mat_eval, rhs) (FMCA/src/H2Matrix/H2MatrixBase.h:113-152)
+ * as called by the Python binding (py/FMCA.cpp:230-240): rhs in column-tree
+ * order, result in row-tree order.  Host memory, synchronous. */
+fmca_status fmca_h2_action_cluster_order(fmca_h2_actor *actor, const double *rhs,
+                                         int64_t ld_rhs, int64_t nrhs, double *out,
+                                         int64_t ld_out, int precision);
+
+/* Device-memory form of the two calls above: d_rhs / d_out are device
+ * pointers, the work is enqueued on `stream` (a hipStream_t, NULL = default
+ * stream) and the call returns without synchronising.  cluster_order != 0
+ * selects the H2MatrixBase::action ordering. */
+fmca_status fmca_h2_action_device(fmca_h2_actor *actor, const double *d_rhs, int64_t ld_rhs,
+                                  int64_t nrhs, double *d_out, int64_t ld_out, int precision,
+                                  int cluster_order, void *stream);
+
+/* Transposed action out = K(Peval, P)^T * rhs: rhs m x nrhs, out n x nrhs,
+ * device memory (the adjoint needed by the differentiation form; FMCA has no
+ * counterpart). */
+fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *d_rhs,
+                                            int64_t ld_rhs, int64_t nrhs, double *d_out,
+                                            int64_t ld_out, int precision, void *stream);
+
+/* Permutation of a tree (cluster order position -> point index), as
+ * pyH2Matrix::rindices / cindices (py/FMCA.cpp:224-229) and
+ * ClusterTreeBase::indices (FMCA/src/Clustering/ClusterTreeBase.h:63-65). */
+fmca_status fmca_h2_indices(const fmca_h2_actor *actor, int which, uint32_t *out);
+
+/* H2MatrixBase::statistics (H2MatrixBase.h:70-105): rows, cols, low-rank
+ * blocks, dense blocks, nonzeros per row, storage in GB. */
+fmca_status fmca_h2_statistics(const fmca_h2_actor *actor, double out[6]);
+
+/* Sizes: [0] dim, [1] m, [2] n, [3] q, [4] row clusters, [5] column clusters,
+ * [6] far blocks, [7] near blocks, [8] near kernel entries, [9] far kernel
+ * entries, [10] depth of the row tree, [11] depth of the column tree,
+ * [12] 1 if both trees are the same object. */
+fmca_status fmca_h2_info(const fmca_h2_actor *actor, double out[13]);
+
+/* Clusters of a tree in FMCA level order (block ids): begin, size, level,
+ * sons (2 per cluster, -1 for leaves), bounding boxes (3 x dim per cluster:
+ * min, max, size).  Arrays sized by fmca_h2_info. */
+fmca_status fmca_h2_clusters(const fmca_h2_actor *actor, int which, int64_t *begin,
+                             int64_t *size, int32_t *level, int32_t *sons, double *bb);
+
+/* FMCA's leaf blocks: row / column block ids and low-rank flag; count = far + near. */
+fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *col,
+                           int8_t *low_rank);
+
+/* Static string for a status code. */
+const char *fmca_status_string(fmca_status status);
+
+/* Last error message of the calling thread ("" if none). */
+const char *fmca_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMCA_AMD_H_ */

@@ -1,0 +1,83 @@
+"""GPU parity of the H2 action against the FP64 CPU oracle (FMCA restatement).
+
+Bar: FP64 GPU action within 1e-12 relative of the oracle's action on the same
+tree (identical blocks; only summation order differs).  The FP32 precision
+form is checked against the FP64 GPU action with a tolerance stated per test.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err, uniform_points
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+# (label, n, m or None (= same points), dim, deg, kernel, l, eta, nrhs)
+CASES = [
+    ("cfg1-exp-3d", 10000, None, 3, 3, "EXPONENTIAL", 0.5, 0.8, 1),
+    ("matern32-3d-32rhs", 5000, None, 3, 4, "MATERN32", 0.5, 0.8, 32),
+    ("matern32-2d-deg5", 8000, None, 2, 5, "MATERN32", 0.3, 0.8, 8),
+    ("gauss-rect", 3000, 5000, 3, 3, "GAUSSIAN", 0.2, 0.8, 3),
+    ("matern52-1d", 4000, None, 1, 6, "MATERN52", 0.4, 0.5, 2),
+    ("invmq-2d", 3000, 2000, 2, 4, "INVMULTIQUADRIC", 0.7, 0.6, 5),
+    ("matern72-3d", 3000, None, 3, 2, "MATERN72", 0.6, 0.8, 1),
+    ("matern92-eta04", 3000, None, 2, 3, "MATERN92", 0.6, 0.4, 1),
+    ("small-all-dense", 100, None, 3, 4, "EXPONENTIAL", 1.0, 0.8, 2),
+    ("rhs-40", 3000, None, 3, 3, "MATERN32", 0.5, 0.8, 40),
+]
+
+
+def _setup(case):
+    import fmca_amd
+    label, n, m, d, deg, ker, l, eta, nrhs = case
+    print(f"[case] {label}", flush=True)
+    P = uniform_points(n, d, seed=11)
+    Peval = P if m is None else uniform_points(m, d, seed=12)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel(ker, l), Peval, P, deg, eta)
+    A = oracle.OracleH2(P, Peval, ker, l, deg, eta)
+    x = np.asfortranarray(np.random.default_rng(13).standard_normal((n, nrhs)))
+    return H, A, x, P, Peval
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp64_action_matches_oracle(case):
+    H, A, x, _, _ = _setup(case)
+    y = H.action(x)
+    yo = A.action(x)
+    err = rel_err(y, yo)
+    print(f"  fp64 vs oracle rel err {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+
+
+@pytest.mark.parametrize("case", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_fp64_cluster_order_matches_oracle(case):
+    H, A, x, _, _ = _setup(case)
+    y = H.action_cluster_order(x)
+    yo = A.action_cluster_order(x)
+    assert rel_err(y, yo) <= FP64_TOL
+
+
+@pytest.mark.parametrize("case", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_fp32_action_close_to_fp64(case):
+    import fmca_amd
+    H, A, x, _, _ = _setup(case)
+    y64 = H.action(x)
+    y32 = H.action(x, precision=fmca_amd.FP32)
+    err = rel_err(y32, y64)
+    print(f"  fp32 vs fp64 rel err {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_fp64_action_vs_dense_cfg1():
+    """Config 1: H2 error against dense FP64 K*x (states the approximation error)."""
+    import fmca_amd
+    P = uniform_points(10000, 3, seed=1)
+    x = np.random.default_rng(2).uniform(-1, 1, (10000, 1))
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("EXPONENTIAL", 0.5), P, P, 3, 0.8)
+    y = H.action(x)
+    yd = oracle.dense_action(P, P, "EXPONENTIAL", 0.5, x)
+    err = rel_err(y, yd)
+    print(f"  H2 vs dense rel err {err:.3e}", flush=True)
+    assert err < 1e-3
