@@ -1,0 +1,215 @@
+"""Benchmark of the H2-matrix kernel action on one MI355X.
+
+Workload (the configuration the headline metric is quoted on):
+  n = 1,000,000 points uniform in [0,1]^3 (numpy default_rng(0)), Matern-3/2
+  kernel with lengthscale 0.5, tensor Chebyshev degree 4, eta = 0.8,
+  32 right-hand sides with i.i.d. N(0,1) entries.
+
+One step = one matrix-free H2 matvec y = K x of the precision form (FP32 kernel
+entries and contractions, FP64 in/out) on the 32 resident right-hand sides.
+The plain FP64 action is timed on the same inputs for the ratio, and the
+relative difference of the two results is reported.
+
+Prints ONE JSON line (rank 0).  Usage:
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H2 matvec Mpoint*RHS/s, n=1e6 3D, FP32+refine vs FP64 at 1e-8 rel err"
+PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 (vector and matrix)
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--n", type=int, default=1_000_000)
+    p.add_argument("--nrhs", type=int, default=32)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="target CPU time of the sampled coupling/near-field loop")
+    return p.parse_args()
+
+
+def timed_steps(H, x, out, precision, steps, warmup, profile):
+    import torch
+    for _ in range(warmup):
+        H.action_device(x, out=out, precision=precision)
+    torch.cuda.synchronize()
+    if profile:
+        H.phase_times(reset=True)
+        H.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        H.action_device(x, out=out, precision=precision)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = None
+    if profile:
+        H.set_profiling(False)
+        phases = H.phase_times(reset=True)
+    return (t1 - t0) / steps, phases
+
+
+def traffic_from_profiles(kernel_key):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    calibrated against a copy kernel; read from the committed summary if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(P, x, info, budget_seconds):
+    """FMCA's OpenMP CPU H2 action as restated by the FP64 oracle, timed on a
+    bounded sample of the column groups and scaled by kernel entries."""
+    import oracle
+    nthreads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    A = oracle.OracleH2(P, P, "MATERN32", 0.5, 4, 0.8)
+    total = info["near_entries"] + info["far_entries"]
+    # calibration pass, then a pass sized to the time budget
+    done, times = A.action_sample(x, 2e8, nthreads)
+    rate = done / max(times[1], 1e-6)
+    budget = min(total, max(2e8, rate * budget_seconds))
+    done, times = A.action_sample(x, budget, nthreads)
+    t_full = times[0] + times[2] + times[1] * total / done
+    nrhs = x.shape[1]
+    return {
+        "value": P.shape[1] * nrhs / t_full / 1e6,
+        "unit": "Mpoint*RHS/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": (f"oracle (FP64 C++/OpenMP restatement of FMCA's action, same tree/blocks) "
+                   f"on the same n={P.shape[1]} points and {nrhs} rhs: forward pass "
+                   f"{times[0]:.2f}s and backward pass {times[2]:.2f}s timed in full, the "
+                   f"coupling/near-field loop timed on {done / total * 100:.1f}% of the "
+                   f"{total:.3e} kernel entries ({times[1]:.1f}s) and scaled by entries; "
+                   f"setup {A.setup_seconds:.1f}s not counted"),
+        "seconds_per_matvec_extrapolated": t_full,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              f"the action is not sharded across GPUs.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fmca_amd
+
+    n, nrhs = args.n, args.nrhs
+    rng = np.random.default_rng(0)
+    P = np.asfortranarray(rng.random((3, n)))
+    K = fmca_amd.CovarianceKernel("MATERN32", 0.5)
+    t0 = time.perf_counter()
+    H = fmca_amd.H2kernelActor(K, P, P, 4, 0.8)
+    setup_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    H.upload()
+    torch.cuda.synchronize()
+    setup_upload = time.perf_counter() - t0
+    info = H.info()
+    xh = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    x = torch.from_numpy(xh).cuda()  # column-major (F-ordered) device copy
+    x = x.t().contiguous().t() if x.stride(0) != 1 else x
+    out32 = torch.empty((nrhs, n), dtype=torch.float64, device="cuda").t()
+    out64 = torch.empty((nrhs, n), dtype=torch.float64, device="cuda").t()
+
+    t32, ph32 = timed_steps(H, x, out32, fmca_amd.FP32, args.steps, args.warmup, True)
+    t64, ph64 = timed_steps(H, x, out64, fmca_amd.FP64, max(1, args.steps // 2),
+                            min(1, args.warmup), True)
+    err = float(torch.linalg.norm(out32 - out64) / torch.linalg.norm(out64))
+    # single right-hand side
+    x1 = x[:, :1].contiguous()
+    o1 = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    t32_1, _ = timed_steps(H, x1, o1, fmca_amd.FP32, args.steps, 1, False)
+    t64_1, _ = timed_steps(H, x1, o1, fmca_amd.FP64, max(1, args.steps // 2), 1, False)
+
+    E_near, E_far, q = info["near_entries"], info["far_entries"], int(info["q"])
+    value = n * nrhs / t32 / 1e6
+
+    # dominant kernel of the timed step: the larger of leaf_output / coupling
+    def avg(ph, k):
+        return ph[k][0] / max(ph[k][1], 1)
+    dom = max(("leaf_output", "coupling"), key=lambda k: avg(ph32, k))
+    t_dom = avg(ph32, dom) / 1e3
+    if dom == "leaf_output":
+        flops = 2.0 * E_near * nrhs + 2.0 * q * n * nrhs
+        entries = E_near
+    else:
+        flops = 2.0 * E_far * nrhs
+        entries = E_far
+    achieved = flops / t_dom / 1e12
+    roofline = {
+        "bound": "mfma",
+        "kernel": "k_leaf_output<float,3,MATERN32,32>" if dom == "leaf_output"
+        else "k_coupling<float,3,MATERN32,32>",
+        "achieved": achieved,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": traffic_from_profiles(dom),
+        "avg_launch_ms": t_dom * 1e3,
+        "algorithmic_flops_per_launch": flops,
+        "kernel_entries_per_launch": entries,
+        "kernel_entries_per_s": entries / t_dom,
+    }
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Mpoint*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: uniform points in [0,1]^3 and N(0,1) rhs, numpy default_rng(0)",
+        "config": {
+            "workload": (f"H2 matvec, n={n} uniform 3D points, MATERN32 l=0.5, tensor "
+                         f"Chebyshev deg 4 (q={q}), eta=0.8, {nrhs} rhs, FP32 precision form"),
+            "n": n, "nrhs": nrhs, "kernel": "MATERN32", "lengthscale": 0.5, "degree": 4,
+            "eta": 0.8, "far_blocks": int(info["far_blocks"]),
+            "near_blocks": int(info["near_blocks"]), "near_entries": E_near,
+            "far_entries": E_far,
+        },
+        "roofline": roofline,
+        "fp64": {
+            "value": n * nrhs / t64 / 1e6, "ms_per_step": t64 * 1e3,
+            "fp32_speedup": t64 / t32,
+            "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph64.items()},
+        },
+        "fp32_vs_fp64_rel_err": err,
+        "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph32.items()},
+        "rhs1": {"fp32_mpts": n / t32_1 / 1e6, "fp64_mpts": n / t64_1 / 1e6,
+                 "fp32_ms": t32_1 * 1e3, "fp64_ms": t64_1 * 1e3},
+        "setup_s": {"plan": setup_plan, "upload": setup_upload},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(P, xh, info, args.cpu_seconds)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -19,6 +19,14 @@ from __future__ import annotations
 
 import numpy as np
 
+# The HIP runtime of this process must be the one torch brings when torch is
+# installed: importing torch first makes the dynamic loader hand the same
+# libamdhip64 to the extension (a second runtime would hide the GPU from torch).
+try:  # pragma: no cover - depends on the installation
+    import torch as _torch  # noqa: F401
+except ImportError:
+    _torch = None
+
 try:
     from . import _fmca_amd_py as _C
 except ImportError as e:  # pragma: no cover - exercised only on a broken install
@@ -147,6 +155,14 @@ class H2kernelActor:
 
     def info(self) -> dict:
         return self._a.info()
+
+    def set_profiling(self, enable: bool = True):
+        """Record HIP events around the six phases of every action."""
+        self._a.set_profiling(bool(enable))
+
+    def phase_times(self, reset: bool = False) -> dict:
+        """{phase: (total ms, number of actions)} of the profiled actions."""
+        return self._a.phase_times(reset)
 
     def clusters(self, which: int) -> dict:
         d = self._a.clusters(which)

@@ -26,12 +26,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # hipcc flags: gfx950 only; packed FP32 instructions are disabled
 # (-packed-fp32-ops) so that float code is issued as plain VALU ops.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fopenmp",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
+    "-mcode-object-version=5", "-pthread",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 SOURCES_HIP = ["kernels.hip"]
@@ -65,8 +66,8 @@ def _objects(verbose):
 
 def build_library(verbose=False):
     objs = _objects(verbose)
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fopenmp", "-o", LIB, *objs,
-          "-Wl,-rpath,/opt/rocm/lib:/opt/rocm/llvm/lib"], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-pthread", "-o", LIB, *objs,
+          "-Wl,-rpath,/opt/rocm/lib"], verbose)
     return LIB
 
 

@@ -137,11 +137,18 @@ struct fmca_h2_actor {
   DevBuf stage_in, stage_out;
   hipStream_t stream = nullptr;
   std::mutex mtx;
+  // per-phase profiling (events recorded on the launch stream)
+  bool profiling = false;
+  std::vector<PhaseEvents> pending;
+  double phase_ms[PH_COUNT] = {0};
+  int64_t phase_count[PH_COUNT] = {0};
 
   const Tree &rows() const { return rows_tree; }
   const Tree &cols() const { return shared ? rows_tree : cols_tree; }
 
   ~fmca_h2_actor() {
+    for (auto &pe : pending)
+      for (auto &e : pe.ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -248,6 +255,27 @@ KernelParams<T> kernel_params(const fmca_h2_actor *a) {
   k.l = (T)a->l;
   k.c = (T)a->c;
   k.nu = (T)a->nu;
+  double sc = 1.0 / a->l;
+  switch (a->kt) {
+    case KT_MATERN32:
+      sc = std::sqrt(3.0) / a->l;
+      break;
+    case KT_MATERN52:
+      sc = std::sqrt(5.0) / a->l;
+      break;
+    case KT_MATERN72:
+      sc = std::sqrt(7.0) / a->l;
+      break;
+    case KT_MATERN92:
+      sc = 3.0 / a->l;
+      break;
+    case KT_GAUSSIAN:
+      sc = -0.5 / (a->l * a->l);
+      break;
+  }
+  k.s = (T)sc;
+  k.inv_l = (T)(1.0 / a->l);
+  k.c2 = (T)(a->c * a->c);
   return k;
 }
 
@@ -280,9 +308,16 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
   HIP_TRY(ws.mom.alloc((size_t)t_src.ncl * q * nrhs * sizeof(T)));
   HIP_TRY(ws.tlhs.alloc((size_t)t_tgt.ncl * q * nrhs * sizeof(T)));
   ActionBuffers<T> buf{ws.xs.template as<T>(), ws.mom.template as<T>(), ws.tlhs.template as<T>()};
+  PhaseEvents *pe = nullptr;
+  if (a->profiling) {
+    PhaseEvents ev;
+    for (auto &x : ev.ev) HIP_TRY(hipEventCreate(&x));
+    a->pending.push_back(ev);
+    pe = &a->pending.back();
+  }
   hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
                                   interp_params<T>(a->ip), kernel_params<T>(a), d_rhs, ld_rhs,
-                                  (int)nrhs, d_out, ld_out, natural, buf, stream);
+                                  (int)nrhs, d_out, ld_out, natural, buf, stream, pe);
   return from_hip(e, "launch_action");
 }
 
@@ -532,6 +567,45 @@ fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *co
   std::copy(p.block_col.begin(), p.block_col.end(), col);
   std::copy(p.block_lr.begin(), p.block_lr.end(), low_rank);
   return FMCA_OK;
+}
+
+fmca_status fmca_h2_set_profiling(fmca_h2_actor *actor, int enable) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  actor->profiling = enable != 0;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t count[6], int reset) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  for (auto &pe : actor->pending) {
+    HIP_TRY(hipEventSynchronize(pe.ev[PH_COUNT]));
+    for (int k = 0; k < PH_COUNT; ++k) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, pe.ev[k], pe.ev[k + 1]));
+      actor->phase_ms[k] += t;
+      actor->phase_count[k] += 1;
+    }
+    for (auto &e : pe.ev) (void)hipEventDestroy(e);
+  }
+  actor->pending.clear();
+  for (int k = 0; k < PH_COUNT; ++k) {
+    if (ms) ms[k] = actor->phase_ms[k];
+    if (count) count[k] = actor->phase_count[k];
+    if (reset) {
+      actor->phase_ms[k] = 0;
+      actor->phase_count[k] = 0;
+    }
+  }
+  return FMCA_OK;
+}
+
+fmca_status fmca_diag_math(int fn, const double *d_x, double *d_y, int64_t n, void *stream) {
+  if (n < 0 || (n && (!d_x || !d_y))) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  hipError_t e = launch_math_probe(fn, d_x, d_y, n, (hipStream_t)stream);
+  if (e == hipErrorInvalidValue) return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown function");
+  return from_hip(e, "launch_math_probe");
 }
 
 }  // extern "C"

@@ -47,10 +47,17 @@ struct InterpParams {
   T w[kMaxP1];
 };
 
+// Kernel parameters with the radial scale folded in on the host, so that no
+// division is evaluated per kernel entry:
+//   EXPONENTIAL s = 1/l, MATERN32 s = sqrt(3)/l, MATERN52 s = sqrt(5)/l,
+//   MATERN72 s = sqrt(7)/l, MATERN92 s = 3/l, GAUSSIAN s = -1/(2 l^2) (on r^2).
 template <typename T>
 struct KernelParams {
   int kt = 0;
   T l = 1, c = 1, nu = 1;
+  T s = 1;      // radial scale (see above)
+  T inv_l = 1;  // 1 / l
+  T c2 = 1;     // c * c
 };
 
 // Level structure of a tree on the host (drives the per-level launches).
@@ -67,6 +74,23 @@ struct ActionBuffers {
   T *tlhs = nullptr;  // ncl_tgt x q x nrhs
 };
 
+// Phases of one action, in launch order (timed with events when profiling).
+enum Phase : int {
+  PH_PERMUTE = 0,
+  PH_UPWARD_LEAF = 1,
+  PH_UPWARD_TRANSFER = 2,
+  PH_COUPLING = 3,
+  PH_DOWNWARD_TRANSFER = 4,
+  PH_LEAF_OUTPUT = 5,
+  PH_COUNT = 6
+};
+
+// ev[k] is recorded on the launch stream before phase k, ev[PH_COUNT] after
+// the last phase.
+struct PhaseEvents {
+  hipEvent_t ev[PH_COUNT + 1];
+};
+
 // Enqueues the whole matrix-free action out = K(rows, cols) * rhs on `stream`.
 // rhs / out are FP64 column-major device arrays; perm_in / perm_out select the
 // natural order (true) or FMCA's cluster order (false).
@@ -76,6 +100,10 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const DevPatternView &pat, const InterpParams<T> &ip,
                          const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
-                         hipStream_t stream);
+                         hipStream_t stream, const PhaseEvents *events = nullptr);
+
+// Evaluates the device exp (fn 0 / 2) or sqrt (fn 1 / 3) used by the kernel
+// loops, in FP64 (fn 0, 1) or FP32 (fn 2, 3), on n device values.
+hipError_t launch_math_probe(int fn, const double *x, double *y, int64_t n, hipStream_t stream);
 
 }  // namespace fmca_amd

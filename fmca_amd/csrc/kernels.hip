@@ -26,15 +26,56 @@ namespace {
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------
-// Scalar helpers.  FP64 uses the ROCm device math library; FP32 uses the
-// hardware transcendental instructions (v_exp_f32 / v_sqrt_f32).
+// Scalar helpers.
+//
+// FP64: exp and sqrt are written out for the kernel-entry loops, which only
+// ever see finite arguments (exp of -s*r <= 0, sqrt of a squared distance):
+//   exp(x)  = 2^k e^f with k = rint(x log2 e) and f = x - k ln2, ln2 split
+//             into a high and a low part (Cody & Waite) so that f is exact to
+//             working precision; e^f from its Taylor polynomial of degree 12
+//             (|f| <= ln2/2, truncation error < 2e-16 relative), evaluated
+//             with Estrin's scheme.
+//   sqrt(x) = hardware reciprocal-square-root seed (v_rsq_f64) refined by a
+//             Goldschmidt step and a Newton correction of the root; x = 0
+//             returns 0.
+// FP32 uses the hardware transcendental instructions (v_exp_f32, v_sqrt_f32).
 template <typename T>
 struct Math;
 
 template <>
 struct Math<double> {
-  static __device__ __forceinline__ double exp_(double x) { return exp(x); }
-  static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double exp_(double x) {
+    const double k = __builtin_rint(x * 1.4426950408889634074);
+    double f = __builtin_fma(-k, 6.93147180369123816490e-01, x);
+    f = __builtin_fma(-k, 1.90821492927058770002e-10, f);
+    // Estrin evaluation of sum_{j<=12} f^j / j! (fewer dependent steps than
+    // Horner and no accumulator copies)
+    const double f2 = f * f, f4 = f2 * f2, f8 = f4 * f4;
+    const double q0 = __builtin_fma(f, 1.0, 1.0);
+    const double q1 = __builtin_fma(f, 1.66666666666666666667e-01, 0.5);
+    const double q2 = __builtin_fma(f, 8.33333333333333333333e-03, 4.16666666666666666667e-02);
+    const double q3 = __builtin_fma(f, 1.98412698412698412698e-04, 1.38888888888888888889e-03);
+    const double q4 = __builtin_fma(f, 2.75573192239858906526e-06, 2.48015873015873015873e-05);
+    const double q5 = __builtin_fma(f, 2.50521083854417187751e-08, 2.75573192239858906526e-07);
+    const double r0 = __builtin_fma(f2, q1, q0);
+    const double r1 = __builtin_fma(f2, q3, q2);
+    const double r2 = __builtin_fma(f2, q5, q4);
+    const double s0 = __builtin_fma(f4, r1, r0);
+    const double s1 = __builtin_fma(f4, 2.08767569878680989792e-09, r2);
+    const double p = __builtin_fma(f8, s1, s0);
+    const double r = __builtin_ldexp(p, (int)k);
+    return x < -745.0 ? 0.0 : r;
+  }
+  static __device__ __forceinline__ double sqrt_(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    const double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x > 0.0 ? g : 0.0;
+  }
   static __device__ __forceinline__ double log_(double x) { return log(x); }
   static __device__ __forceinline__ double eps() { return 2.220446049250313e-16; }
 };
@@ -49,50 +90,55 @@ struct Math<float> {
   static __device__ __forceinline__ float eps() { return 1.1920928955078125e-07f; }
 };
 
-// Kernel value from the squared distance (CovarianceKernel.h:56-116).  KT < 0
-// selects the type at run time (uniform branch) for the less common kernels.
+__device__ __forceinline__ double fmat(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Kernel value from the squared distance (CovarianceKernel.h:56-116) with the
+// scales of KernelParams.  KT < 0 selects the type at run time (uniform
+// branch) for the less common kernels.
 template <int KT, typename T>
 __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
   using M = Math<T>;
   const int kt = KT >= 0 ? KT : kp.kt;
-  const T l = kp.l;
   switch (kt) {
     case KT_EXPONENTIAL:
-      return M::exp_(-M::sqrt_(r2) / l);
+      return M::exp_(-M::sqrt_(r2) * kp.s);
     case KT_MATERN32: {
-      const T a = T(1.7320508075688772935) * M::sqrt_(r2) / l;
+      const T a = M::sqrt_(r2) * kp.s;
       return (T(1) + a) * M::exp_(-a);
     }
     case KT_MATERN52: {
-      const T a = T(2.2360679774997896964) * M::sqrt_(r2) / l;
-      return (T(1) + a + a * a / T(3)) * M::exp_(-a);
+      const T a = M::sqrt_(r2) * kp.s;
+      return (T(1) + a + a * a * T(1. / 3.)) * M::exp_(-a);
     }
     case KT_MATERN72: {
-      const T r = M::sqrt_(r2), x = r / l;
+      const T x = M::sqrt_(r2) * kp.inv_l;
       return (T(1) + T(2.6457513110645905905) * x + T(14. / 5) * x * x +
               T(7. * 2.6457513110645905905 / 15.) * x * x * x) *
              M::exp_(-T(2.6457513110645905905) * x);
     }
     case KT_MATERN92: {
-      const T r = M::sqrt_(r2), x = r / l;
+      const T x = M::sqrt_(r2) * kp.inv_l;
       return (T(1) + T(3) * x + T(27. / 7.) * x * x + T(18. / 7.) * x * x * x +
               T(27. / 35.) * x * x * x * x) *
              M::exp_(-T(3) * x);
     }
     case KT_GAUSSIAN:
-      return M::exp_(T(-0.5) * r2 / (l * l));
+      return M::exp_(r2 * kp.s);
     case KT_INVMULTIQUADRIC:
-      return T(1) / M::sqrt_(r2 / (l * l) + kp.c * kp.c);
+      return T(1) / M::sqrt_(r2 * kp.inv_l * kp.inv_l + kp.c2);
     case KT_MULTIQUADRIC:
-      return M::sqrt_(r2 / (l * l) + kp.c * kp.c);
+      return M::sqrt_(r2 * kp.inv_l * kp.inv_l + kp.c2);
     case KT_BIHARMONIC2D: {
-      const T r = M::sqrt_(r2);
-      return r < M::eps() ? T(0) : M::log_(r / l) * (r / l) * (r / l);
+      const T r = M::sqrt_(r2), x = r * kp.inv_l;
+      return r < M::eps() ? T(0) : M::log_(x) * x * x;
     }
     case KT_BIHARMONIC3D:
-      return M::sqrt_(r2) / l;
+      return M::sqrt_(r2) * kp.inv_l;
     case KT_TRIHARMONIC3D: {
-      const T x = M::sqrt_(r2) / l;
+      const T x = M::sqrt_(r2) * kp.inv_l;
       return x * x * x;
     }
     case KT_TPS2D: {
@@ -193,8 +239,11 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
           }
         }
       }
-      if (alpha < q)
-        for (int r = 0; r < nr; ++r) mom[((int64_t)c * q + alpha) * nrhs + rc + r] = acc[r];
+      if (alpha < q) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if (r < nr) mom[((int64_t)c * q + alpha) * nrhs + rc + r] = acc[r];
+      }
     }
   }
 }
@@ -314,30 +363,50 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
         }
         __syncthreads();
         if (alpha < q) {
-          int b[D];
+          // tensor loop over the source nodes: the squared distance is
+          // accumulated dimension by dimension, one fma per entry innermost
+          int beta = 0;
+          if constexpr (D == 1) {
+            for (int b0 = 0; b0 < P1; ++b0, ++beta) {
+              const T d0 = x[0] - Y[b0];
+              const T kv = kernel_r2<KT, T>(d0 * d0, kp);
 #pragma unroll
-          for (int i = 0; i < D; ++i) b[i] = 0;
-          for (int beta = 0; beta < q; ++beta) {
-            T r2 = 0;
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-              const T dd = x[i] - Y[i * kMaxP1 + b[i]];
-              r2 += dd * dd;
+              for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
             }
-            const T kv = kernel_r2<KT, T>(r2, kp);
+          } else if constexpr (D == 2) {
+            for (int b1 = 0; b1 < P1; ++b1) {
+              const T d1 = x[1] - Y[kMaxP1 + b1];
+              const T t1 = d1 * d1;
+              for (int b0 = 0; b0 < P1; ++b0, ++beta) {
+                const T d0 = x[0] - Y[b0];
+                const T kv = kernel_r2<KT, T>(fmat(d0, d0, t1), kp);
 #pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
-            // advance the tensor index (uniform across the wave)
+                for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
+              }
+            }
+          } else {
+            for (int b2 = 0; b2 < P1; ++b2) {
+              const T d2 = x[2] - Y[2 * kMaxP1 + b2];
+              const T t2 = d2 * d2;
+              for (int b1 = 0; b1 < P1; ++b1) {
+                const T d1 = x[1] - Y[kMaxP1 + b1];
+                const T t1 = fmat(d1, d1, t2);
+                for (int b0 = 0; b0 < P1; ++b0, ++beta) {
+                  const T d0 = x[0] - Y[b0];
+                  const T kv = kernel_r2<KT, T>(fmat(d0, d0, t1), kp);
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
-              if (++b[i] < P1) break;
-              b[i] = 0;
+                  for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
+                }
+              }
             }
           }
         }
       }
-      if (alpha < q)
-        for (int r = 0; r < nr; ++r) tlhs[((int64_t)t * q + alpha) * nrhs + rc + r] = acc[r];
+      if (alpha < q) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if (r < nr) tlhs[((int64_t)t * q + alpha) * nrhs + rc + r] = acc[r];
+      }
     }
   }
 }
@@ -499,7 +568,9 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
       }
       if (act) {
         const int64_t row = natural ? (int64_t)rows.perm[b0 + i] : (int64_t)(b0 + i);
-        for (int r = 0; r < nr; ++r) out[row + (int64_t)(rc + r) * ldo] = (double)acc[r];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if (r < nr) out[row + (int64_t)(rc + r) * ldo] = (double)acc[r];
       }
     }
   }
@@ -608,9 +679,13 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                     const HostLevels &lrows, const HostLevels &lcols, const DevPatternView &pat,
                     const InterpParams<T> &ip, const KernelParams<T> &kp, const double *rhs,
                     int64_t ld_rhs, int nrhs, double *out, int64_t ld_out, bool natural,
-                    ActionBuffers<T> buf, hipStream_t stream) {
+                    ActionBuffers<T> buf, hipStream_t stream, const PhaseEvents *ev) {
   const int q = ip.q, P1 = ip.P1;
   hipError_t err;
+  auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev->ev[k], stream);
+  };
+  mark(PH_PERMUTE);
   // 1. permute into cluster order
   {
     const int64_t total = cols.npts * nrhs;
@@ -619,6 +694,7 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                        cols.npts, natural ? cols.perm : nullptr, buf.xs);
   }
   // 2. leaf moments
+  mark(PH_UPWARD_LEAF);
   {
     if (nrhs == 1) {
       const size_t lds = ((size_t)D * P1 * 64 + 64 * 1) * sizeof(T);
@@ -632,6 +708,7 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 3. upward transfers, deepest inner level first
+  mark(PH_UPWARD_TRANSFER);
   const size_t budget = 64 * 1024 / sizeof(T) - (size_t)D * P1 * P1;
   const int nrc_up = (int)std::max<size_t>(1, std::min<size_t>(nrhs, budget / (3 * (size_t)q)));
   const int nrc_dn = (int)std::max<size_t>(1, std::min<size_t>(nrhs, budget / (2 * (size_t)q)));
@@ -643,10 +720,12 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 4. coupling (every target cluster is written, empty far lists give zeros)
+  mark(PH_COUPLING);
   if ((err = dispatch_nr<T, D>(false, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
                                stream)) != hipSuccess)
     return err;
   // 5. downward transfers, top level first
+  mark(PH_DOWNWARD_TRANSFER);
   for (int L = 1; L <= lrows.depth; ++L) {
     const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
     const size_t lds = ((size_t)2 * q * nrc_dn + (size_t)D * P1 * P1) * sizeof(T);
@@ -655,8 +734,10 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 6. leaf output: far field through V^T plus the near field
-  return dispatch_nr<T, D>(true, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
-                           stream);
+  mark(PH_LEAF_OUTPUT);
+  err = dispatch_nr<T, D>(true, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural, stream);
+  mark(PH_COUNT);
+  return err;
 }
 
 }  // namespace
@@ -665,23 +746,46 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
 
 namespace fmca_amd {
 
+namespace {
+template <typename T>
+__global__ void k_math_probe(int fn, const double *__restrict__ x, double *__restrict__ y,
+                             int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T v = (T)x[i];
+  y[i] = (double)(fn == 0 ? Math<T>::exp_(v) : Math<T>::sqrt_(v));
+}
+}  // namespace
+
+hipError_t launch_math_probe(int fn, const double *x, double *y, int64_t n, hipStream_t stream) {
+  const int grid = (int)((n + kBlock - 1) / kBlock);
+  if (n <= 0) return hipSuccess;
+  if (fn == 0 || fn == 1)
+    hipLaunchKernelGGL((k_math_probe<double>), dim3(grid), dim3(kBlock), 0, stream, fn, x, y, n);
+  else if (fn == 2 || fn == 3)
+    hipLaunchKernelGGL((k_math_probe<float>), dim3(grid), dim3(kBlock), 0, stream, fn - 2, x, y, n);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const HostLevels &lrows, const HostLevels &lcols,
                          const DevPatternView &pat, const InterpParams<T> &ip,
                          const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
-                         hipStream_t stream) {
+                         hipStream_t stream, const PhaseEvents *events) {
   switch (rows.dim) {
     case 1:
       return action_d<T, 1>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream);
+                            ld_out, natural_order, buf, stream, events);
     case 2:
       return action_d<T, 2>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream);
+                            ld_out, natural_order, buf, stream, events);
     case 3:
       return action_d<T, 3>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream);
+                            ld_out, natural_order, buf, stream, events);
   }
   return hipErrorInvalidValue;
 }
@@ -691,12 +795,12 @@ template hipError_t launch_action<double>(const DevTreeView<double> &, const Dev
                                           const DevPatternView &, const InterpParams<double> &,
                                           const KernelParams<double> &, const double *, int64_t,
                                           int, double *, int64_t, bool, ActionBuffers<double>,
-                                          hipStream_t);
+                                          hipStream_t, const PhaseEvents *);
 template hipError_t launch_action<float>(const DevTreeView<float> &, const DevTreeView<float> &,
                                          const HostLevels &, const HostLevels &,
                                          const DevPatternView &, const InterpParams<float> &,
                                          const KernelParams<float> &, const double *, int64_t, int,
                                          double *, int64_t, bool, ActionBuffers<float>,
-                                         hipStream_t);
+                                         hipStream_t, const PhaseEvents *);
 
 }  // namespace fmca_amd

@@ -1,21 +1,67 @@
 // Host-side planner: cluster tree, block pattern, Chebyshev data, transfers.
-// Level-synchronous and OpenMP-parallel; produces FMCA's tree and blocks.
+// Level-synchronous and thread-parallel; produces FMCA's tree and blocks.
 #include "plan.hpp"
 
-#include <omp.h>
-
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <thread>
 #include <utility>
 
 namespace fmca_amd {
 
 namespace {
 constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+// Host worker threads: FMCA_AMD_THREADS, else OMP_NUM_THREADS, else the
+// hardware concurrency, at most 64.  The planner runs its parallel loops on
+// plain std::threads so that it brings no OpenMP runtime into the process.
+int num_workers() {
+  for (const char *v : {"FMCA_AMD_THREADS", "OMP_NUM_THREADS"})
+    if (const char *e = std::getenv(v)) {
+      const int k = std::atoi(e);
+      if (k > 0) return std::min(k, 64);
+    }
+  const unsigned h = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(h, 64u));
+}
+
+// f(i) for i in [0, n), dynamically scheduled in chunks of `grain`.
+template <typename F>
+void parallel_for(int64_t n, int64_t grain, F &&f) {
+  const int nw = (int)std::min<int64_t>(num_workers(), (n + grain - 1) / std::max<int64_t>(grain, 1));
+  if (nw <= 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (;;) {
+      const int64_t lo = next.fetch_add(grain);
+      if (lo >= n) break;
+      const int64_t hi = std::min(n, lo + grain);
+      for (int64_t i = lo; i < hi; ++i) f(i);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nw; ++t) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
+
+// f(tid, lo, hi) on nt contiguous static chunks of [0, n).
+template <typename F>
+void parallel_chunks(int64_t n, int nt, F &&f) {
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back([&, t] { f(t, n * t / nt, n * (t + 1) / nt); });
+  f(0, 0, n / nt);
+  for (auto &th : pool) th.join();
+}
 constexpr double kPi = 3.14159265358979323846264338327950288;
 const char *const kNames[KT_COUNT] = {
     "EXPONENTIAL",     "MATERN32",     "MATERN52",     "MATERN72",
@@ -105,13 +151,23 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
   root.begin = 0;
   root.size = n;
   // eps-enlarged root box (ClusterTreeInitializer_ClusterTree.h:44-56)
+  const int nw = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), n / 65536 + 1));
   for (int i = 0; i < dim; ++i) {
+    std::vector<double> tmn(nw, P[i]), tmx(nw, P[i]);
+    parallel_chunks(n, nw, [&](int t, int64_t lo, int64_t hi) {
+      double a = P[i], b = P[i];
+      for (int64_t j = lo; j < hi; ++j) {
+        const double v = P[i + (size_t)dim * j];
+        a = v < a ? v : a;
+        b = v > b ? v : b;
+      }
+      tmn[t] = a;
+      tmx[t] = b;
+    });
     double mn = P[i], mx = P[i];
-#pragma omp parallel for reduction(min : mn) reduction(max : mx) schedule(static)
-    for (int64_t j = 1; j < n; ++j) {
-      const double v = P[i + (size_t)dim * j];
-      mn = v < mn ? v : mn;
-      mx = v > mx ? v : mx;
+    for (int t = 0; t < nw; ++t) {
+      mn = tmn[t] < mn ? tmn[t] : mn;
+      mx = tmx[t] > mx ? tmx[t] : mx;
     }
     mn -= kEps * std::abs(mn);
     mx += kEps * std::abs(mx);
@@ -132,10 +188,9 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
     // Sorting (key, index) pairs by key performs the same comparisons as
     // sorting indices through the coordinate, so std::sort yields the same
     // permutation, ties included.
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t k = 0; k < cnt; ++k) {
+    parallel_for(cnt, 1, [&](int64_t k) {
       const WorkNode &nd = nodes[lb + k];
-      if (nd.size <= thr) continue;
+      if (nd.size <= thr) return;
       split[k] = 1;
       int longest = 0;
       for (int i = 1; i < dim; ++i)
@@ -162,7 +217,7 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
       c2.bb[longest + 2 * dim] = c2.bb[longest + dim] - c2.bb[longest];
       c1.level = c2.level = level + 1;
       c1.parent = c2.parent = (int32_t)(lb + k);
-    }
+    });
     for (int64_t k = 0; k < cnt; ++k)
       if (split[k]) {
         nodes[lb + k].son0 = (int32_t)nodes.size();
@@ -199,8 +254,9 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
   }
   // Shrink to tight boxes, deepest level first (ClusterTreeInitializer_ClusterTree.h:88-132).
   for (int L = T.depth; L >= 0; --L) {
-#pragma omp parallel for schedule(dynamic, 16)
-    for (int c = T.level_ptr[L]; c < T.level_ptr[L + 1]; ++c) {
+    const int cl0 = T.level_ptr[L];
+    parallel_for(T.level_ptr[L + 1] - cl0, 16, [&](int64_t ci) {
+      const int c = cl0 + (int)ci;
       double mn[kMaxDim], mx[kMaxDim];
       if (T.size[c]) {
         const double *p0 = P + (size_t)dim * T.perm[T.begin[c]];
@@ -238,7 +294,7 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
         T.bbmax[(size_t)c * dim + i] = mx[i];
         T.bbsize[(size_t)c * dim + i] = sz;
       }
-    }
+    });
   }
 }
 
@@ -277,17 +333,14 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
   // Level-synchronous expansion of the block tree; the set of leaf blocks is
   // the one of FMCA's depth-first construction (H2MatrixBase.h:160-191).
   std::vector<std::pair<int32_t, int32_t>> frontier{{0, 0}};
-  const int nth = omp_get_max_threads();
+  const int nth = num_workers();
   while (!frontier.empty()) {
     std::vector<std::vector<std::pair<int32_t, int32_t>>> next(nth);
     std::vector<std::vector<int32_t>> brow(nth), bcol(nth);
     std::vector<std::vector<int8_t>> blr(nth);
     const int64_t F = (int64_t)frontier.size();
-#pragma omp parallel num_threads(nth)
-    {
-      const int tid = omp_get_thread_num();
-      const int nt = omp_get_num_threads();
-      const int64_t lo = F * tid / nt, hi = F * (tid + 1) / nt;
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nth, F / 256 + 1));
+    parallel_chunks(F, nt, [&](int tid, int64_t lo, int64_t hi) {
       for (int64_t k = lo; k < hi; ++k) {
         const int t = frontier[k].first, s = frontier[k].second;
         const int adm = admissibility(R, t, C, s, eta);
@@ -308,7 +361,7 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
           blr[tid].push_back(lr);
         }
       }
-    }
+    });
     frontier.clear();
     for (int t = 0; t < nth; ++t) {
       frontier.insert(frontier.end(), next[t].begin(), next[t].end());
@@ -382,8 +435,8 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
 void compute_transfers(std::vector<double> &T, const Tree &tree, const Interpolation &ip) {
   const int d = tree.dim, P1 = ip.P1;
   T.assign((size_t)tree.ncl * d * P1 * P1, 0.);
-#pragma omp parallel for schedule(static)
-  for (int c = 1; c < tree.ncl; ++c) {
+  parallel_for(tree.ncl - 1, 64, [&](int64_t ci) {
+    const int c = 1 + (int)ci;
     const int p = tree.parent[c];
     double vals[kMaxP1];
     for (int i = 0; i < d; ++i) {
@@ -398,7 +451,7 @@ void compute_transfers(std::vector<double> &T, const Tree &tree, const Interpola
         for (int a = 0; a < P1; ++a) Ti[a * P1 + b] = vals[a];
       }
     }
-  }
+  });
 }
 
 Statistics statistics(const Pattern &pat, const Tree &R, const Tree &C, int q) {

@@ -174,8 +174,33 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
              check(fmca_h2_statistics(a.h, out.mutable_data()));
              return out;
            })
+      .def("set_profiling", [](Actor &a, bool on) { check(fmca_h2_set_profiling(a.h, on ? 1 : 0)); })
+      .def(
+          "phase_times",
+          [](Actor &a, bool reset) {
+            double ms[6];
+            int64_t cnt[6];
+            {
+              py::gil_scoped_release nogil;
+              check(fmca_h2_phase_times(a.h, ms, cnt, reset ? 1 : 0));
+            }
+            py::dict d;
+            const char *names[6] = {"permute",  "upward_leaf",        "upward_transfer",
+                                    "coupling", "downward_transfer", "leaf_output"};
+            for (int k = 0; k < 6; ++k) d[names[k]] = py::make_tuple(ms[k], cnt[k]);
+            return d;
+          },
+          py::arg("reset") = false)
       .def("info", &info)
       .def("clusters", &clusters)
       .def("blocks", &blocks);
+  m.def(
+      "diag_math",
+      [](int fn, uintptr_t x, uintptr_t y, int64_t n, uintptr_t stream) {
+        py::gil_scoped_release nogil;
+        check(fmca_diag_math(fn, reinterpret_cast<const double *>(x),
+                             reinterpret_cast<double *>(y), n, reinterpret_cast<void *>(stream)));
+      },
+      py::arg("fn"), py::arg("x"), py::arg("y"), py::arg("n"), py::arg("stream") = 0);
   m.def("status_string", [](int s) { return std::string(fmca_status_string((fmca_status)s)); });
 }

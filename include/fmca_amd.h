@@ -121,6 +121,21 @@ fmca_status fmca_h2_clusters(const fmca_h2_actor *actor, int which, int64_t *beg
 fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *col,
                            int8_t *low_rank);
 
+/* Per-phase timing (no FMCA counterpart; FMCA times whole calls with Tictoc,
+ * FMCA/src/util/Tictoc.h:22-42).  When enabled, every action records HIP
+ * events on its launch stream around its six phases:
+ *   0 permute, 1 upward leaf moments, 2 upward transfers, 3 coupling,
+ *   4 downward transfers, 5 leaf output (near field + leaf basis).
+ * fmca_h2_phase_times waits for the recorded events and returns the summed
+ * milliseconds and the number of actions per phase; reset != 0 clears them. */
+fmca_status fmca_h2_set_profiling(fmca_h2_actor *actor, int enable);
+fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t count[6], int reset);
+
+/* Diagnostic (no FMCA counterpart): evaluates the device exp (fn 0 FP64,
+ * 2 FP32) or sqrt (fn 1 FP64, 3 FP32) that the kernel-entry loops use, on n
+ * device values, so that their accuracy can be tested. */
+fmca_status fmca_diag_math(int fn, const double *d_x, double *d_y, int64_t n, void *stream);
+
 /* Static string for a status code. */
 const char *fmca_status_string(fmca_status status);
 
