@@ -102,6 +102,19 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
                          hipStream_t stream, const PhaseEvents *events = nullptr);
 
+// Matrix-core (MFMA) forms of the coupling (output = false) and the leaf
+// output (output = true), used from kMfmaMinRhs right-hand sides on.
+constexpr int kMfmaMinRhs = 8;
+
+template <typename T>
+bool mfma_supported(bool output, int D, int P1, int q, int nrhs);
+
+template <typename T>
+hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                       const DevPatternView &pat, const InterpParams<T> &ip,
+                       const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom, T *tlhs,
+                       double *out, int64_t ldo, bool natural, hipStream_t stream);
+
 // Evaluates the device exp (fn 0 / 2) or sqrt (fn 1 / 3) used by the kernel
 // loops, in FP64 (fn 0, 1) or FP32 (fn 2, 3), on n device values.
 hipError_t launch_math_probe(int fn, const double *x, double *y, int64_t n, hipStream_t stream);

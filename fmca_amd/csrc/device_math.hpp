@@ -1,0 +1,154 @@
+// Device helpers shared by the HIP kernels: scalar math of the kernel-entry
+// loops, kernel functions, barycentric Lagrange factors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace fmca_amd {
+namespace dev {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Scalar helpers.
+//
+// FP64: exp and sqrt are written out for the kernel-entry loops, which only
+// ever see finite arguments (exp of -s*r <= 0, sqrt of a squared distance):
+//   exp(x)  = 2^k e^f with k = rint(x log2 e) and f = x - k ln2, ln2 split
+//             into a high and a low part (Cody & Waite) so that f is exact to
+//             working precision; e^f from its Taylor polynomial of degree 12
+//             (|f| <= ln2/2, truncation error < 2e-16 relative), evaluated
+//             with Estrin's scheme.
+//   sqrt(x) = hardware reciprocal-square-root seed (v_rsq_f64) refined by a
+//             Goldschmidt step and a Newton correction of the root; x = 0
+//             returns 0.
+// FP32 uses the hardware transcendental instructions (v_exp_f32, v_sqrt_f32).
+template <typename T>
+struct Math;
+
+template <>
+struct Math<double> {
+  static __device__ __forceinline__ double exp_(double x) {
+    const double k = __builtin_rint(x * 1.4426950408889634074);
+    double f = __builtin_fma(-k, 6.93147180369123816490e-01, x);
+    f = __builtin_fma(-k, 1.90821492927058770002e-10, f);
+    // Estrin evaluation of sum_{j<=12} f^j / j! (fewer dependent steps than
+    // Horner and no accumulator copies)
+    const double f2 = f * f, f4 = f2 * f2, f8 = f4 * f4;
+    const double q0 = __builtin_fma(f, 1.0, 1.0);
+    const double q1 = __builtin_fma(f, 1.66666666666666666667e-01, 0.5);
+    const double q2 = __builtin_fma(f, 8.33333333333333333333e-03, 4.16666666666666666667e-02);
+    const double q3 = __builtin_fma(f, 1.98412698412698412698e-04, 1.38888888888888888889e-03);
+    const double q4 = __builtin_fma(f, 2.75573192239858906526e-06, 2.48015873015873015873e-05);
+    const double q5 = __builtin_fma(f, 2.50521083854417187751e-08, 2.75573192239858906526e-07);
+    const double r0 = __builtin_fma(f2, q1, q0);
+    const double r1 = __builtin_fma(f2, q3, q2);
+    const double r2 = __builtin_fma(f2, q5, q4);
+    const double s0 = __builtin_fma(f4, r1, r0);
+    const double s1 = __builtin_fma(f4, 2.08767569878680989792e-09, r2);
+    const double p = __builtin_fma(f8, s1, s0);
+    const double r = __builtin_ldexp(p, (int)k);
+    return x < -745.0 ? 0.0 : r;
+  }
+  static __device__ __forceinline__ double sqrt_(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    const double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x > 0.0 ? g : 0.0;
+  }
+  static __device__ __forceinline__ double log_(double x) { return log(x); }
+  static __device__ __forceinline__ double eps() { return 2.220446049250313e-16; }
+};
+
+template <>
+struct Math<float> {
+  static __device__ __forceinline__ float exp_(float x) {
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float log_(float x) { return logf(x); }
+  static __device__ __forceinline__ float eps() { return 1.1920928955078125e-07f; }
+};
+
+__device__ __forceinline__ double fmat(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Kernel value from the squared distance (CovarianceKernel.h:56-116) with the
+// scales of KernelParams.  KT < 0 selects the type at run time (uniform
+// branch) for the less common kernels.
+template <int KT, typename T>
+__device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
+  using M = Math<T>;
+  const int kt = KT >= 0 ? KT : kp.kt;
+  switch (kt) {
+    case KT_EXPONENTIAL:
+      return M::exp_(-M::sqrt_(r2) * kp.s);
+    case KT_MATERN32: {
+      const T a = M::sqrt_(r2) * kp.s;
+      return (T(1) + a) * M::exp_(-a);
+    }
+    case KT_MATERN52: {
+      const T a = M::sqrt_(r2) * kp.s;
+      return (T(1) + a + a * a * T(1. / 3.)) * M::exp_(-a);
+    }
+    case KT_MATERN72: {
+      const T x = M::sqrt_(r2) * kp.inv_l;
+      return (T(1) + T(2.6457513110645905905) * x + T(14. / 5) * x * x +
+              T(7. * 2.6457513110645905905 / 15.) * x * x * x) *
+             M::exp_(-T(2.6457513110645905905) * x);
+    }
+    case KT_MATERN92: {
+      const T x = M::sqrt_(r2) * kp.inv_l;
+      return (T(1) + T(3) * x + T(27. / 7.) * x * x + T(18. / 7.) * x * x * x +
+              T(27. / 35.) * x * x * x * x) *
+             M::exp_(-T(3) * x);
+    }
+    case KT_GAUSSIAN:
+      return M::exp_(r2 * kp.s);
+    case KT_INVMULTIQUADRIC:
+      return T(1) / M::sqrt_(r2 * kp.inv_l * kp.inv_l + kp.c2);
+    case KT_MULTIQUADRIC:
+      return M::sqrt_(r2 * kp.inv_l * kp.inv_l + kp.c2);
+    case KT_BIHARMONIC2D: {
+      const T r = M::sqrt_(r2), x = r * kp.inv_l;
+      return r < M::eps() ? T(0) : M::log_(x) * x * x;
+    }
+    case KT_BIHARMONIC3D:
+      return M::sqrt_(r2) * kp.inv_l;
+    case KT_TRIHARMONIC3D: {
+      const T x = M::sqrt_(r2) * kp.inv_l;
+      return x * x * x;
+    }
+    case KT_TPS2D: {
+      const T r = M::sqrt_(r2);
+      return r > T(0) ? r * r * M::log_(r) : T(0);
+    }
+  }
+  return T(0);
+}
+
+// Barycentric Lagrange factors l_a(u), a < P1, written with a stride
+// (TensorProductInterpolator.h:64-81; a factor within eps of its node is 1).
+template <typename T>
+__device__ __forceinline__ void lagrange_1d(T u, const InterpParams<T> &ip, T *out, int stride) {
+  T s = 0;
+  for (int j = 0; j < ip.P1; ++j) s += ip.w[j] / (u - ip.xi[j]);
+  for (int a = 0; a < ip.P1; ++a) {
+    const T d = u - ip.xi[a];
+    out[a * stride] = fabs(d) > Math<T>::eps() ? ip.w[a] / d / s : T(1);
+  }
+}
+
+}  // namespace dev
+}  // namespace fmca_amd

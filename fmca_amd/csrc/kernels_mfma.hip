@@ -1,0 +1,467 @@
+// Matrix-core (MFMA) forms of the two kernel-entry phases for several rhs.
+//
+// Both phases are contractions whose left operand is generated on the fly:
+//   leaf output   Y_t (rows x rhs) = V_t^T l_t + sum_near K(P_t, P_s) X_s
+//   coupling      l_t (q x rhs)    = sum_far K(xi_t, xi_s) m_s
+// They run on v_mfma_{f32,f64}_16x16x4 tiles.  In that instruction lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; its four accumulator values are
+// C[4(l/16)+v][l%16] for f32 but C[(l/16)+4v][l%16] for f64 (v = 0..3; both
+// measured on gfx950 with lane-coded operands).  Each lane therefore evaluates exactly the one kernel entry (or
+// leaf-basis value) that its A slot needs -- entries go from VALU registers
+// straight into the matrix core, with no LDS round trip and no shuffles --
+// while the right operand (rhs rows / moments) is staged in LDS once per
+// workgroup and read lane-parallel.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+#include "device_math.hpp"
+
+namespace fmca_amd {
+
+namespace {
+
+using namespace dev;
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // row of the 16x16 result tile held in accumulator item v of lane group lk
+  static __device__ __forceinline__ int crow(int lk, int v) { return 4 * lk + v; }
+};
+
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int crow(int lk, int v) { return lk + 4 * v; }
+};
+
+constexpr int kWaves = 4;         // waves per workgroup
+constexpr int kRT = 4;            // 16-row tiles per wave -> 256 rows per pass
+constexpr int kRows = kWaves * kRT * 16;
+constexpr int kChunk = 64;        // source points / nodes staged per LDS chunk
+
+// Packed tensor index of every moment: a0 | a1 << 8 | a2 << 16.
+__device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0xff; }
+
+// ---------------------------------------------------------------------------
+// Leaf output on MFMA: one workgroup per target leaf, 4 waves, each wave owns
+// row tiles {wave, wave+4, wave+8, wave+12} of the current 256-row pass and
+// 16*NT rhs columns.
+template <typename T, int D, int KT, int NT>
+__global__ __launch_bounds__(256) void k_leaf_output_mfma(
+    DevTreeView<T> rows, DevTreeView<T> cols, DevPatternView pat, InterpParams<T> ip,
+    KernelParams<T> kp, int nrhs, const T *__restrict__ xs, const T *__restrict__ tlhs,
+    double *__restrict__ out, int64_t ldo, int natural) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr int NC = 16 * NT;
+  extern __shared__ unsigned char smem_raw[];
+  const int P1 = ip.P1, q = ip.q;
+  const int qpad = (q + 3) & ~3;
+  T *F = reinterpret_cast<T *>(smem_raw);              // [D][P1][kRows] factors of the rows
+  int *aidx = reinterpret_cast<int *>(F + D * P1 * kRows);  // [qpad] packed indices
+  T *U = reinterpret_cast<T *>(aidx + ((qpad + 3) & ~3));   // union region
+  T *Ms = U;                                            // [kChunk][NC] coefficients
+  T *Ys = U;                                            // [D][kChunk] source points
+  T *Xs = U + D * kChunk;                               // [kChunk][NC] source rhs
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int li = blockIdx.x;
+  const int c = rows.leaves[li];
+  const int b0 = rows.begin[c], sz = rows.size[c];
+  const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
+  T mn[D], bs[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    mn[d] = rows.bbmin[c * D + d];
+    bs[d] = rows.bbsize[c * D + d];
+  }
+  for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
+    int rem = e < q ? e : 0, pk = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      pk |= (rem % P1) << (8 * d);
+      rem /= P1;
+    }
+    aidx[e] = pk;
+  }
+  for (int pbase = 0; pbase < sz; pbase += kRows) {
+    // Lagrange factors of the rows of this pass (thread <-> row)
+    __syncthreads();
+    {
+      const int i = pbase + threadIdx.x;
+      if (i < sz) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const T xv = rows.pts[(int64_t)d * rows.npts + b0 + i];
+          lagrange_1d<T>((xv - mn[d]) / bs[d], ip, F + (d * P1) * kRows + threadIdx.x, kRows);
+        }
+      } else {
+        for (int d = 0; d < D; ++d)
+          for (int a = 0; a < P1; ++a) F[(d * P1 + a) * kRows + threadIdx.x] = T(0);
+      }
+    }
+    // this lane's A rows: tile (wave + 4 rt), row lr
+    T x[kRT][D];
+    bool tile_on[kRT];
+#pragma unroll
+    for (int rt = 0; rt < kRT; ++rt) {
+      const int tile_row0 = pbase + (wave + kWaves * rt) * 16;
+      tile_on[rt] = tile_row0 < sz;
+      const int i = tile_row0 + lr;
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        x[rt][d] = i < sz ? rows.pts[(int64_t)d * rows.npts + b0 + i] : T(0);
+    }
+    for (int rc = 0; rc < nrhs; rc += NC) {
+      acc_t acc[kRT][NT];
+#pragma unroll
+      for (int rt = 0; rt < kRT; ++rt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = acc_t{0, 0, 0, 0};
+      // far field: A = V^T (rows x q), B = l_t (q x NC), K-steps of 4 moments
+      for (int a0 = 0; a0 < qpad; a0 += kChunk) {
+        const int na = min(kChunk, qpad - a0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
+          const int al = e / NC, r = e % NC;
+          const int alpha = a0 + al;
+          Ms[e] = (alpha < q && rc + r < nrhs) ? tlhs[((int64_t)c * q + alpha) * nrhs + rc + r]
+                                              : T(0);
+        }
+        __syncthreads();
+        for (int ks = 0; ks < na; ks += 4) {
+          const int al = ks + lk;
+          const int pk = aidx[a0 + al];
+          T b[NT];
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[al * NC + nt * 16 + lr];
+#pragma unroll
+          for (int rt = 0; rt < kRT; ++rt) {
+            if (!tile_on[rt]) continue;
+            const int row = (wave + kWaves * rt) * 16 + lr;
+            T v = F[(0 * P1 + unpack(pk, 0)) * kRows + row];
+#pragma unroll
+            for (int d = 1; d < D; ++d) v *= F[(d * P1 + unpack(pk, d)) * kRows + row];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(v, b[nt], acc[rt][nt]);
+          }
+        }
+      }
+      // near field: A = K(rows, sources) generated per lane, B = source rhs
+      for (int nb = n0; nb < n1; ++nb) {
+        const int s = pat.near_src[nb];
+        const int sb = cols.begin[s], ss = cols.size[s];
+        for (int j0 = 0; j0 < ss; j0 += kChunk) {
+          const int nj = min(kChunk, ss - j0);
+          __syncthreads();
+          for (int e = threadIdx.x; e < D * kChunk; e += blockDim.x) {
+            const int d = e / kChunk, jj = e % kChunk;
+            Ys[e] = jj < nj ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj] : T(0);
+          }
+          for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
+            const int jj = e / NC, r = e % NC;
+            Xs[e] = (jj < nj && rc + r < nrhs) ? xs[(int64_t)(sb + j0 + jj) * nrhs + rc + r]
+                                              : T(0);
+          }
+          __syncthreads();
+          const int nsteps = (nj + 3) & ~3;
+          for (int ks = 0; ks < nsteps; ks += 4) {
+            const int sj = ks + lk;
+            T y[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) y[d] = Ys[d * kChunk + sj];
+            T b[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) b[nt] = Xs[sj * NC + nt * 16 + lr];
+#pragma unroll
+            for (int rt = 0; rt < kRT; ++rt) {
+              if (!tile_on[rt]) continue;
+              T r2 = 0;
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const T dd = x[rt][d] - y[d];
+                r2 = fmat(dd, dd, r2);
+              }
+              const T e = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+              for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+            }
+          }
+        }
+      }
+      // store: C row crow(lk, v) of tile (wave + 4 rt), column lr of rhs tile nt
+#pragma unroll
+      for (int rt = 0; rt < kRT; ++rt) {
+        if (!tile_on[rt]) continue;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int col = rc + nt * 16 + lr;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = pbase + (wave + kWaves * rt) * 16 + M::crow(lk, v);
+            if (i < sz && col < nrhs) {
+              const int64_t row = natural ? (int64_t)rows.perm[b0 + i] : (int64_t)(b0 + i);
+              out[row + (int64_t)col * ldo] = (double)acc[rt][nt][v];
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coupling on MFMA: one workgroup per target cluster; rows = the q target
+// interpolation nodes (tiles of 16, kRT per wave, so q <= 256), K = the q
+// source nodes of each far cluster, N = 16*NT rhs columns.
+template <typename T, int D, int KT, int NT>
+__global__ __launch_bounds__(256) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
+                                                       DevPatternView pat, InterpParams<T> ip,
+                                                       KernelParams<T> kp, int nrhs,
+                                                       const T *__restrict__ mom,
+                                                       T *__restrict__ tlhs) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr int NC = 16 * NT;
+  extern __shared__ unsigned char smem_raw[];
+  const int P1 = ip.P1, q = ip.q;
+  const int qpad = (q + 3) & ~3;
+  int *bidx = reinterpret_cast<int *>(smem_raw);                     // [qpad]
+  T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][kMaxP1]
+  T *Ms = Y + D * kMaxP1;                                            // [kChunk][NC]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int t = blockIdx.x;
+  const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
+  for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
+    int rem = e < q ? e : 0, pk = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      pk |= (rem % P1) << (8 * d);
+      rem /= P1;
+    }
+    bidx[e] = pk;
+  }
+  // this lane's target node per row tile
+  T x[kRT][D];
+  bool tile_on[kRT];
+#pragma unroll
+  for (int rt = 0; rt < kRT; ++rt) {
+    const int tile_row0 = (wave + kWaves * rt) * 16;
+    tile_on[rt] = tile_row0 < q;
+    int rem = tile_row0 + lr < q ? tile_row0 + lr : 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int ad = rem % P1;
+      rem /= P1;
+      x[rt][d] = ip.xi[ad] * rows.bbsize[t * D + d] + rows.bbmin[t * D + d];
+    }
+  }
+  for (int rc = 0; rc < nrhs; rc += NC) {
+    acc_t acc[kRT][NT];
+#pragma unroll
+    for (int rt = 0; rt < kRT; ++rt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = acc_t{0, 0, 0, 0};
+    for (int f = f0; f < f1; ++f) {
+      const int s = pat.far_src[f];
+      for (int be = 0; be < qpad; be += kChunk) {
+        const int nbeta = min(kChunk, qpad - be);
+        __syncthreads();
+        if (be == 0)
+          for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
+            const int d = e / P1, b = e % P1;
+            Y[d * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+          }
+        for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
+          const int bl = e / NC, r = e % NC;
+          const int beta = be + bl;
+          Ms[e] = (beta < q && rc + r < nrhs) ? mom[((int64_t)s * q + beta) * nrhs + rc + r]
+                                             : T(0);
+        }
+        __syncthreads();
+        for (int ks = 0; ks < nbeta; ks += 4) {
+          const int bl = ks + lk;
+          const int pk = bidx[be + bl];
+          T y[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) y[d] = Y[d * kMaxP1 + unpack(pk, d)];
+          T b[NT];
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NC + nt * 16 + lr];
+#pragma unroll
+          for (int rt = 0; rt < kRT; ++rt) {
+            if (!tile_on[rt]) continue;
+            T r2 = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const T dd = x[rt][d] - y[d];
+              r2 = fmat(dd, dd, r2);
+            }
+            const T e = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < kRT; ++rt) {
+      if (!tile_on[rt]) continue;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int col = rc + nt * 16 + lr;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int alpha = (wave + kWaves * rt) * 16 + M::crow(lk, v);
+          if (alpha < q && col < nrhs)
+            tlhs[((int64_t)t * q + alpha) * nrhs + col] = acc[rt][nt][v];
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int NT>
+size_t leaf_output_mfma_lds(int D, int P1, int q) {
+  const int qpad = (q + 3) & ~3;
+  const size_t u1 = (size_t)kChunk * 16 * NT;
+  const size_t u2 = (size_t)D * kChunk + (size_t)kChunk * 16 * NT;
+  return (size_t)D * P1 * kRows * sizeof(T) + (size_t)((qpad + 3) & ~3) * sizeof(int) +
+         (u1 > u2 ? u1 : u2) * sizeof(T);
+}
+
+template <typename T, int NT>
+size_t coupling_mfma_lds(int D, int q) {
+  const int qpad = (q + 3) & ~3;
+  return (size_t)((qpad + 3) & ~3) * sizeof(int) +
+         ((size_t)D * kMaxP1 + (size_t)kChunk * 16 * NT) * sizeof(T);
+}
+
+template <typename T, int D, int NT>
+hipError_t launch_mfma_nt(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                          const DevPatternView &pat, const InterpParams<T> &ip,
+                          const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom,
+                          T *tlhs, double *out, int64_t ldo, bool natural, hipStream_t stream) {
+  const size_t lds_out = leaf_output_mfma_lds<T, NT>(D, ip.P1, ip.q);
+  const size_t lds_cpl = coupling_mfma_lds<T, NT>(D, ip.q);
+#define FMCA_MFMA_CASE(K)                                                                   \
+  case K:                                                                                   \
+    if (output)                                                                             \
+      hipLaunchKernelGGL((k_leaf_output_mfma<T, D, K, NT>), dim3(rows.nleaves), dim3(256),  \
+                         lds_out, stream, rows, cols,                                       \
+                         pat, ip, kp, nrhs, xs, tlhs, out, ldo, natural ? 1 : 0);           \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_coupling_mfma<T, D, K, NT>), dim3(rows.ncl), dim3(256),         \
+                         lds_cpl, stream, rows, cols, pat, ip, kp,                          \
+                         nrhs, mom, tlhs);                                                  \
+    break;
+  switch (kp.kt) {
+    FMCA_MFMA_CASE(KT_EXPONENTIAL)
+    FMCA_MFMA_CASE(KT_MATERN32)
+    FMCA_MFMA_CASE(KT_MATERN52)
+    FMCA_MFMA_CASE(KT_GAUSSIAN)
+    default:
+      if (output)
+        hipLaunchKernelGGL((k_leaf_output_mfma<T, D, -1, NT>), dim3(rows.nleaves), dim3(256),
+                           lds_out, stream, rows, cols, pat, ip, kp, nrhs, xs, tlhs, out, ldo,
+                           natural ? 1 : 0);
+      else
+        hipLaunchKernelGGL((k_coupling_mfma<T, D, -1, NT>), dim3(rows.ncl), dim3(256),
+                           lds_cpl, stream, rows, cols, pat, ip, kp, nrhs, mom, tlhs);
+  }
+#undef FMCA_MFMA_CASE
+  return hipGetLastError();
+}
+
+// rhs columns per pass: 16, 32 or (FP32 only) 64
+template <typename T>
+int mfma_nt(int nrhs) {
+  if (nrhs <= 16) return 1;
+  if (nrhs <= 32 || sizeof(T) == 8) return 2;
+  return 4;
+}
+
+template <typename T, int D>
+hipError_t launch_mfma_d(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                         const DevPatternView &pat, const InterpParams<T> &ip,
+                         const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom, T *tlhs,
+                         double *out, int64_t ldo, bool natural, hipStream_t stream) {
+  switch (mfma_nt<T>(nrhs)) {
+    case 1:
+      return launch_mfma_nt<T, D, 1>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
+                                     ldo, natural, stream);
+    case 2:
+      return launch_mfma_nt<T, D, 2>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
+                                     ldo, natural, stream);
+    default:
+      return launch_mfma_nt<T, D, 4>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
+                                     ldo, natural, stream);
+  }
+}
+
+}  // namespace
+
+template <typename T>
+bool mfma_supported(bool output, int D, int P1, int q, int nrhs) {
+  if (nrhs < kMfmaMinRhs) return false;
+  const size_t limit = 64 * 1024;
+  const int nt = mfma_nt<T>(nrhs);
+  if (output) {
+    const size_t b = nt == 1   ? leaf_output_mfma_lds<T, 1>(D, P1, q)
+                     : nt == 2 ? leaf_output_mfma_lds<T, 2>(D, P1, q)
+                               : leaf_output_mfma_lds<T, 4>(D, P1, q);
+    return b <= limit;
+  }
+  if (q > kWaves * kRT * 16) return false;
+  const size_t b = nt == 1   ? coupling_mfma_lds<T, 1>(D, q)
+                   : nt == 2 ? coupling_mfma_lds<T, 2>(D, q)
+                             : coupling_mfma_lds<T, 4>(D, q);
+  return b <= limit;
+}
+
+template <typename T>
+hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                       const DevPatternView &pat, const InterpParams<T> &ip,
+                       const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom, T *tlhs,
+                       double *out, int64_t ldo, bool natural, hipStream_t stream) {
+  switch (rows.dim) {
+    case 1:
+      return launch_mfma_d<T, 1>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out, ldo,
+                                 natural, stream);
+    case 2:
+      return launch_mfma_d<T, 2>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out, ldo,
+                                 natural, stream);
+    case 3:
+      return launch_mfma_d<T, 3>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out, ldo,
+                                 natural, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+template bool mfma_supported<float>(bool, int, int, int, int);
+template bool mfma_supported<double>(bool, int, int, int, int);
+template hipError_t launch_mfma<float>(bool, const DevTreeView<float> &,
+                                       const DevTreeView<float> &, const DevPatternView &,
+                                       const InterpParams<float> &, const KernelParams<float> &,
+                                       int, const float *, const float *, float *, double *,
+                                       int64_t, bool, hipStream_t);
+template hipError_t launch_mfma<double>(bool, const DevTreeView<double> &,
+                                        const DevTreeView<double> &, const DevPatternView &,
+                                        const InterpParams<double> &, const KernelParams<double> &,
+                                        int, const double *, const double *, double *, double *,
+                                        int64_t, bool, hipStream_t);
+
+}  // namespace fmca_amd

@@ -26,6 +26,11 @@ CASES = [
     ("matern92-eta04", 3000, None, 2, 3, "MATERN92", 0.6, 0.4, 1),
     ("small-all-dense", 100, None, 3, 4, "EXPONENTIAL", 1.0, 0.8, 2),
     ("rhs-40", 3000, None, 3, 3, "MATERN32", 0.5, 0.8, 40),
+    ("q216-16rhs-two-row-passes", 4000, None, 3, 5, "EXPONENTIAL", 0.5, 0.8, 16),
+    ("q343-9rhs-valu-coupling", 3000, None, 3, 6, "GAUSSIAN", 0.3, 0.8, 9),
+    ("1d-16rhs", 5000, None, 1, 4, "MATERN32", 0.2, 0.8, 16),
+    ("rect-33rhs", 4000, 3000, 3, 3, "MATERN52", 0.4, 0.8, 33),
+    ("2d-65rhs", 6000, None, 2, 4, "EXPONENTIAL", 0.3, 0.8, 65),
 ]
 
 
@@ -59,7 +64,10 @@ def test_fp64_cluster_order_matches_oracle(case):
     assert rel_err(y, yo) <= FP64_TOL
 
 
-@pytest.mark.parametrize("case", CASES[:4], ids=[c[0] for c in CASES[:4]])
+FP32_CASES = CASES[:4] + CASES[-5:]
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=[c[0] for c in FP32_CASES])
 def test_fp32_action_close_to_fp64(case):
     import fmca_amd
     H, A, x, _, _ = _setup(case)
