@@ -1,0 +1,251 @@
+"""Pins the FP64 CPU oracle (FMCA restatement) without importing FMCA.
+
+FMCA cannot be compiled here (Eigen3 is absent), and its tests hold no
+golden vectors for the H2 action, so every oracle function is pinned by
+checks that a plausible mistake would fail: closed forms and independent
+formulas, interpolation identities, tree and pattern invariants, FMCA's own
+nestedness check, a hand-worked tiny tree, and convergence to dense K*x.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+from scipy import special, stats
+
+import oracle
+from conftest import rel_err, uniform_points
+
+KERNELS_WITH_K0_ONE = ["EXPONENTIAL", "MATERN32", "MATERN52", "MATERN72", "MATERN92", "GAUSSIAN"]
+
+
+# --- kernels (CovarianceKernel.h:56-116) -------------------------------------
+def matern_nu(r, l, nu):
+    """Textbook Matern: 2^(1-nu)/Gamma(nu) (sqrt(2 nu) r/l)^nu K_nu(sqrt(2 nu) r/l)."""
+    z = np.sqrt(2 * nu) * r / l
+    with np.errstate(invalid="ignore"):
+        v = 2 ** (1 - nu) / special.gamma(nu) * z ** nu * special.kv(nu, z)
+    return np.where(z == 0, 1.0, v)
+
+
+@pytest.mark.parametrize("name,nu", [("EXPONENTIAL", 0.5), ("MATERN32", 1.5), ("MATERN52", 2.5),
+                                     ("MATERN72", 3.5), ("MATERN92", 4.5)])
+def test_matern_family_matches_bessel_form(name, nu):
+    r = np.concatenate([[0.0], np.linspace(1e-3, 4, 200)])
+    for l in (0.3, 1.0, 2.5):
+        got = oracle.kernel_eval(name, r, l=l)
+        assert np.allclose(got, matern_nu(r, l, nu), rtol=1e-12, atol=1e-15)
+
+
+def test_gaussian_uses_half_convention():
+    r = np.linspace(0, 3, 50)
+    for l in (0.1, 0.5, 2.0):
+        ref = stats.norm.pdf(r, scale=l) / stats.norm.pdf(0, scale=l)
+        assert np.allclose(oracle.kernel_eval("GAUSSIAN", r, l=l), ref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("name", KERNELS_WITH_K0_ONE)
+def test_kernel_at_zero_is_one(name):
+    assert oracle.kernel_eval(name, [0.0], l=0.7)[0] == 1.0
+
+
+def test_other_kernels_hand_values():
+    # INVMULTIQUADRIC 1/sqrt((r/l)^2 + c^2), MULTIQUADRIC sqrt(.), TPS2D r^2 log r
+    assert oracle.kernel_eval("INVMULTIQUADRIC", [3.0], l=1.0, c=4.0)[0] == pytest.approx(0.2)
+    assert oracle.kernel_eval("MULTIQUADRIC", [6.0], l=2.0, c=4.0)[0] == pytest.approx(5.0)
+    assert oracle.kernel_eval("TPS2D", [math.e], l=1.0)[0] == pytest.approx(math.e ** 2)
+    assert oracle.kernel_eval("TPS2D", [0.0])[0] == 0.0
+    assert oracle.kernel_eval("BIHARMONIC3D", [3.0], l=2.0)[0] == pytest.approx(1.5)
+    assert oracle.kernel_eval("TRIHARMONIC3D", [3.0], l=1.5)[0] == pytest.approx(8.0)
+    assert oracle.kernel_eval("BIHARMONIC2D", [2 * math.e], l=2.0)[0] == pytest.approx(math.e ** 2)
+    assert oracle.kernel_eval("BIHARMONIC2D", [0.0])[0] == 0.0
+    with pytest.raises(ValueError):
+        oracle.kernel_id("MATERNNU")
+
+
+# --- interpolation (TensorProductInterpolator.h:39-81, MultiIndexSet.h) ------
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4, 5, 8])
+def test_chebyshev_nodes_are_roots_and_descending(deg):
+    xi, w, _, _ = oracle.interp(1, deg)
+    # roots of T_{deg+1}(2x - 1) on [0,1], strictly decreasing
+    T = np.polynomial.chebyshev.Chebyshev.basis(deg + 1)
+    assert np.max(np.abs(T(2 * xi - 1))) < 1e-12
+    assert np.all(np.diff(xi) < 0) and xi.min() > 0 and xi.max() < 1
+    # barycentric weights alternate in sign and are symmetric in magnitude
+    assert np.all(np.sign(w[:-1]) != np.sign(w[1:])) if deg > 0 else True
+    assert np.allclose(np.abs(w), np.abs(w[::-1]))
+
+
+@pytest.mark.parametrize("dim,deg", [(1, 3), (2, 2), (2, 4), (3, 3)])
+def test_multi_index_order_graded_lex(dim, deg):
+    _, _, mi, Xi = oracle.interp(dim, deg)
+    allidx = sorted(itertools.product(range(deg + 1), repeat=dim), key=lambda a: (sum(a), a))
+    assert [tuple(r) for r in mi] == allidx
+    xi, _, _, _ = oracle.interp(1, deg)
+    assert np.array_equal(Xi, xi[mi.T])
+
+
+@pytest.mark.parametrize("dim,deg", [(1, 5), (2, 3), (3, 2), (3, 4)])
+def test_lagrange_delta_partition_and_reproduction(dim, deg):
+    _, _, mi, Xi = oracle.interp(dim, deg)
+    q = Xi.shape[1]
+    # delta property at the nodes
+    assert np.allclose(oracle.eval_polynomials(dim, deg, Xi), np.eye(q), atol=1e-13)
+    x = np.random.default_rng(0).random((dim, 200))
+    L = oracle.eval_polynomials(dim, deg, x)
+    assert np.allclose(L.sum(axis=0), 1.0, atol=1e-12)
+    # reproduction of a random polynomial of per-axis degree <= deg
+    coef = np.random.default_rng(1).standard_normal(q)
+
+    def f(pts):
+        return sum(c * np.prod(pts ** a[:, None], axis=0) for c, a in zip(coef, mi))
+    assert np.allclose(f(Xi) @ L, f(x), atol=1e-10 * np.abs(coef).sum())
+
+
+# --- cluster tree (ClusterTreeInitializer_ClusterTree.h, ClusterTreeSplitter.h)
+def test_tiny_tree_by_hand():
+    P = np.array([[0.9, 0.1, 0.5, 0.3, 0.7, 0.2, 0.8, 0.4]])
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 1.0, 1, 0.8)  # q = 2: leaves of <= 3 points
+    assert A.q == 2
+    assert A.indices(1).tolist() == [1, 5, 3, 7, 2, 4, 6, 0]
+    cl = A.clusters(1)
+    assert cl["size"].tolist() == [8, 4, 4, 2, 2, 2, 2]
+    assert cl["begin"].tolist() == [0, 0, 4, 0, 2, 4, 6]
+    assert cl["level"].tolist() == [0, 1, 1, 2, 2, 2, 2]
+    assert cl["sons"].tolist() == [[1, 2], [3, 4], [5, 6]] + [[-1, -1]] * 4
+    # tight boxes after shrink-to-fit
+    assert cl["bbmin"][:, 0].tolist() == [0.1, 0.1, 0.5, 0.1, 0.3, 0.5, 0.8]
+    assert cl["bbmax"][:, 0].tolist() == [0.9, 0.4, 0.9, 0.2, 0.4, 0.7, 0.9]
+
+
+def _check_tree(A, which, P, q):
+    n = P.shape[1]
+    perm = A.indices(which)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    cl = A.clusters(which)
+    nc = len(cl["size"])
+    assert cl["size"][0] == n and cl["begin"][0] == 0
+    assert np.all(np.diff(cl["level"]) >= 0)  # level order
+    for c in range(nc):
+        b, s = cl["begin"][c], cl["size"][c]
+        pts = P[:, perm[b:b + s]]
+        s0, s1 = cl["sons"][c]
+        if s0 < 0:
+            assert s <= 2 * q - 1 and (s >= q or c == 0)
+            lo, hi = pts.min(axis=1), pts.max(axis=1)
+        else:
+            assert s0 > c and s1 == s0 + 1
+            assert cl["begin"][s0] == b and cl["size"][s0] == s // 2
+            assert cl["begin"][s1] == b + s // 2 and cl["size"][s1] == s - s // 2
+            p0 = P[:, perm[b:b + s // 2]]
+            p1 = P[:, perm[b + s // 2:b + s]]
+            # some axis separates the halves (the cardinality split)
+            assert np.any(p0.max(axis=1) <= p1.min(axis=1))
+            lo = np.minimum(cl["bbmin"][s0], cl["bbmin"][s1])
+            hi = np.maximum(cl["bbmax"][s0], cl["bbmax"][s1])
+        flat = hi - lo < 2e2 * np.finfo(float).eps
+        lo = np.where(flat, lo - 1e2 * np.finfo(float).eps, lo)
+        hi = np.where(flat, hi + 1e2 * np.finfo(float).eps, hi)
+        assert np.array_equal(cl["bbmin"][c], lo) and np.array_equal(cl["bbmax"][c], hi)
+        assert np.array_equal(cl["bbsize"][c], hi - lo)
+
+
+@pytest.mark.parametrize("n,d,deg", [(5000, 3, 3), (3000, 2, 4), (2000, 1, 5), (7, 3, 2)])
+def test_tree_invariants(n, d, deg):
+    P = uniform_points(n, d, seed=n)
+    A = oracle.OracleH2(P, P, "MATERN32", 0.5, deg, 0.8)
+    _check_tree(A, 1, P, A.q)
+
+
+def test_tree_with_duplicates_and_flat_axis():
+    rng = np.random.default_rng(3)
+    P = rng.random((3, 1500))
+    P[:, 700:900] = P[:, 700:701]   # 200 copies of one point
+    P[2, :] = 0.25                  # a flat axis
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 2, 0.8)
+    _check_tree(A, 1, P, A.q)
+
+
+@pytest.mark.parametrize("d,deg", [(3, 3), (2, 5), (3, 4), (1, 6)])
+def test_transfer_nestedness(d, deg):
+    """FMCA's check_transfer_matrices KAT (compute_cluster_bases_impl.h:59-80)."""
+    P = uniform_points(4000, d, seed=7)
+    A = oracle.OracleH2(P, uniform_points(3000, d, seed=8), "EXPONENTIAL", 0.5, deg, 0.8)
+    assert A.check_transfer(0) < 1e-13
+    assert A.check_transfer(1) < 1e-13
+
+
+# --- block pattern (H2MatrixBase.h:160-191, Admissibility.h:95-115) -----------
+def test_pattern_covers_every_pair_once_and_admissible():
+    P = uniform_points(1500, 2, seed=2)
+    Pe = uniform_points(1200, 2, seed=3)
+    eta = 0.6
+    A = oracle.OracleH2(P, Pe, "EXPONENTIAL", 0.5, 2, eta)
+    rows, cols = A.clusters(0), A.clusters(1)
+    r, c, lr = A.blocks()
+    cover = np.zeros((1200, 1500), dtype=np.int32)
+    for t, s, low in zip(r, c, lr):
+        cover[rows["begin"][t]:rows["begin"][t] + rows["size"][t],
+              cols["begin"][s]:cols["begin"][s] + cols["size"][s]] += 1
+        rt = 0.5 * np.linalg.norm(rows["bbsize"][t])
+        rs = 0.5 * np.linalg.norm(cols["bbsize"][s])
+        ct = 0.5 * (rows["bbmin"][t] + rows["bbmax"][t])
+        cs = 0.5 * (cols["bbmin"][s] + cols["bbmax"][s])
+        dist = max(np.linalg.norm(ct - cs) - rt - rs, 0.0)
+        if low:
+            assert max(rt, rs) <= eta * dist
+            assert A.q ** 2 < rows["size"][t] * cols["size"][s]
+        else:
+            leaf = rows["sons"][t][0] < 0 or cols["sons"][s][0] < 0
+            assert leaf or max(rt, rs) <= eta * dist
+    assert np.all(cover == 1)
+
+
+# --- whole action --------------------------------------------------------------
+def test_action_error_vs_dense_decreases_with_degree_and_eta():
+    P = uniform_points(3000, 3, seed=4)
+    x = np.random.default_rng(5).uniform(-1, 1, (3000, 2))
+    yd = oracle.dense_action(P, P, "EXPONENTIAL", 0.5, x)
+    errs = [rel_err(oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, deg, 0.8).action(x), yd)
+            for deg in (1, 2, 3, 4)]
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 1e-4
+    errs_eta = [rel_err(oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 2, eta).action(x), yd)
+                for eta in (0.8, 0.4, 0.2)]
+    assert errs_eta[2] < errs_eta[0], errs_eta
+
+
+def test_action_forms_agree():
+    P = uniform_points(2500, 2, seed=6)
+    Pe = uniform_points(1800, 2, seed=9)
+    A = oracle.OracleH2(P, Pe, "MATERN32", 0.4, 3, 0.8)
+    x = np.random.default_rng(1).standard_normal((2500, 3))
+    y = A.action(x)
+    ci, ri = A.indices(1), A.indices(0)
+    yc = A.action_cluster_order(x[ci])
+    assert np.allclose(yc, y[ri], rtol=0, atol=1e-13 * np.abs(y).max())
+    rows = A.clusters(0)
+    leaves = np.flatnonzero(rows["sons"][:, 0] < 0)[[0, 3, -1]]
+    yl = A.action_leaves(x, leaves)
+    sel = np.concatenate([ri[rows["begin"][t]:rows["begin"][t] + rows["size"][t]] for t in leaves])
+    assert np.allclose(yl, y[sel], rtol=0, atol=1e-13 * np.abs(y).max())
+
+
+def test_action_linearity_and_symmetry():
+    P = uniform_points(2000, 3, seed=10)
+    A = oracle.OracleH2(P, P, "MATERN52", 0.5, 3, 0.8)
+    rng = np.random.default_rng(2)
+    x, z = rng.standard_normal((2000, 1)), rng.standard_normal((2000, 1))
+    Ax, Az = A.action(x), A.action(z)
+    assert rel_err(A.action(x + 2 * z), Ax + 2 * Az) < 1e-14
+    # same tree for rows and columns: the H2 matrix is symmetric
+    assert abs((z.T @ Ax - x.T @ Az).item()) < 1e-12 * abs((z.T @ Ax).item())
+
+
+def test_pcg_solves_regularised_system():
+    P = uniform_points(1500, 2, seed=12)
+    A = oracle.OracleH2(P, P, "GAUSSIAN", 0.2, 4, 0.8)
+    b = np.random.default_rng(3).standard_normal(1500)
+    x, it, err = A.pcg(b, ridge=1e-2, tol=1e-8)
+    r = b - A.action(x)[:, 0] - 1e-2 * x
+    assert err <= 1e-8 and np.linalg.norm(r) / np.linalg.norm(b) <= 1.1e-8 and it > 1

@@ -1,0 +1,157 @@
+"""Host-side product checks that need no GPU: the planner reproduces the
+oracle's (FMCA's) tree, boxes and block pattern exactly, the C-ABI library
+exports everything include/fmca_amd.h declares, and invalid input is reported
+through status codes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, gpu_available, uniform_points
+
+import fmca_amd
+
+LIB = os.path.join(ROOT, "fmca_amd", "libfmca_amd.so")
+HEADER = os.path.join(ROOT, "include", "fmca_amd.h")
+
+
+def _block_set(r, c, lr):
+    return set(zip(np.asarray(r).tolist(), np.asarray(c).tolist(), np.asarray(lr).astype(bool).tolist()))
+
+
+CASES = [
+    (10000, None, 3, 3, 0.8),
+    (20000, None, 2, 5, 0.8),
+    (6000, 4500, 3, 4, 0.8),
+    (3000, None, 1, 4, 0.5),
+    (5000, 2000, 2, 2, 0.3),
+    (100, None, 3, 4, 0.8),
+]
+
+
+@pytest.mark.parametrize("n,m,d,deg,eta", CASES)
+def test_planner_matches_oracle(n, m, d, deg, eta):
+    P = uniform_points(n, d, seed=n + d)
+    Pe = P if m is None else uniform_points(m, d, seed=m + 1)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), Pe, P, deg, eta)
+    A = oracle.OracleH2(P, Pe, "MATERN32", 0.5, deg, eta)
+    for which in (0, 1):
+        assert np.array_equal(H.indices(which), A.indices(which))
+        cp, co = H.clusters(which), A.clusters(which)
+        for k in ("begin", "size", "level", "sons", "bbmin", "bbmax", "bbsize"):
+            assert np.array_equal(cp[k], co[k]), k
+    assert _block_set(*H.blocks()) == _block_set(*A.blocks())
+    info = H.info()
+    counts = A.entry_counts()
+    assert info["near_entries"] == counts["near_entries"]
+    assert info["far_entries"] == counts["far_entries"]
+    assert info["far_blocks"] == counts["far_blocks"] and info["near_blocks"] == counts["near_blocks"]
+
+
+def test_planner_ties_and_duplicates():
+    rng = np.random.default_rng(11)
+    P = np.round(rng.random((3, 6000)) * 40) / 40   # many equal coordinates
+    P[:, 100:400] = P[:, 50:51]
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("EXPONENTIAL", 0.3), P, P, 3, 0.8)
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.3, 3, 0.8)
+    assert np.array_equal(H.indices(1), A.indices(1))
+    assert _block_set(*H.blocks()) == _block_set(*A.blocks())
+
+
+def test_statistics_semantics():
+    """H2MatrixBase::statistics (H2MatrixBase.h:70-105)."""
+    P = uniform_points(8000, 3, seed=3)
+    H = fmca_amd.H2Matrix(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 3, 0.8)
+    st = H.statistics().ravel()
+    r, c, lr = H.blocks()
+    cl = H.clusters(0)
+    mem = lr.sum() * 64 ** 2 + sum(cl["size"][t] * cl["size"][s] for t, s, l in zip(r, c, lr) if not l)
+    assert st[0] == 8000 and st[1] == 8000
+    assert st[2] == lr.sum() and st[3] == (~lr).sum()
+    assert st[4] == round(mem / 8000) and st[5] == pytest.approx(mem * 8 / 1e9)
+
+
+def test_c_abi_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    text = open(HEADER).read()
+    names = re.findall(r"^(?:fmca_status|const char \*)\s*(fmca_\w+)\s*\(", text, re.M)
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    lib.fmca_status_string.restype = ctypes.c_char_p
+    assert lib.fmca_status_string(0) == b"ok"
+    assert lib.fmca_status_string(2) == b"unknown kernel type"
+
+
+def test_c_abi_create_plan_without_device():
+    """The plan is host-side; create / queries / destroy work with plain pointers."""
+    lib = ctypes.CDLL(LIB)
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("ktype", ctypes.c_char_p), ("l", ctypes.c_double), ("c", ctypes.c_double),
+                    ("nu", ctypes.c_double)]
+    P = uniform_points(3000, 3, seed=1)
+    h = ctypes.c_void_p()
+    desc = Desc(b"matern32", 0.5, 1.0, 1.0)
+    ptr = P.ctypes.data_as(ctypes.c_void_p)
+    st = lib.fmca_h2_create(ctypes.byref(desc), 3, ctypes.c_int64(3000), ptr, ctypes.c_int64(3000),
+                            ptr, 3, ctypes.c_double(0.8), ctypes.byref(h))
+    assert st == 0 and h.value
+    info = (ctypes.c_double * 13)()
+    assert lib.fmca_h2_info(h, info) == 0
+    assert info[0] == 3 and info[3] == 64 and info[12] == 1.0
+    perm = np.empty(3000, dtype=np.uint32)
+    assert lib.fmca_h2_indices(h, 0, perm.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.array_equal(np.sort(perm), np.arange(3000))
+    assert lib.fmca_h2_destroy(h) == 0
+    # status codes instead of FMCA's asserts
+    bad = Desc(b"MATERNNU", 0.5, 1.0, 1.0)
+    assert lib.fmca_h2_create(ctypes.byref(bad), 3, ctypes.c_int64(3000), ptr,
+                              ctypes.c_int64(3000), ptr, 3, ctypes.c_double(0.8),
+                              ctypes.byref(h)) == 2
+    assert lib.fmca_h2_create(ctypes.byref(desc), 4, ctypes.c_int64(10), ptr, ctypes.c_int64(10),
+                              ptr, 3, ctypes.c_double(0.8), ctypes.byref(h)) == 3
+
+
+def test_python_errors():
+    P = uniform_points(500, 2, seed=1)
+    K = fmca_amd.CovarianceKernel("GAUSSIAN", 0.3)
+    with pytest.raises(ValueError):
+        fmca_amd.CovarianceKernel("NOPE")
+    with pytest.raises(TypeError):
+        fmca_amd.H2Matrix(K, P.astype(np.float32), P)
+    with pytest.raises(ValueError):
+        fmca_amd.H2Matrix(K, P, P, 10, 0.8)      # degree > 9
+    Q = P.copy()
+    Q[0, 3] = np.nan
+    with pytest.raises(ValueError):
+        fmca_amd.H2Matrix(K, Q, Q, 3, 0.8)
+    with pytest.raises(ValueError):
+        fmca_amd.H2Matrix(fmca_amd.CovarianceKernel("GAUSSIAN", -1.0), P, P, 3, 0.8)
+    H = fmca_amd.H2Matrix(K, P, P, 3, 0.8)
+    with pytest.raises(TypeError):
+        H.action(np.ones(500, dtype=np.float32))   # FMCA's noconvert(): float64 only
+    with pytest.raises(ValueError):
+        H.action(np.ones(499))
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device status")
+def test_action_without_device_reports_status():
+    P = uniform_points(500, 2, seed=1)
+    H = fmca_amd.H2Matrix(fmca_amd.CovarianceKernel("GAUSSIAN", 0.3), P, P, 3, 0.8)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        H.action(np.ones(500))
+
+
+def test_moment_order_table_matches_fmca():
+    """The GPU keeps moments in tensor order; FMCA's graded-lex order maps onto it."""
+    from itertools import product
+    for d, deg in [(2, 3), (3, 4)]:
+        _, _, mi, _ = oracle.interp(d, deg)
+        P1 = deg + 1
+        lin = [sum(a[i] * P1 ** i for i in range(d)) for a in mi]
+        assert sorted(lin) == list(range(P1 ** d))
+        assert lin[0] == 0 and lin[1:d + 1] == sorted(P1 ** i for i in range(d))[::-1]
