@@ -137,6 +137,37 @@ class H2kernelActor:
                                         precision, s.cuda_stream)
         return out if rhs.dim() == 2 else out[:, 0]
 
+    def set_kernel(self, kernel: CovarianceKernel):
+        """Swap the kernel (type and parameters); the plan is kept."""
+        self._a.set_kernel(kernel.kernelType(), kernel.l, kernel.c, kernel.nu)
+        self.kernel = kernel
+
+    # -- solver ----------------------------------------------------------------
+    def cg(self, b, x0=None, ridge: float = 0.0, tol: float = 1e-8, precision: int = FP32,
+           refine_factor: float = 0.1, max_iter: int = 100000, max_outer: int = 1000,
+           stream=None):
+        """Solve (K + ridge I) x = b by CG (FMCA's pCG, PCG.h:15-44, identity
+        preconditioner) for every column of b.
+
+        precision=FP64: plain CG.  precision=FP32: CG with reliable updates --
+        FP32 matvecs drive the recurrence, FP64 vectors and inner products, and
+        the iterated residual is replaced by the FP64 residual each time it has
+        dropped by refine_factor.  b, x0: float64 CUDA tensors (n,) or
+        (n, nrhs); x0 defaults to b as in pCG.  Returns (x, info) with
+        info['rel_residual'] measured with the FP64 action.
+        """
+        import torch
+        bm = _as_device_matrix(b, self.n)
+        nrhs = bm.shape[1]
+        x = torch.empty((nrhs, self.n), dtype=torch.float64, device=bm.device).t()
+        x.copy_(bm if x0 is None else _as_device_matrix(x0, self.n))
+        s = stream if stream is not None else torch.cuda.current_stream(bm.device)
+        info = self._a.cg_device(bm.data_ptr(), bm.stride(1) if nrhs > 1 else self.n, nrhs,
+                                 x.data_ptr(), x.stride(1) if nrhs > 1 else self.n, float(ridge),
+                                 float(tol), int(precision), float(refine_factor), int(max_iter),
+                                 int(max_outer), s.cuda_stream)
+        return (x if b.dim() == 2 else x[:, 0]), info
+
     # -- plan queries ---------------------------------------------------------
     def upload(self):
         self._a.upload()

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -135,6 +136,7 @@ struct fmca_h2_actor {
   Workspace<double> ws64;
   Workspace<float> ws32;
   DevBuf stage_in, stage_out;
+  DevBuf cg_ws, cg_scal;  // CG vectors and per-column scalars
   hipStream_t stream = nullptr;
   std::mutex mtx;
   // per-phase profiling (events recorded on the launch stream)
@@ -249,9 +251,9 @@ InterpParams<T> interp_params(const Interpolation &ip) {
 }
 
 template <typename T>
-KernelParams<T> kernel_params(const fmca_h2_actor *a) {
+KernelParams<T> kernel_params(const fmca_h2_actor *a, int variant) {
   KernelParams<T> k;
-  k.kt = a->kt;
+  k.kt = a->kt + KT_COUNT * variant;
   k.l = (T)a->l;
   k.c = (T)a->c;
   k.nu = (T)a->nu;
@@ -284,7 +286,7 @@ template <typename T>
 fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r_rows,
                     const DevTreeReals<T> &r_cols, const double *d_rhs, int64_t ld_rhs,
                     int64_t nrhs, double *d_out, int64_t ld_out, bool natural, bool transpose,
-                    hipStream_t stream) {
+                    hipStream_t stream, int variant) {
   const Tree &tr_rows = a->rows();
   const Tree &tr_cols = a->cols();
   const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
@@ -316,14 +318,14 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
     pe = &a->pending.back();
   }
   hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
-                                  interp_params<T>(a->ip), kernel_params<T>(a), d_rhs, ld_rhs,
+                                  interp_params<T>(a->ip), kernel_params<T>(a, variant), d_rhs, ld_rhs,
                                   (int)nrhs, d_out, ld_out, natural, buf, stream, pe);
   return from_hip(e, "launch_action");
 }
 
 fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
                         double *d_out, int64_t ld_out, int precision, bool natural,
-                        bool transpose, hipStream_t stream) {
+                        bool transpose, hipStream_t stream, int variant = KV_VALUE) {
   if (transpose && !a->shared && !a->have_pat_t) {
     compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
     HIP_TRY(a->d_pat_t.upload(a->pat_t));
@@ -331,10 +333,12 @@ fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, i
   }
   if (precision == FMCA_PRECISION_FP64)
     return enqueue<double>(a, a->ws64, a->d_rows_64, a->shared ? a->d_rows_64 : a->d_cols_64,
-                           d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream);
+                           d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
+                           variant);
   if (precision == FMCA_PRECISION_FP32)
     return enqueue<float>(a, a->ws32, a->d_rows_32, a->shared ? a->d_rows_32 : a->d_cols_32,
-                          d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream);
+                          d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
+                          variant);
   return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
 }
 
@@ -499,6 +503,39 @@ fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *
                      (hipStream_t)stream);
 }
 
+fmca_status fmca_h2_set_kernel(fmca_h2_actor *actor, const fmca_kernel_desc *kernel) {
+  if (!actor || !kernel || !kernel->ktype) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  const int kt = parse_kernel_type(kernel->ktype);
+  if (kt < 0) return fail(FMCA_ERR_UNKNOWN_KERNEL, std::string("kernel ") + kernel->ktype);
+  if (!(kernel->l > 0) || !std::isfinite(kernel->l))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "lengthscale must be positive");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  actor->kt = kt;
+  actor->l = kernel->l;
+  actor->c = kernel->c;
+  actor->nu = kernel->nu;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_action_variant_device(fmca_h2_actor *actor, int variant, int transpose,
+                                          const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
+                                          double *d_out, int64_t ld_out, int precision,
+                                          void *stream) {
+  if (!actor || nrhs < 0 || (nrhs && (!d_rhs || !d_out)))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or negative nrhs");
+  if (variant < KV_VALUE || variant > KV_PHI) return fail(FMCA_ERR_INVALID_ARGUMENT, "variant");
+  const int64_t rows_in = transpose ? actor->m : actor->n;
+  const int64_t rows_out = transpose ? actor->n : actor->m;
+  if (ld_rhs < rows_in || ld_out < rows_out)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (nrhs == 0) return FMCA_OK;
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  fmca_status st = ensure_device(actor);
+  if (st != FMCA_OK) return st;
+  return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, true, transpose != 0,
+                     (hipStream_t)stream, variant);
+}
+
 fmca_status fmca_h2_indices(const fmca_h2_actor *actor, int which, uint32_t *out) {
   if (!actor || !out) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
   const Tree &T = which == FMCA_ROWS ? actor->rows() : actor->cols();
@@ -567,6 +604,160 @@ fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *co
   std::copy(p.block_col.begin(), p.block_col.end(), col);
   std::copy(p.block_lr.begin(), p.block_lr.end(), low_rank);
   return FMCA_OK;
+}
+
+fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b, int64_t nrhs,
+                              double *d_x, int64_t ld_x, const fmca_cg_options *opts,
+                              fmca_cg_result *res, void *stream_ptr) {
+  if (!a || !opts || !res || nrhs < 1 || !d_b || !d_x)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or nrhs < 1");
+  if (a->m != a->n) return fail(FMCA_ERR_INVALID_ARGUMENT, "CG needs a square kernel matrix");
+  if (ld_b < a->n || ld_x < a->n) return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (nrhs > 65535) return fail(FMCA_ERR_NOT_SUPPORTED, "more than 65535 right-hand sides");
+  if (opts->precision != FMCA_PRECISION_FP64 && opts->precision != FMCA_PRECISION_FP32)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (!(opts->tol > 0) || !(opts->ridge >= 0))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "tol must be > 0 and ridge >= 0");
+  std::lock_guard<std::mutex> lock(a->mtx);
+  fmca_status st = ensure_device(a);
+  if (st != FMCA_OK) return st;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t sm = (hipStream_t)stream_ptr;
+  const int64_t n = a->n;
+  const int nr = (int)nrhs;
+  const double ridge = opts->ridge;
+  // Cluster order avoids a permutation per matvec; it needs one tree for both sides.
+  const bool cl = a->shared;
+  const uint32_t *perm = a->d_rows_i.perm.as<uint32_t>();
+  const size_t vec = (size_t)n * nr;
+  HIP_TRY(a->cg_ws.alloc(8 * vec * sizeof(double)));
+  HIP_TRY(a->cg_scal.alloc((8 * (size_t)nr) * sizeof(double) + blas1_dot_scratch(nr)));
+  double *w = a->cg_ws.as<double>();
+  double *B = w, *X = w + vec, *R = w + 2 * vec, *Dv = w + 3 * vec, *RR = w + 4 * vec,
+         *Pv = w + 5 * vec, *Q = w + 6 * vec, *Y = w + 7 * vec;
+  double *sc = a->cg_scal.as<double>();
+  double *d_dot = sc, *d_alpha = sc + nr, *d_beta = sc + 2 * nr, *d_one = sc + 3 * nr;
+  double *d_scratch = sc + 8 * nr;
+  std::vector<double> h(4 * (size_t)nr);
+  double *h_dot = h.data(), *h_alpha = h.data() + nr, *h_beta = h.data() + 2 * nr,
+         *h_one = h.data() + 3 * nr;
+  for (int j = 0; j < nr; ++j) h_one[j] = 1.0;
+  auto chk = [&](hipError_t e, const char *what) { return from_hip(e, what); };
+#define CG_TRY(expr)                                  \
+  do {                                                \
+    fmca_status _s = chk((expr), #expr);              \
+    if (_s != FMCA_OK) return _s;                     \
+  } while (0)
+  auto dot = [&](const double *u, const double *v, double *hout) -> fmca_status {
+    CG_TRY(blas1_dot(n, nr, u, n, v, n, d_dot, d_scratch, sm));
+    CG_TRY(hipMemcpyAsync(hout, d_dot, nr * sizeof(double), hipMemcpyDeviceToHost, sm));
+    CG_TRY(hipStreamSynchronize(sm));
+    return FMCA_OK;
+  };
+  auto upload = [&](double *dst, const double *src) -> hipError_t {
+    return hipMemcpyAsync(dst, src, nr * sizeof(double), hipMemcpyHostToDevice, sm);
+  };
+  auto matvec = [&](const double *in, double *out, int prec) -> fmca_status {
+    if (prec == FMCA_PRECISION_FP64) res->matvecs_fp64++; else res->matvecs_fp32++;
+    return enqueue_any(a, in, n, nr, out, n, prec, !cl, false, sm);
+  };
+  *res = fmca_cg_result{};
+  CG_TRY(upload(d_one, h_one));
+  if (cl) {
+    CG_TRY(blas1_permute(n, nr, perm, true, d_b, ld_b, B, n, sm));
+    CG_TRY(blas1_permute(n, nr, perm, true, d_x, ld_x, X, n, sm));
+  } else {
+    CG_TRY(hipMemcpy2DAsync(B, n * sizeof(double), d_b, ld_b * sizeof(double), n * sizeof(double),
+                            nr, hipMemcpyDeviceToDevice, sm));
+    CG_TRY(hipMemcpy2DAsync(X, n * sizeof(double), d_x, ld_x * sizeof(double), n * sizeof(double),
+                            nr, hipMemcpyDeviceToDevice, sm));
+  }
+  std::vector<double> bn2(nr), rn2(nr), rho(nr), target(nr);
+  (void)Dv;
+  (void)RR;
+  if ((st = dot(B, B, bn2.data())) != FMCA_OK) return st;
+  for (int j = 0; j < nr; ++j)
+    if (bn2[j] == 0) bn2[j] = 1;  // zero rhs: absolute residual
+  // r = b - (K + ridge I) x, FP64
+  if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+  CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+  // CG with reliable updates: the recurrence runs with matvecs of the chosen
+  // precision (FP64 vectors, inner products and updates throughout); whenever
+  // a column's iterated residual has dropped by refine_factor since the last
+  // replacement, or reaches the target, the true residual b - (K + ridge I) x
+  // is recomputed with the FP64 action and replaces it, keeping the search
+  // direction.  With FP64 matvecs only the final check replaces the residual.
+  const int prec = opts->precision;
+  const int max_iter = opts->max_iter > 0 ? opts->max_iter : 100000;
+  const int max_updates = opts->max_outer > 0 ? opts->max_outer : 1000;
+  const double delta = (opts->refine_factor > 0 && opts->refine_factor < 1) ? opts->refine_factor
+                                                                              : 0.1;
+  if ((st = dot(R, R, rn2.data())) != FMCA_OK) return st;
+  std::vector<char> active(nr);
+  std::vector<double> rmax(nr);
+  bool any = false;
+  double worst = 0;
+  for (int j = 0; j < nr; ++j) {
+    rho[j] = rn2[j];
+    rmax[j] = std::sqrt(rn2[j]);
+    target[j] = opts->tol * std::sqrt(bn2[j]);
+    active[j] = rmax[j] > target[j];
+    any = any || active[j];
+    worst = std::max(worst, rmax[j] / std::sqrt(bn2[j]));
+  }
+  res->rel_residual = worst;
+  CG_TRY(hipMemcpyAsync(Pv, R, vec * sizeof(double), hipMemcpyDeviceToDevice, sm));
+  while (any && res->iterations < max_iter && res->outer < max_updates) {
+    if ((st = matvec(Pv, Q, prec)) != FMCA_OK) return st;
+    CG_TRY(blas1_add_ridge(n, nr, ridge, Pv, n, Q, n, sm));
+    if ((st = dot(Pv, Q, h_dot)) != FMCA_OK) return st;
+    for (int j = 0; j < nr; ++j) h_alpha[j] = active[j] && h_dot[j] != 0 ? rho[j] / h_dot[j] : 0.0;
+    CG_TRY(upload(d_alpha, h_alpha));
+    CG_TRY(blas1_axpy(n, nr, 1.0, d_alpha, Pv, n, X, n, sm));
+    CG_TRY(blas1_axpy(n, nr, -1.0, d_alpha, Q, n, R, n, sm));
+    if ((st = dot(R, R, rn2.data())) != FMCA_OK) return st;
+    res->iterations++;
+    bool update = false;
+    for (int j = 0; j < nr; ++j) {
+      if (!active[j]) continue;
+      const double rn = std::sqrt(rn2[j]);
+      if (!std::isfinite(rn)) return fail(FMCA_ERR_NONFINITE, "CG residual is not finite");
+      if (rn <= target[j] || (prec != FMCA_PRECISION_FP64 && rn < delta * rmax[j])) update = true;
+    }
+    if (update) {
+      if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+      CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+      if ((st = dot(R, R, rn2.data())) != FMCA_OK) return st;
+      res->outer++;
+      worst = 0;
+      for (int j = 0; j < nr; ++j) {
+        const double rn = std::sqrt(rn2[j]);
+        rmax[j] = rn;
+        worst = std::max(worst, rn / std::sqrt(bn2[j]));
+        if (active[j] && rn <= target[j]) active[j] = 0;
+      }
+      res->rel_residual = worst;
+    }
+    any = false;
+    for (int j = 0; j < nr; ++j) {
+      h_beta[j] = active[j] && rho[j] != 0 ? rn2[j] / rho[j] : 0.0;
+      if (active[j]) rho[j] = rn2[j];
+      any = any || active[j];
+    }
+    if (!any) break;
+    CG_TRY(upload(d_beta, h_beta));
+    CG_TRY(blas1_xpby(n, nr, d_beta, R, n, Pv, n, sm));
+  }
+  res->converged = !any;
+  if (cl)
+    CG_TRY(blas1_permute(n, nr, perm, false, X, n, d_x, ld_x, sm));
+  else
+    CG_TRY(hipMemcpy2DAsync(d_x, ld_x * sizeof(double), X, n * sizeof(double), n * sizeof(double),
+                            nr, hipMemcpyDeviceToDevice, sm));
+  CG_TRY(hipStreamSynchronize(sm));
+#undef CG_TRY
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res->converged ? FMCA_OK : fail(FMCA_ERR_NOT_CONVERGED, "CG did not reach tol");
 }
 
 fmca_status fmca_h2_set_profiling(fmca_h2_actor *actor, int enable) {

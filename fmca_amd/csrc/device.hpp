@@ -51,6 +51,10 @@ struct InterpParams {
 // division is evaluated per kernel entry:
 //   EXPONENTIAL s = 1/l, MATERN32 s = sqrt(3)/l, MATERN52 s = sqrt(5)/l,
 //   MATERN72 s = sqrt(7)/l, MATERN92 s = 3/l, GAUSSIAN s = -1/(2 l^2) (on r^2).
+//
+// kt >= KT_COUNT selects a derivative of kernel kt % KT_COUNT (variant
+// kt / KT_COUNT): 1 = dk/dl, 2 = phi(r) = k'(r) / r (the radial factor of the
+// gradient with respect to a point, phi(0) = 0 where k' / r is singular).
 template <typename T>
 struct KernelParams {
   int kt = 0;
@@ -59,6 +63,8 @@ struct KernelParams {
   T inv_l = 1;  // 1 / l
   T c2 = 1;     // c * c
 };
+
+enum KernelVariant : int { KV_VALUE = 0, KV_DL = 1, KV_PHI = 2 };
 
 // Level structure of a tree on the host (drives the per-level launches).
 struct HostLevels {
@@ -114,6 +120,24 @@ hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeVie
                        const DevPatternView &pat, const InterpParams<T> &ip,
                        const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom, T *tlhs,
                        double *out, int64_t ldo, bool natural, hipStream_t stream);
+
+// Column-batched FP64 vector kernels (blas1.hip); columns of n x nrhs arrays.
+size_t blas1_dot_scratch(int nrhs);
+hipError_t blas1_dot(int64_t n, int nrhs, const double *a, int64_t lda, const double *b,
+                     int64_t ldb, double *out, double *scratch, hipStream_t stream);
+hipError_t blas1_axpy(int64_t n, int nrhs, double s, const double *coef, const double *x,
+                      int64_t ldx, double *y, int64_t ldy, hipStream_t stream);
+hipError_t blas1_xpby(int64_t n, int nrhs, const double *coef, const double *z, int64_t ldz,
+                      double *p, int64_t ldp, hipStream_t stream);
+hipError_t blas1_residual(int64_t n, int nrhs, double ridge, const double *b, int64_t ldb,
+                          const double *y, int64_t ldy, const double *x, int64_t ldx, double *r,
+                          int64_t ldr, hipStream_t stream);
+hipError_t blas1_add_ridge(int64_t n, int nrhs, double ridge, const double *x, int64_t ldx,
+                           double *y, int64_t ldy, hipStream_t stream);
+hipError_t blas1_fill(int64_t n, int nrhs, double v, double *x, int64_t ldx, hipStream_t stream);
+// out[i, j] = in[perm[i], j] (gather = true) or out[perm[i], j] = in[i, j]
+hipError_t blas1_permute(int64_t n, int nrhs, const uint32_t *perm, bool gather, const double *in,
+                         int64_t ldi, double *out, int64_t ldo, hipStream_t stream);
 
 // Evaluates the device exp (fn 0 / 2) or sqrt (fn 1 / 3) used by the kernel
 // loops, in FP64 (fn 0, 1) or FP32 (fn 2, 3), on n device values.

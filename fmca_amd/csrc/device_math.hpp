@@ -135,7 +135,71 @@ __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
       return r > T(0) ? r * r * M::log_(r) : T(0);
     }
   }
-  return T(0);
+  // derivatives: dk/dl (variant 1) and phi = k'(r)/r (variant 2).  For every
+  // kernel of the form f(r/l): dk/dl = -(r^2 / l) phi(r).
+  const int base = kt % KT_COUNT, var = kt / KT_COUNT;
+  const T il = kp.inv_l, il2 = il * il;
+  T phi = 0;
+  switch (base) {
+    case KT_EXPONENTIAL: {
+      const T r = M::sqrt_(r2), e = M::exp_(-r * il);
+      if (var == KV_DL) return r * il2 * e;
+      phi = r > T(0) ? -(il / r) * e : T(0);
+      break;
+    }
+    case KT_MATERN32: {
+      const T x = M::sqrt_(r2) * kp.s;
+      phi = -kp.s * kp.s * M::exp_(-x);
+      break;
+    }
+    case KT_MATERN52: {
+      const T x = M::sqrt_(r2) * kp.s;
+      phi = -kp.s * kp.s * T(1. / 3.) * (T(1) + x) * M::exp_(-x);
+      break;
+    }
+    case KT_MATERN72: {
+      const T x = M::sqrt_(r2) * kp.s;
+      phi = -kp.s * kp.s * T(1. / 15.) * (T(3) + T(3) * x + x * x) * M::exp_(-x);
+      break;
+    }
+    case KT_MATERN92: {
+      const T x = M::sqrt_(r2) * kp.s;
+      phi = -kp.s * kp.s * T(1. / 105.) * (T(15) + x * (T(15) + x * (T(6) + x))) * M::exp_(-x);
+      break;
+    }
+    case KT_GAUSSIAN:
+      phi = -il2 * M::exp_(r2 * kp.s);
+      break;
+    case KT_INVMULTIQUADRIC: {
+      const T u = r2 * il2 + kp.c2;
+      phi = -il2 / (u * M::sqrt_(u));
+      break;
+    }
+    case KT_MULTIQUADRIC:
+      phi = il2 / M::sqrt_(r2 * il2 + kp.c2);
+      break;
+    case KT_BIHARMONIC2D: {
+      const T r = M::sqrt_(r2);
+      phi = r < M::eps() ? T(0) : il2 * (T(1) + T(2) * M::log_(r * il));
+      break;
+    }
+    case KT_BIHARMONIC3D: {
+      const T r = M::sqrt_(r2);
+      if (var == KV_DL) return -r * il2;
+      phi = r > T(0) ? il / r : T(0);
+      break;
+    }
+    case KT_TRIHARMONIC3D:
+      phi = T(3) * M::sqrt_(r2) * il2 * il;
+      break;
+    case KT_TPS2D: {
+      if (var == KV_DL) return T(0);
+      const T r = M::sqrt_(r2);
+      phi = r > T(0) ? T(2) * M::log_(r) + T(1) : T(0);
+      break;
+    }
+  }
+  return var == KV_DL ? -r2 * il * phi : phi;
 }
 
 // Barycentric Lagrange factors l_a(u), a < P1, written with a stride

@@ -126,6 +126,9 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
   m.attr("FP32") = (int)FMCA_PRECISION_FP32;
   m.attr("ROWS") = (int)FMCA_ROWS;
   m.attr("COLS") = (int)FMCA_COLS;
+  m.attr("KERNEL_VALUE") = (int)FMCA_KERNEL_VALUE;
+  m.attr("KERNEL_DL") = (int)FMCA_KERNEL_DL;
+  m.attr("KERNEL_PHI") = (int)FMCA_KERNEL_PHI;
   py::class_<Actor>(m, "Actor")
       .def(py::init(&create), py::arg("ktype"), py::arg("l"), py::arg("c"), py::arg("nu"),
            py::arg("Peval"), py::arg("P"), py::arg("deg"), py::arg("eta"))
@@ -162,6 +165,52 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
           },
           py::arg("rhs"), py::arg("ld_rhs"), py::arg("nrhs"), py::arg("out"), py::arg("ld_out"),
           py::arg("precision") = 0, py::arg("stream") = 0)
+      .def(
+          "cg_device",
+          [](Actor &a, uintptr_t b, int64_t ld_b, int64_t nrhs, uintptr_t x, int64_t ld_x,
+             double ridge, double tol, int precision, double refine_factor, int max_iter,
+             int max_outer, uintptr_t stream) {
+            fmca_cg_options o{ridge, tol, precision, refine_factor, max_iter, max_outer};
+            fmca_cg_result r{};
+            fmca_status st;
+            {
+              py::gil_scoped_release nogil;
+              st = fmca_h2_cg_device(a.h, reinterpret_cast<const double *>(b), ld_b, nrhs,
+                                     reinterpret_cast<double *>(x), ld_x, &o, &r,
+                                     reinterpret_cast<void *>(stream));
+            }
+            if (st != FMCA_OK && st != FMCA_ERR_NOT_CONVERGED) check(st);
+            py::dict d;
+            d["converged"] = (bool)r.converged;
+            d["iterations"] = r.iterations;
+            d["outer"] = r.outer;
+            d["matvecs_fp64"] = r.matvecs_fp64;
+            d["matvecs_fp32"] = r.matvecs_fp32;
+            d["rel_residual"] = r.rel_residual;
+            d["seconds"] = r.seconds;
+            return d;
+          },
+          py::arg("b"), py::arg("ld_b"), py::arg("nrhs"), py::arg("x"), py::arg("ld_x"),
+          py::arg("ridge"), py::arg("tol"), py::arg("precision"), py::arg("refine_factor"),
+          py::arg("max_iter"), py::arg("max_outer"), py::arg("stream") = 0)
+      .def("set_kernel",
+           [](Actor &a, const std::string &ktype, double l, double c, double nu) {
+             fmca_kernel_desc k{ktype.c_str(), l, c, nu};
+             check(fmca_h2_set_kernel(a.h, &k));
+           })
+      .def(
+          "action_variant_device",
+          [](Actor &a, int variant, bool transpose, uintptr_t rhs, int64_t ld_rhs, int64_t nrhs,
+             uintptr_t out, int64_t ld_out, int precision, uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            check(fmca_h2_action_variant_device(
+                a.h, variant, transpose ? 1 : 0, reinterpret_cast<const double *>(rhs), ld_rhs,
+                nrhs, reinterpret_cast<double *>(out), ld_out, precision,
+                reinterpret_cast<void *>(stream)));
+          },
+          py::arg("variant"), py::arg("transpose"), py::arg("rhs"), py::arg("ld_rhs"),
+          py::arg("nrhs"), py::arg("out"), py::arg("ld_out"), py::arg("precision") = 0,
+          py::arg("stream") = 0)
       .def("indices",
            [](const Actor &a, int which) {
              py::array_t<uint32_t> out(which == FMCA_ROWS ? a.m : a.n);

@@ -96,6 +96,71 @@ fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *
                                             int64_t ld_rhs, int64_t nrhs, double *d_out,
                                             int64_t ld_out, int precision, void *stream);
 
+/* Options of the CG solve of (K + ridge I) x = b, anchored on FMCA's pCG
+ * (FMCA/src/util/PCG.h:15-44: identity preconditioner here, stop at
+ * ||b - (K + ridge I) x|| / ||b|| <= tol, tol = 1e-8 in FMCA).  Vectors, inner
+ * products and x updates are FP64 in both forms; precision selects the matvec
+ * of the CG recurrence:
+ *   FMCA_PRECISION_FP64  plain CG;
+ *   FMCA_PRECISION_FP32  CG with reliable updates: FP32 matvecs drive the
+ *                        recurrence, and whenever a column's iterated residual
+ *                        has dropped by refine_factor since its last
+ *                        replacement (or reaches tol) the true residual is
+ *                        recomputed with the FP64 action and replaces it.
+ * Convergence is always decided on the FP64 residual. */
+typedef struct {
+  double ridge;
+  double tol;
+  int precision;
+  double refine_factor; /* 0 < f < 1, default 0.1 */
+  int max_iter;         /* cap on CG iterations (matvecs of `precision`) */
+  int max_outer;        /* cap on FP64 residual replacements */
+} fmca_cg_options;
+
+typedef struct {
+  int converged;        /* every column reached tol */
+  int iterations;       /* CG iterations */
+  int outer;            /* FP64 residual replacements */
+  int matvecs_fp64;     /* FP64 actions */
+  int matvecs_fp32;     /* FP32 actions */
+  double rel_residual;  /* max over columns of ||b - (K + ridge I) x|| / ||b||, FP64 action,
+                           at the last replacement */
+  double seconds;       /* wall time of the solve */
+} fmca_cg_result;
+
+/* CG solve of (K(P, P) + ridge I) x = b for nrhs independent columns
+ * (requires m == n).  d_b: n x nrhs, d_x: n x nrhs initial guess on entry
+ * (FMCA's pCG starts from x = b), solution on exit; device memory, natural
+ * order.  Synchronises `stream`. */
+fmca_status fmca_h2_cg_device(fmca_h2_actor *actor, const double *d_b, int64_t ld_b,
+                              int64_t nrhs, double *d_x, int64_t ld_x,
+                              const fmca_cg_options *opts, fmca_cg_result *result, void *stream);
+
+/* Replaces the kernel of an actor (type and parameters) without rebuilding
+ * the plan: tree, bases and block pattern do not depend on the kernel
+ * (FMCA rebuilds the evaluator per action, py/FMCA.cpp:233-235). */
+fmca_status fmca_h2_set_kernel(fmca_h2_actor *actor, const fmca_kernel_desc *kernel);
+
+/* Kernel variants of an action (no FMCA counterpart; the differentiation
+ * form).  The H2 operator is linear in the kernel function, so derivatives
+ * of the compressed matrix are H2 actions with derivative kernels on the same
+ * tree, bases and blocks:
+ *   FMCA_KERNEL_VALUE  k(r)
+ *   FMCA_KERNEL_DL     dk/dl (derivative with respect to the lengthscale)
+ *   FMCA_KERNEL_PHI    k'(r) / r, the radial factor of the gradient with
+ *                      respect to a point: d/dx_i sum_j k(|x_i - x_j|) w_j =
+ *                      sum_j phi(r_ij) (x_i - x_j) w_j  (phi(0) = 0 where
+ *                      k'(r)/r is singular) */
+enum { FMCA_KERNEL_VALUE = 0, FMCA_KERNEL_DL = 1, FMCA_KERNEL_PHI = 2 };
+
+/* out = K_variant(Peval, P) rhs (transpose = 0; rhs n x nrhs, out m x nrhs) or
+ * K_variant(Peval, P)^T rhs (transpose = 1; rhs m x nrhs, out n x nrhs), device
+ * memory, natural order, enqueued on `stream`. */
+fmca_status fmca_h2_action_variant_device(fmca_h2_actor *actor, int variant, int transpose,
+                                          const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
+                                          double *d_out, int64_t ld_out, int precision,
+                                          void *stream);
+
 /* Permutation of a tree (cluster order position -> point index), as
  * pyH2Matrix::rindices / cindices (py/FMCA.cpp:224-229) and
  * ClusterTreeBase::indices (FMCA/src/Clustering/ClusterTreeBase.h:63-65). */
