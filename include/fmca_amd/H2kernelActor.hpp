@@ -1,0 +1,129 @@
+// C++ facade over the C ABI with FMCA's H2kernelActor interface.
+//
+// Drop-in for FMCA::H2kernelActor<Covariance, Moments, H2ClusterTree,
+// MatrixEvaluator> (FMCA/src/H2Matrix/H2kernelActor.h:17-94): the same
+// constructor (kernel, Peval, P, mpole_deg, eta) and `M action(const M &rhs)`.
+// The template arguments after the kernel type are accepted and ignored (the
+// moments are tensor-product Chebyshev, the tree a ClusterTree with
+// CardinalityBisection, the evaluator unsymmetric Nystrom -- the hot-path
+// instantiation of FMCA).
+//
+// Matrix types: anything column-major with rows(), cols(), data() and a
+// (rows, cols) constructor -- Eigen::MatrixXd qualifies.  Kernel types:
+// anything with kernelType(), l(), c(), nu() -- FMCA::CovarianceKernel
+// (CovarianceKernel.h:154-161) and fmca_amd::CovarianceKernel qualify.
+//
+// Errors raise fmca_amd::Error (FMCA asserts instead).
+#pragma once
+
+#include <cctype>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../fmca_amd.h"
+
+namespace fmca_amd {
+
+class Error : public std::runtime_error {
+ public:
+  Error(fmca_status s, const std::string &what)
+      : std::runtime_error(what + ": " + fmca_status_string(s) + " (" + fmca_last_error() + ")"),
+        status_(s) {}
+  fmca_status status() const { return status_; }
+
+ private:
+  fmca_status status_;
+};
+
+inline void check(fmca_status s, const char *what) {
+  if (s != FMCA_OK) throw Error(s, what);
+}
+
+// FMCA::CovarianceKernel(ktype, l, c, nu) with its parameters held by value.
+class CovarianceKernel {
+ public:
+  CovarianceKernel() : CovarianceKernel("EXPONENTIAL") {}
+  explicit CovarianceKernel(const std::string &ktype, double l = 1., double c = 1.,
+                            double nu = 1.)
+      : ktype_(ktype), l_(l), c_(c), nu_(nu) {
+    for (auto &ch : ktype_) ch = (char)std::toupper((unsigned char)ch);
+  }
+  std::string kernelType() const { return ktype_; }
+  const double &l() const { return l_; }
+  const double &c() const { return c_; }
+  const double &nu() const { return nu_; }
+
+ private:
+  std::string ktype_;
+  double l_, c_, nu_;
+};
+
+enum class Precision : int { FP64 = FMCA_PRECISION_FP64, FP32 = FMCA_PRECISION_FP32 };
+
+template <typename Covariance = CovarianceKernel, typename... FmcaPolicies>
+class H2kernelActor {
+ public:
+  template <typename PointMatrix>
+  H2kernelActor(const Covariance &kernel, const PointMatrix &Peval, const PointMatrix &P,
+                unsigned mpole_deg, double eta)
+      : m_((int64_t)Peval.cols()), n_((int64_t)P.cols()) {
+    const std::string kt = kernel.kernelType();
+    fmca_kernel_desc desc{kt.c_str(), (double)kernel.l(), (double)kernel.c(),
+                          (double)kernel.nu()};
+    if (Peval.rows() != P.rows()) throw Error(FMCA_ERR_INVALID_ARGUMENT, "point dimensions");
+    fmca_h2_actor *h = nullptr;
+    check(fmca_h2_create(&desc, (int)P.rows(), m_, Peval.data(), n_, P.data(), (int)mpole_deg,
+                         eta, &h),
+          "fmca_h2_create");
+    h_.reset(h, [](fmca_h2_actor *p) { fmca_h2_destroy(p); });
+  }
+
+  // H2kernelActor::action (H2kernelActor.h:38-80): K(Peval, P) * rhs, natural order.
+  template <typename M>
+  M action(const M &rhs, Precision precision = Precision::FP64) const {
+    if ((int64_t)rhs.rows() != n_) throw Error(FMCA_ERR_INVALID_ARGUMENT, "rhs rows");
+    M out((decltype(rhs.rows()))m_, rhs.cols());
+    check(fmca_h2_action(h_.get(), rhs.data(), n_, (int64_t)rhs.cols(), out.data(), m_,
+                         (int)precision),
+          "fmca_h2_action");
+    return out;
+  }
+
+  // H2MatrixBase::action (H2MatrixBase.h:113-152): cluster-ordered rhs and result.
+  template <typename M>
+  M action_cluster_order(const M &rhs, Precision precision = Precision::FP64) const {
+    if ((int64_t)rhs.rows() != n_) throw Error(FMCA_ERR_INVALID_ARGUMENT, "rhs rows");
+    M out((decltype(rhs.rows()))m_, rhs.cols());
+    check(fmca_h2_action_cluster_order(h_.get(), rhs.data(), n_, (int64_t)rhs.cols(),
+                                       out.data(), m_, (int)precision),
+          "fmca_h2_action_cluster_order");
+    return out;
+  }
+
+  // ClusterTreeBase::indices of the row (FMCA_ROWS) or column (FMCA_COLS) tree.
+  std::vector<uint32_t> indices(int which) const {
+    std::vector<uint32_t> out(which == FMCA_ROWS ? m_ : n_);
+    check(fmca_h2_indices(h_.get(), which, out.data()), "fmca_h2_indices");
+    return out;
+  }
+
+  // H2MatrixBase::statistics (H2MatrixBase.h:70-105).
+  std::vector<double> statistics() const {
+    std::vector<double> out(6);
+    check(fmca_h2_statistics(h_.get(), out.data()), "fmca_h2_statistics");
+    return out;
+  }
+
+  int64_t rows() const { return m_; }
+  int64_t cols() const { return n_; }
+  fmca_h2_actor *handle() const { return h_.get(); }
+
+ private:
+  std::shared_ptr<fmca_h2_actor> h_;
+  int64_t m_, n_;
+};
+
+}  // namespace fmca_amd

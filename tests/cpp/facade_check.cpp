@@ -1,0 +1,71 @@
+// Exercises include/fmca_amd/H2kernelActor.hpp the way FMCA code uses
+// FMCA::H2kernelActor: construct from (kernel, Peval, P, deg, eta), call
+// action(rhs) on a column-major matrix.  Reads points and rhs from a binary
+// file, writes the result; "plan" mode only builds the actor.
+//   facade_check plan|action <in.bin> <out.bin>
+// in.bin: int64 dim, n, nrhs, deg; double eta, l; char[16] kernel;
+//         double P[dim*n] (column-major); double rhs[n*nrhs]
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "fmca_amd/H2kernelActor.hpp"
+
+// minimal column-major matrix with Eigen's rows()/cols()/data() interface
+struct Mat {
+  Mat(long r, long c) : r_(r), c_(c), v_((size_t)r * c) {}
+  long rows() const { return r_; }
+  long cols() const { return c_; }
+  const double *data() const { return v_.data(); }
+  double *data() { return v_.data(); }
+  long r_, c_;
+  std::vector<double> v_;
+};
+
+// a kernel type of the caller's own, with the FMCA::CovarianceKernel accessors
+struct MyKernel {
+  std::string t;
+  double l_;
+  std::string kernelType() const { return t; }
+  double l() const { return l_; }
+  double c() const { return 1.0; }
+  double nu() const { return 1.0; }
+};
+
+int main(int argc, char **argv) {
+  if (argc < 4) return 2;
+  std::ifstream in(argv[2], std::ios::binary);
+  int64_t hdr[4];
+  double par[2];
+  char kname[16];
+  in.read((char *)hdr, sizeof(hdr));
+  in.read((char *)par, sizeof(par));
+  in.read(kname, 16);
+  const long dim = hdr[0], n = hdr[1], nrhs = hdr[2];
+  Mat P(dim, n), rhs(n, nrhs);
+  in.read((char *)P.data(), sizeof(double) * dim * n);
+  in.read((char *)rhs.data(), sizeof(double) * n * nrhs);
+  if (!in) return 3;
+  try {
+    const MyKernel kernel{std::string(kname), par[1]};
+    // FMCA's four template arguments are accepted (and ignored) after the kernel type
+    fmca_amd::H2kernelActor<MyKernel, struct Moments, struct H2ClusterTree, struct Evaluator>
+        act(kernel, P, P, (unsigned)hdr[3], par[0]);
+    std::ofstream out(argv[3], std::ios::binary);
+    if (std::strcmp(argv[1], "plan") == 0) {
+      auto idx = act.indices(FMCA_COLS);
+      auto st = act.statistics();
+      out.write((const char *)idx.data(), sizeof(uint32_t) * idx.size());
+      out.write((const char *)st.data(), sizeof(double) * st.size());
+      return 0;
+    }
+    Mat y = act.action(rhs);
+    out.write((const char *)y.data(), sizeof(double) * y.rows() * y.cols());
+  } catch (const fmca_amd::Error &e) {
+    std::cerr << e.what() << std::endl;
+    return 10 + (int)e.status();
+  }
+  return 0;
+}
