@@ -31,7 +31,7 @@ try:
     from . import _fmca_amd_py as _C
 except ImportError as e:  # pragma: no cover - exercised only on a broken install
     raise ImportError(
-        "fmca_amd: the compiled extension is missing; run `python -m fmca_amd.build`"
+        "fmca_amd: the compiled extension is missing; run `python fmca_amd/build.py`"
     ) from e
 
 FP64 = _C.FP64

@@ -3,7 +3,7 @@
   libfmca_amd.so        C-ABI library (host planner + HIP kernels), hipcc
   _fmca_amd_py*.so      pybind11 module over the C ABI
 
-Run ``python -m fmca_amd.build`` (or ``__graft_entry__.build()``).  Objects are
+Run ``python fmca_amd/build.py`` (or ``__graft_entry__.build()``).  Objects are
 rebuilt every time; compilation of independent sources runs in parallel.
 """
 from __future__ import annotations
