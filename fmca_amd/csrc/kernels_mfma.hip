@@ -49,7 +49,17 @@ struct Mfma<double> {
 constexpr int kWaves = 4;         // waves per workgroup
 constexpr int kRT = 4;            // 16-row tiles per wave -> 256 rows per pass
 constexpr int kRows = kWaves * kRT * 16;
+// LDS row strides padded by 16 elements: the four 16-lane groups of a wave
+// read rows k..k+3 of a staged operand, and a stride = 16 (mod 32) puts them
+// on disjoint banks (for 32 and for 64 banks)
+constexpr int kRowsP = kRows + 16;
 constexpr int kChunk = 64;        // source points / nodes staged per LDS chunk
+
+// Occupancy target (waves per SIMD) of the MFMA kernels: FP32 with up to 32
+// rhs columns fits 4 waves without spilling; FP64 (twice the registers per
+// value) and 64-column tiles keep the default.
+template <typename T, int NT>
+constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2) ? 4 : 1;
 
 // Packed tensor index of every moment: a0 | a1 << 8 | a2 << 16.
 __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0xff; }
@@ -59,23 +69,26 @@ __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0x
 // row tiles {wave, wave+4, wave+8, wave+12} of the current 256-row pass and
 // 16*NT rhs columns.
 template <typename T, int D, int KT, int NT>
-__global__ __launch_bounds__(256) void k_leaf_output_mfma(
+__global__ __launch_bounds__(256, kMinWaves<T, NT>) void k_leaf_output_mfma(
     DevTreeView<T> rows, DevTreeView<T> cols, DevPatternView pat, InterpParams<T> ip,
     KernelParams<T> kp, int nrhs, const T *__restrict__ xs, const T *__restrict__ tlhs,
     double *__restrict__ out, int64_t ldo, int natural) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   constexpr int NC = 16 * NT;
+  constexpr int NCP = NC + 16;  // padded LDS row stride
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
-  T *F = reinterpret_cast<T *>(smem_raw);              // [D][P1][kRows] factors of the rows
-  int *aidx = reinterpret_cast<int *>(F + D * P1 * kRows);  // [qpad] packed indices
-  T *U = reinterpret_cast<T *>(aidx + ((qpad + 3) & ~3));   // union region
-  T *Ms = U;                                            // [kChunk][NC] coefficients
+  T *F = reinterpret_cast<T *>(smem_raw);              // [D][P1][kRowsP] factors of the rows
+  int *aidx = reinterpret_cast<int *>(F + D * P1 * kRowsP);  // [qpad] packed indices
+  T *U = reinterpret_cast<T *>(aidx + ((qpad + 3) & ~3));    // union region
+  T *Ms = U;                                            // [kChunk][NCP] coefficients
   T *Ys = U;                                            // [D][kChunk] source points
-  T *Xs = U + D * kChunk;                               // [kChunk][NC] source rhs
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T *Xs = U + D * kChunk;                               // [kChunk][NCP] source rhs
+  const int lane = threadIdx.x & 63;
+  // wave index as a scalar: the row-tile guards below are then uniform branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int li = blockIdx.x;
   const int c = rows.leaves[li];
@@ -105,11 +118,11 @@ __global__ __launch_bounds__(256) void k_leaf_output_mfma(
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           const T xv = rows.pts[(int64_t)d * rows.npts + b0 + i];
-          lagrange_1d<T>((xv - mn[d]) / bs[d], ip, F + (d * P1) * kRows + threadIdx.x, kRows);
+          lagrange_1d<T>((xv - mn[d]) / bs[d], ip, F + (d * P1) * kRowsP + threadIdx.x, kRowsP);
         }
       } else {
         for (int d = 0; d < D; ++d)
-          for (int a = 0; a < P1; ++a) F[(d * P1 + a) * kRows + threadIdx.x] = T(0);
+          for (int a = 0; a < P1; ++a) F[(d * P1 + a) * kRowsP + threadIdx.x] = T(0);
       }
     }
     // this lane's A rows: tile (wave + 4 rt), row lr
@@ -137,8 +150,9 @@ __global__ __launch_bounds__(256) void k_leaf_output_mfma(
         for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
           const int al = e / NC, r = e % NC;
           const int alpha = a0 + al;
-          Ms[e] = (alpha < q && rc + r < nrhs) ? tlhs[((int64_t)c * q + alpha) * nrhs + rc + r]
-                                              : T(0);
+          Ms[al * NCP + r] = (alpha < q && rc + r < nrhs)
+                                 ? tlhs[((int64_t)c * q + alpha) * nrhs + rc + r]
+                                 : T(0);
         }
         __syncthreads();
         for (int ks = 0; ks < na; ks += 4) {
@@ -146,14 +160,14 @@ __global__ __launch_bounds__(256) void k_leaf_output_mfma(
           const int pk = aidx[a0 + al];
           T b[NT];
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[al * NC + nt * 16 + lr];
+          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[al * NCP + nt * 16 + lr];
 #pragma unroll
           for (int rt = 0; rt < kRT; ++rt) {
             if (!tile_on[rt]) continue;
             const int row = (wave + kWaves * rt) * 16 + lr;
-            T v = F[(0 * P1 + unpack(pk, 0)) * kRows + row];
+            T v = F[(0 * P1 + unpack(pk, 0)) * kRowsP + row];
 #pragma unroll
-            for (int d = 1; d < D; ++d) v *= F[(d * P1 + unpack(pk, d)) * kRows + row];
+            for (int d = 1; d < D; ++d) v *= F[(d * P1 + unpack(pk, d)) * kRowsP + row];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(v, b[nt], acc[rt][nt]);
           }
@@ -172,11 +186,13 @@ __global__ __launch_bounds__(256) void k_leaf_output_mfma(
           }
           for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
             const int jj = e / NC, r = e % NC;
-            Xs[e] = (jj < nj && rc + r < nrhs) ? xs[(int64_t)(sb + j0 + jj) * nrhs + rc + r]
-                                              : T(0);
+            Xs[jj * NCP + r] = (jj < nj && rc + r < nrhs)
+                                   ? xs[(int64_t)(sb + j0 + jj) * nrhs + rc + r]
+                                   : T(0);
           }
           __syncthreads();
           const int nsteps = (nj + 3) & ~3;
+#pragma unroll 2
           for (int ks = 0; ks < nsteps; ks += 4) {
             const int sj = ks + lk;
             T y[D];
@@ -184,7 +200,7 @@ __global__ __launch_bounds__(256) void k_leaf_output_mfma(
             for (int d = 0; d < D; ++d) y[d] = Ys[d * kChunk + sj];
             T b[NT];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) b[nt] = Xs[sj * NC + nt * 16 + lr];
+            for (int nt = 0; nt < NT; ++nt) b[nt] = Xs[sj * NCP + nt * 16 + lr];
 #pragma unroll
             for (int rt = 0; rt < kRT; ++rt) {
               if (!tile_on[rt]) continue;
@@ -227,7 +243,7 @@ __global__ __launch_bounds__(256) void k_leaf_output_mfma(
 // interpolation nodes (tiles of 16, kRT per wave, so q <= 256), K = the q
 // source nodes of each far cluster, N = 16*NT rhs columns.
 template <typename T, int D, int KT, int NT>
-__global__ __launch_bounds__(256) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
+__global__ __launch_bounds__(256, kMinWaves<T, NT>) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
                                                        DevPatternView pat, InterpParams<T> ip,
                                                        KernelParams<T> kp, int nrhs,
                                                        const T *__restrict__ mom,
@@ -235,13 +251,16 @@ __global__ __launch_bounds__(256) void k_coupling_mfma(DevTreeView<T> rows, DevT
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   constexpr int NC = 16 * NT;
+  constexpr int NCP = NC + 16;  // padded LDS row stride
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
   int *bidx = reinterpret_cast<int *>(smem_raw);                     // [qpad]
   T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][kMaxP1]
-  T *Ms = Y + D * kMaxP1;                                            // [kChunk][NC]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T *Ms = Y + D * kMaxP1;                                            // [kChunk][NCP]
+  const int lane = threadIdx.x & 63;
+  // wave index as a scalar: the row-tile guards below are then uniform branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int t = blockIdx.x;
   const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
@@ -288,10 +307,12 @@ __global__ __launch_bounds__(256) void k_coupling_mfma(DevTreeView<T> rows, DevT
         for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
           const int bl = e / NC, r = e % NC;
           const int beta = be + bl;
-          Ms[e] = (beta < q && rc + r < nrhs) ? mom[((int64_t)s * q + beta) * nrhs + rc + r]
-                                             : T(0);
+          Ms[bl * NCP + r] = (beta < q && rc + r < nrhs)
+                                 ? mom[((int64_t)s * q + beta) * nrhs + rc + r]
+                                 : T(0);
         }
         __syncthreads();
+#pragma unroll 2
         for (int ks = 0; ks < nbeta; ks += 4) {
           const int bl = ks + lk;
           const int pk = bidx[be + bl];
@@ -300,7 +321,7 @@ __global__ __launch_bounds__(256) void k_coupling_mfma(DevTreeView<T> rows, DevT
           for (int d = 0; d < D; ++d) y[d] = Y[d * kMaxP1 + unpack(pk, d)];
           T b[NT];
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NC + nt * 16 + lr];
+          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NCP + nt * 16 + lr];
 #pragma unroll
           for (int rt = 0; rt < kRT; ++rt) {
             if (!tile_on[rt]) continue;
@@ -337,9 +358,9 @@ __global__ __launch_bounds__(256) void k_coupling_mfma(DevTreeView<T> rows, DevT
 template <typename T, int NT>
 size_t leaf_output_mfma_lds(int D, int P1, int q) {
   const int qpad = (q + 3) & ~3;
-  const size_t u1 = (size_t)kChunk * 16 * NT;
-  const size_t u2 = (size_t)D * kChunk + (size_t)kChunk * 16 * NT;
-  return (size_t)D * P1 * kRows * sizeof(T) + (size_t)((qpad + 3) & ~3) * sizeof(int) +
+  const size_t u1 = (size_t)kChunk * (16 * NT + 16);
+  const size_t u2 = (size_t)D * kChunk + (size_t)kChunk * (16 * NT + 16);
+  return (size_t)D * P1 * kRowsP * sizeof(T) + (size_t)((qpad + 3) & ~3) * sizeof(int) +
          (u1 > u2 ? u1 : u2) * sizeof(T);
 }
 
@@ -347,7 +368,7 @@ template <typename T, int NT>
 size_t coupling_mfma_lds(int D, int q) {
   const int qpad = (q + 3) & ~3;
   return (size_t)((qpad + 3) & ~3) * sizeof(int) +
-         ((size_t)D * kMaxP1 + (size_t)kChunk * 16 * NT) * sizeof(T);
+         ((size_t)D * kMaxP1 + (size_t)kChunk * (16 * NT + 16)) * sizeof(T);
 }
 
 template <typename T, int D, int NT>
