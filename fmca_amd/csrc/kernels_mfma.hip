@@ -69,7 +69,7 @@ __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0x
 // row tiles {wave, wave+4, wave+8, wave+12} of the current 256-row pass and
 // 16*NT rhs columns.
 template <typename T, int D, int KT, int NT>
-__global__ __launch_bounds__(256, kMinWaves<T, NT>) void k_leaf_output_mfma(
+__global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
     DevTreeView<T> rows, DevTreeView<T> cols, DevPatternView pat, InterpParams<T> ip,
     KernelParams<T> kp, int nrhs, const T *__restrict__ xs, const T *__restrict__ tlhs,
     double *__restrict__ out, int64_t ldo, int natural) {
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256, kMinWaves<T, NT>) void k_leaf_output_mfma(
 // interpolation nodes (tiles of 16, kRT per wave, so q <= 256), K = the q
 // source nodes of each far cluster, N = 16*NT rhs columns.
 template <typename T, int D, int KT, int NT>
-__global__ __launch_bounds__(256, kMinWaves<T, NT>) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
+__global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
                                                        DevPatternView pat, InterpParams<T> ip,
                                                        KernelParams<T> kp, int nrhs,
                                                        const T *__restrict__ mom,
