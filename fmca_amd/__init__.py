@@ -152,7 +152,8 @@ class H2kernelActor:
         precision=FP64: plain CG.  precision=FP32: CG with reliable updates --
         FP32 matvecs drive the recurrence, FP64 vectors and inner products, and
         the iterated residual is replaced by the FP64 residual each time it has
-        dropped by refine_factor.  b, x0: float64 CUDA tensors (n,) or
+        dropped by refine_factor (all columns together; the search direction
+        is re-orthogonalised against the new residual).  b, x0: float64 CUDA tensors (n,) or
         (n, nrhs); x0 defaults to b as in pCG.  Returns (x, info) with
         info['rel_residual'] measured with the FP64 action.
         """

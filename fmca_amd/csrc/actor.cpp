@@ -80,7 +80,7 @@ struct DevTreeInts {
 
 template <typename T>
 struct DevTreeReals {
-  DevBuf pts, bbmin, bbsize, transfer;
+  DevBuf pts, unit, bbmin, bbsize, transfer;
 };
 
 struct DevPattern {
@@ -183,11 +183,30 @@ hipError_t upload_ints(DevTreeInts &d, const Tree &T) {
   return d.leaves.upload(T.leaves);
 }
 
+// Coordinates of every point in the unit box of its leaf, (x - bbmin) / bbsize:
+// the argument of the leaf basis (NystromMoments::moment_matrix,
+// NystromMoments.h:24-34).  Formed once in FP64, so that the FP32 kernels see
+// it rounded once instead of amplifying the rounding of absolute coordinates
+// by 1 / bbsize.
+std::vector<double> leaf_unit_coords(const Tree &tr, const std::vector<double> &pts) {
+  const int d = tr.dim;
+  const size_t n = (size_t)tr.npts;
+  std::vector<double> u(pts.size());
+  for (int c : tr.leaves)
+    for (int i = 0; i < d; ++i) {
+      const double mn = tr.bbmin[(size_t)c * d + i], sz = tr.bbsize[(size_t)c * d + i];
+      for (int64_t j = tr.begin[c]; j < tr.begin[c] + tr.size[c]; ++j)
+        u[i * n + j] = (pts[i * n + j] - mn) / sz;
+    }
+  return u;
+}
+
 template <typename T>
 hipError_t upload_reals(DevTreeReals<T> &d, const Tree &tr, const std::vector<double> &pts,
                         const std::vector<double> &transfer) {
   hipError_t e;
   if ((e = d.pts.upload(convert<T>(pts))) != hipSuccess) return e;
+  if ((e = d.unit.upload(convert<T>(leaf_unit_coords(tr, pts)))) != hipSuccess) return e;
   if ((e = d.bbmin.upload(convert<T>(tr.bbmin))) != hipSuccess) return e;
   if ((e = d.bbsize.upload(convert<T>(tr.bbsize))) != hipSuccess) return e;
   return d.transfer.upload(convert<T>(transfer));
@@ -203,6 +222,7 @@ DevTreeView<T> make_view(const Tree &tr, const DevTreeInts &di, const DevTreeRea
   v.npts = tr.npts;
   v.perm = di.perm.as<uint32_t>();
   v.pts = dr.pts.template as<T>();
+  v.unit = dr.unit.template as<T>();
   v.bbmin = dr.bbmin.template as<T>();
   v.bbsize = dr.bbsize.template as<T>();
   v.begin = di.begin.as<int32_t>();
@@ -681,12 +701,15 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
   // r = b - (K + ridge I) x, FP64
   if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
   CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
-  // CG with reliable updates: the recurrence runs with matvecs of the chosen
-  // precision (FP64 vectors, inner products and updates throughout); whenever
-  // a column's iterated residual has dropped by refine_factor since the last
-  // replacement, or reaches the target, the true residual b - (K + ridge I) x
-  // is recomputed with the FP64 action and replaces it, keeping the search
-  // direction.  With FP64 matvecs only the final check replaces the residual.
+  // CG with reliable updates (Sleijpen & van der Vorst 1996; the form of
+  // Clark et al. 2010): the recurrence runs with matvecs of the chosen
+  // precision (FP64 vectors, inner products and updates throughout).  Once
+  // every active column's iterated residual has dropped by refine_factor
+  // since the last replacement or reached its target, the true residual
+  // b - (K + ridge I) x is recomputed with the FP64 action and replaces it;
+  // the search direction is kept and re-orthogonalised against the new
+  // residual, p -= (r.p / r.r) r.  With FP64 matvecs only the final check
+  // replaces the residual.
   const int prec = opts->precision;
   const int max_iter = opts->max_iter > 0 ? opts->max_iter : 100000;
   const int max_updates = opts->max_outer > 0 ? opts->max_outer : 1000;
@@ -717,13 +740,18 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
     CG_TRY(blas1_axpy(n, nr, -1.0, d_alpha, Q, n, R, n, sm));
     if ((st = dot(R, R, rn2.data())) != FMCA_OK) return st;
     res->iterations++;
-    bool update = false;
+    // FP64: replace as soon as one column reaches its target.  Lower
+    // precision: once all active columns have reached theirs or dropped by delta.
+    bool any_done = false, all_ready = true;
     for (int j = 0; j < nr; ++j) {
       if (!active[j]) continue;
       const double rn = std::sqrt(rn2[j]);
       if (!std::isfinite(rn)) return fail(FMCA_ERR_NONFINITE, "CG residual is not finite");
-      if (rn <= target[j] || (prec != FMCA_PRECISION_FP64 && rn < delta * rmax[j])) update = true;
+      const bool done = rn <= target[j];
+      any_done = any_done || done;
+      all_ready = all_ready && (done || rn < delta * rmax[j]);
     }
+    const bool update = prec == FMCA_PRECISION_FP64 ? any_done : all_ready;
     if (update) {
       if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
       CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
@@ -737,6 +765,13 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
         if (active[j] && rn <= target[j]) active[j] = 0;
       }
       res->rel_residual = worst;
+      if (prec != FMCA_PRECISION_FP64) {
+        if ((st = dot(R, Pv, h_dot)) != FMCA_OK) return st;
+        for (int j = 0; j < nr; ++j)
+          h_alpha[j] = active[j] && rn2[j] != 0 ? h_dot[j] / rn2[j] : 0.0;
+        CG_TRY(upload(d_alpha, h_alpha));
+        CG_TRY(blas1_axpy(n, nr, -1.0, d_alpha, R, n, Pv, n, sm));
+      }
     }
     any = false;
     for (int j = 0; j < nr; ++j) {

@@ -18,6 +18,7 @@ struct DevTreeView {
   int64_t npts = 0;
   const uint32_t *perm = nullptr;   // cluster position -> natural index
   const T *pts = nullptr;           // dim x npts, cluster order
+  const T *unit = nullptr;          // dim x npts: (x - bbmin) / bbsize of the point's leaf
   const T *bbmin = nullptr;         // ncl x dim
   const T *bbsize = nullptr;        // ncl x dim
   const int32_t *begin = nullptr;   // ncl

@@ -57,12 +57,6 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
   const int c = tr.leaves[blockIdx.x];
   const int b0 = tr.begin[c], sz = tr.size[c];
   const int P1 = ip.P1, q = ip.q;
-  T mn[D], inv[D];
-#pragma unroll
-  for (int i = 0; i < D; ++i) {
-    mn[i] = tr.bbmin[c * D + i];
-    inv[i] = tr.bbsize[c * D + i];
-  }
   for (int rc = 0; rc < nrhs; rc += NR) {
     const int nr = min(NR, nrhs - rc);
     for (int abase = 0; abase < q; abase += kBlock) {
@@ -84,10 +78,9 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
         __syncthreads();
         for (int e = threadIdx.x; e < D * 64; e += kBlock) {
           const int i = e / 64, jj = e % 64;
-          if (jj < np) {
-            const T x = tr.pts[(int64_t)i * tr.npts + b0 + p0 + jj];
-            lagrange_1d<T>((x - mn[i]) / inv[i], ip, F + (i * P1) * 64 + jj, 64);
-          }
+          if (jj < np)
+            lagrange_1d<T>(tr.unit[(int64_t)i * tr.npts + b0 + p0 + jj], ip,
+                           F + (i * P1) * 64 + jj, 64);
         }
         for (int e = threadIdx.x; e < 64 * NR; e += kBlock) {
           const int jj = e / NR, r = e % NR;
@@ -371,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
       if (act) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
-          lagrange_1d<T>((x[d] - rows.bbmin[c * D + d]) / rows.bbsize[c * D + d], ip,
+          lagrange_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
                          F + (d * P1) * kBlock + threadIdx.x, kBlock);
       }
       int a[D];

@@ -94,12 +94,6 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
   const int c = rows.leaves[li];
   const int b0 = rows.begin[c], sz = rows.size[c];
   const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
-  T mn[D], bs[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    mn[d] = rows.bbmin[c * D + d];
-    bs[d] = rows.bbsize[c * D + d];
-  }
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
     int rem = e < q ? e : 0, pk = 0;
 #pragma unroll
@@ -116,10 +110,9 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
       const int i = pbase + threadIdx.x;
       if (i < sz) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const T xv = rows.pts[(int64_t)d * rows.npts + b0 + i];
-          lagrange_1d<T>((xv - mn[d]) / bs[d], ip, F + (d * P1) * kRowsP + threadIdx.x, kRowsP);
-        }
+        for (int d = 0; d < D; ++d)
+          lagrange_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
+                         F + (d * P1) * kRowsP + threadIdx.x, kRowsP);
       } else {
         for (int d = 0; d < D; ++d)
           for (int a = 0; a < P1; ++a) F[(d * P1 + a) * kRowsP + threadIdx.x] = T(0);

@@ -103,10 +103,11 @@ fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *
  * of the CG recurrence:
  *   FMCA_PRECISION_FP64  plain CG;
  *   FMCA_PRECISION_FP32  CG with reliable updates: FP32 matvecs drive the
- *                        recurrence, and whenever a column's iterated residual
- *                        has dropped by refine_factor since its last
- *                        replacement (or reaches tol) the true residual is
- *                        recomputed with the FP64 action and replaces it.
+ *                        recurrence; once every active column's iterated
+ *                        residual has dropped by refine_factor since its last
+ *                        replacement (or reached tol), the true residual is
+ *                        recomputed with the FP64 action and replaces it, and
+ *                        the search direction is re-orthogonalised against it.
  * Convergence is always decided on the FP64 residual. */
 typedef struct {
   double ridge;
