@@ -6,12 +6,16 @@ Workload (the configuration the headline metric is quoted on):
   32 right-hand sides with i.i.d. N(0,1) entries.
 
 One step = one matrix-free H2 matvec y = K x of the precision form (FP32 kernel
-entries and contractions, FP64 in/out) on the 32 resident right-hand sides.
-The plain FP64 action is timed on the same inputs for the ratio, and the
-relative difference of the two results is reported.
+entries and contractions on the matrix cores, FP64 in/out) on the 32 resident
+right-hand sides.  Beside it, on the same inputs: the plain FP64 action (the
+ratio and the relative difference of the two results), the single-rhs
+actions, the host-buffer boundary, and the CG solve of (K + 100 I) a = x to a
+1e-8 relative FP64 residual with FP32 matvecs + FP64 reliable updates and with
+plain FP64 CG (iterations, time, solution difference).
 
-Prints ONE JSON line (rank 0).  Usage:
+Prints ONE JSON line.  Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                  [--no-solve] [--calibrate-copy-mib M]
 """
 from __future__ import annotations
 
@@ -40,6 +44,11 @@ def parse():
     p.add_argument("--n", type=int, default=1_000_000)
     p.add_argument("--nrhs", type=int, default=32)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-solve", action="store_true")
+    p.add_argument("--ridge", type=float, default=100.0)
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="also run one elementwise copy of this size (reads and writes "
+                        "M MiB) as the reference kernel for HBM traffic counters")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target CPU time of the sampled coupling/near-field loop")
     return p.parse_args()
@@ -66,16 +75,46 @@ def timed_steps(H, x, out, precision, steps, warmup, profile):
     return (t1 - t0) / steps, phases
 
 
-def traffic_from_profiles(kernel_key):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    calibrated against a copy kernel; read from the committed summary if present."""
+def mfma_tiles(nrhs):
+    """rhs tiles of 16 columns per workgroup pass of the MFMA kernels."""
+    return 1 if nrhs <= 16 else (2 if nrhs <= 32 else 4)
+
+
+def traffic_from_profiles(kernel_name):
+    """HBM bytes per launch of `kernel_name` from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE runs, each scaled by what the counter reads for a copy kernel of
+    known size (profiles/traffic.json, written from those runs); None if the
+    kernel was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get(kernel_key)
-    except (OSError, ValueError):
+        k = d.get("kernels", {}).get(kernel_name)
+        return None if k is None else k["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
         return None
+
+
+def solve(H, x, ridge):
+    """CG on (K + ridge I) a = x to a 1e-8 relative FP64 residual, FP32 matvecs
+    with FP64 reliable updates and plain FP64; both from a = 0."""
+    import torch
+    import fmca_amd
+    out = {"ridge": ridge, "tol": 1e-8, "nrhs": x.shape[1]}
+    sol = {}
+    for name, prec in (("fp32_refined", fmca_amd.FP32), ("fp64", fmca_amd.FP64)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8, precision=prec,
+                       max_iter=2000)
+        torch.cuda.synchronize()
+        info["wall_s"] = time.perf_counter() - t0
+        out[name] = info
+        sol[name] = a
+    out["solution_rel_diff"] = float(torch.linalg.norm(sol["fp32_refined"] - sol["fp64"])
+                                     / torch.linalg.norm(sol["fp64"]))
+    out["speedup_fp32_refined_vs_fp64"] = out["fp64"]["wall_s"] / out["fp32_refined"]["wall_s"]
+    return out
 
 
 def cpu_baseline(P, x, info, budget_seconds):
@@ -143,31 +182,50 @@ def main():
     o1 = torch.empty((n, 1), dtype=torch.float64, device="cuda")
     t32_1, _ = timed_steps(H, x1, o1, fmca_amd.FP32, args.steps, 1, False)
     t64_1, _ = timed_steps(H, x1, o1, fmca_amd.FP64, max(1, args.steps // 2), 1, False)
+    # the C-ABI host-buffer boundary (fmca_h2_action): H2D + action + D2H
+    H.action(xh, precision=fmca_amd.FP32)
+    t0 = time.perf_counter()
+    H.action(xh, precision=fmca_amd.FP32)
+    t_host = time.perf_counter() - t0
+    solve_res = None if args.no_solve else solve(H, x, args.ridge)
+    if args.calibrate_copy_mib > 0:
+        nel = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     E_near, E_far, q = info["near_entries"], info["far_entries"], int(info["q"])
     value = n * nrhs / t32 / 1e6
 
-    # dominant kernel of the timed step: the larger of leaf_output / coupling
+    # Dominant kernel of the timed step: the larger of the leaf output (near
+    # field + V^T l_t) and the coupling, each one launch per action, timed with
+    # HIP events on the launch stream.  Algorithmic flops per launch:
+    #   leaf output  2 E_near nrhs + 2 q n nrhs   (near entries, far basis)
+    #   coupling     2 E_far nrhs                 (q x q block products)
     def avg(ph, k):
         return ph[k][0] / max(ph[k][1], 1)
     dom = max(("leaf_output", "coupling"), key=lambda k: avg(ph32, k))
     t_dom = avg(ph32, dom) / 1e3
+    nt = mfma_tiles(nrhs)
     if dom == "leaf_output":
         flops = 2.0 * E_near * nrhs + 2.0 * q * n * nrhs
         entries = E_near
+        kname = f"k_leaf_output_mfma<float, 3, 1, {nt}>"
     else:
         flops = 2.0 * E_far * nrhs
         entries = E_far
+        kname = f"k_coupling_mfma<float, 3, 1, {nt}>"
     achieved = flops / t_dom / 1e12
     roofline = {
         "bound": "mfma",
-        "kernel": "k_leaf_output<float,3,MATERN32,32>" if dom == "leaf_output"
-        else "k_coupling<float,3,MATERN32,32>",
+        "kernel": kname,
         "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": traffic_from_profiles(dom),
+        "traffic": traffic_from_profiles(kname),
         "avg_launch_ms": t_dom * 1e3,
         "algorithmic_flops_per_launch": flops,
         "kernel_entries_per_launch": entries,
@@ -204,8 +262,14 @@ def main():
         "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph32.items()},
         "rhs1": {"fp32_mpts": n / t32_1 / 1e6, "fp64_mpts": n / t64_1 / 1e6,
                  "fp32_ms": t32_1 * 1e3, "fp64_ms": t64_1 * 1e3},
+        "host_boundary": {"fp32_ms": t_host * 1e3, "mpts": n * nrhs / t_host / 1e6,
+                          "note": "fmca_h2_action on host arrays: H2D copy + action + D2H"},
         "setup_s": {"plan": setup_plan, "upload": setup_upload},
+        "device_memory": {"library_peak_bytes": fmca_amd.device_memory()[1],
+                          "torch_peak_bytes": int(torch.cuda.max_memory_allocated())},
     }
+    if solve_res is not None:
+        res["solve"] = solve_res
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(P, xh, info, args.cpu_seconds)
     print(json.dumps(res), flush=True)

@@ -39,6 +39,12 @@ FP32 = _C.FP32
 ROWS = _C.ROWS
 COLS = _C.COLS
 
+
+def device_memory(reset_peak: bool = False):
+    """(live, peak) device bytes held by the library's plans and workspaces."""
+    return _C.device_memory(reset_peak)
+
+
 KERNEL_TYPES = (
     "EXPONENTIAL", "MATERN32", "MATERN52", "MATERN72", "MATERN92", "GAUSSIAN",
     "INVMULTIQUADRIC", "MULTIQUADRIC", "BIHARMONIC2D", "BIHARMONIC3D", "TRIHARMONIC3D",
@@ -205,6 +211,11 @@ class H2kernelActor:
     def blocks(self):
         r, c, lr = self._a.blocks()
         return r, c, lr.astype(bool)
+
+    def launch_order(self, which: str) -> np.ndarray:
+        """Workgroup -> work item of the coupling ("coupling": target cluster)
+        or of the leaf output ("leaf_output": target leaf index); -1 = idle."""
+        return self._a.launch_order({"coupling": 0, "leaf_output": 1}[which])
 
 
 class H2Matrix(H2kernelActor):

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -42,23 +43,45 @@ fmca_status from_hip(hipError_t e, const char *what) {
     if (_e != hipSuccess) return from_hip(_e, #expr);            \
   } while (0)
 
+// Device bytes held by all DevBufs of the process, and their high-water mark.
+std::atomic<int64_t> g_dev_live{0}, g_dev_peak{0};
+
+void note_device_bytes(int64_t delta) {
+  const int64_t now = g_dev_live.fetch_add(delta) + delta;
+  int64_t pk = g_dev_peak.load();
+  while (now > pk && !g_dev_peak.compare_exchange_weak(pk, now)) {
+  }
+}
+
 // A device allocation owned by the actor.
 struct DevBuf {
   void *p = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // requested size
+  size_t held = 0;   // allocated size
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      note_device_bytes(-(int64_t)held);
+    }
+    p = nullptr;
+    bytes = held = 0;
   }
   hipError_t alloc(size_t b) {
     if (b <= bytes && p) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(b, 256));
-    if (e == hipSuccess) bytes = b;
+    release();
+    const size_t h = std::max<size_t>(b, 256);
+    hipError_t e = hipMalloc(&p, h);
+    if (e == hipSuccess) {
+      bytes = b;
+      held = h;
+      note_device_bytes((int64_t)h);
+    } else {
+      p = nullptr;
+    }
     return e;
   }
   template <typename T>
@@ -84,25 +107,28 @@ struct DevTreeReals {
 };
 
 struct DevPattern {
-  DevBuf far_ptr, far_src, far_targets, near_ptr, near_src;
+  DevBuf far_ptr, far_src, far_order, near_ptr, near_src, near_order;
   DevPatternView view(const Pattern &p) const {
     DevPatternView v;
-    v.nfar_targets = (int)p.far_targets.size();
     v.far_ptr = far_ptr.as<int32_t>();
     v.far_src = far_src.as<int32_t>();
-    v.far_targets = far_targets.as<int32_t>();
+    v.nfar_order = (int)p.far_order.size();
+    v.far_order = far_order.as<int32_t>();
     v.nnear_leaves = (int)p.near_ptr.size() - 1;
     v.near_ptr = near_ptr.as<int32_t>();
     v.near_src = near_src.as<int32_t>();
+    v.nnear_order = (int)p.near_order.size();
+    v.near_order = near_order.as<int32_t>();
     return v;
   }
   hipError_t upload(const Pattern &p) {
     hipError_t e;
     if ((e = far_ptr.upload(p.far_ptr)) != hipSuccess) return e;
     if ((e = far_src.upload(p.far_src)) != hipSuccess) return e;
-    if ((e = far_targets.upload(p.far_targets)) != hipSuccess) return e;
+    if ((e = far_order.upload(p.far_order)) != hipSuccess) return e;
     if ((e = near_ptr.upload(p.near_ptr)) != hipSuccess) return e;
-    return near_src.upload(p.near_src);
+    if ((e = near_src.upload(p.near_src)) != hipSuccess) return e;
+    return near_order.upload(p.near_order);
   }
 };
 
@@ -616,6 +642,16 @@ fmca_status fmca_h2_clusters(const fmca_h2_actor *actor, int which, int64_t *beg
   return FMCA_OK;
 }
 
+fmca_status fmca_h2_launch_order(const fmca_h2_actor *actor, int which, int32_t *out,
+                                 int64_t *len) {
+  if (!actor || !len) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (which != 0 && which != 1) return fail(FMCA_ERR_INVALID_ARGUMENT, "which must be 0 or 1");
+  const std::vector<int32_t> &o = which == 0 ? actor->pat.far_order : actor->pat.near_order;
+  *len = (int64_t)o.size();
+  if (out) std::copy(o.begin(), o.end(), out);
+  return FMCA_OK;
+}
+
 fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *col,
                            int8_t *low_rank) {
   if (!actor || !row || !col || !low_rank) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
@@ -824,6 +860,13 @@ fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t coun
       actor->phase_count[k] = 0;
     }
   }
+  return FMCA_OK;
+}
+
+fmca_status fmca_device_memory(int64_t *live_bytes, int64_t *peak_bytes, int reset_peak) {
+  if (live_bytes) *live_bytes = g_dev_live.load();
+  if (peak_bytes) *peak_bytes = g_dev_peak.load();
+  if (reset_peak) g_dev_peak.store(g_dev_live.load());
   return FMCA_OK;
 }
 

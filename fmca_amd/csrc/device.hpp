@@ -32,13 +32,15 @@ struct DevTreeView {
 
 // Interaction lists of one (target tree, source tree) pair.
 struct DevPatternView {
-  int nfar_targets = 0;
-  const int32_t *far_ptr = nullptr;      // ncl_target + 1
+  const int32_t *far_ptr = nullptr;     // ncl_target + 1
   const int32_t *far_src = nullptr;
-  const int32_t *far_targets = nullptr;  // targets with a non-empty far list
+  int nfar_order = 0;                   // coupling grid size
+  const int32_t *far_order = nullptr;   // workgroup -> target cluster (-1: none)
   int nnear_leaves = 0;
-  const int32_t *near_ptr = nullptr;     // nleaves_target + 1
+  const int32_t *near_ptr = nullptr;    // nleaves_target + 1
   const int32_t *near_src = nullptr;
+  int nnear_order = 0;                  // leaf output grid size
+  const int32_t *near_order = nullptr;  // workgroup -> target leaf index (-1: none)
 };
 
 template <typename T>

@@ -189,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
   const int P1 = ip.P1, q = ip.q;
   T *Y = reinterpret_cast<T *>(smem_raw);  // [D][P1] source nodes
   T *Mv = Y + D * kMaxP1;                  // [q][NR] source moments
-  const int t = blockIdx.x;
+  const int t = pat.far_order[blockIdx.x];
+  if (t < 0) return;
   const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
   for (int rc = 0; rc < nrhs; rc += NR) {
     const int nr = min(NR, nrhs - rc);
@@ -343,7 +344,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
   T *Mv = U;                               // [kAlphaTile][NR] far-field coefficients
   T *Ys = U;                               // [D][kSrcTile] source points
   T *Xs = U + D * kSrcTile;                // [kSrcTile][NR] source rhs
-  const int li = blockIdx.x;
+  const int li = pat.near_order[blockIdx.x];
+  if (li < 0) return;
   const int c = rows.leaves[li];
   const int b0 = rows.begin[c], sz = rows.size[c];
   const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
@@ -453,7 +455,7 @@ hipError_t launch_far_near(const DevTreeView<T> &rows, const DevTreeView<T> &col
   const int q = ip.q;
   const int threads = q <= 64 ? 64 : (q <= 128 ? 128 : kBlock);
   const size_t lds = coupling_lds<T, NR>(D, q);
-  hipLaunchKernelGGL((k_coupling<T, D, KT, NR>), dim3(rows.ncl), dim3(threads), lds, stream, rows,
+  hipLaunchKernelGGL((k_coupling<T, D, KT, NR>), dim3(pat.nfar_order), dim3(threads), lds, stream, rows,
                      cols, pat, ip, kp, nrhs, buf.mom, buf.tlhs);
   return hipGetLastError();
 }
@@ -464,7 +466,7 @@ hipError_t launch_output(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
                          int64_t ldo, bool natural, hipStream_t stream) {
   const size_t lds = output_lds<T, NR>(D, ip.P1);
-  hipLaunchKernelGGL((k_leaf_output<T, D, KT, NR>), dim3(rows.nleaves), dim3(kBlock), lds,
+  hipLaunchKernelGGL((k_leaf_output<T, D, KT, NR>), dim3(pat.nnear_order), dim3(kBlock), lds,
                      stream, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.tlhs, out, ldo,
                      natural ? 1 : 0);
   return hipGetLastError();

@@ -90,7 +90,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int li = blockIdx.x;
+  const int li = pat.near_order[blockIdx.x];
+  if (li < 0) return;
   const int c = rows.leaves[li];
   const int b0 = rows.begin[c], sz = rows.size[c];
   const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
@@ -255,7 +256,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int t = blockIdx.x;
+  const int t = pat.far_order[blockIdx.x];
+  if (t < 0) return;
   const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
     int rem = e < q ? e : 0, pk = 0;
@@ -374,11 +376,11 @@ hipError_t launch_mfma_nt(bool output, const DevTreeView<T> &rows, const DevTree
 #define FMCA_MFMA_CASE(K)                                                                   \
   case K:                                                                                   \
     if (output)                                                                             \
-      hipLaunchKernelGGL((k_leaf_output_mfma<T, D, K, NT>), dim3(rows.nleaves), dim3(256),  \
+      hipLaunchKernelGGL((k_leaf_output_mfma<T, D, K, NT>), dim3(pat.nnear_order), dim3(256),  \
                          lds_out, stream, rows, cols,                                       \
                          pat, ip, kp, nrhs, xs, tlhs, out, ldo, natural ? 1 : 0);           \
     else                                                                                    \
-      hipLaunchKernelGGL((k_coupling_mfma<T, D, K, NT>), dim3(rows.ncl), dim3(256),         \
+      hipLaunchKernelGGL((k_coupling_mfma<T, D, K, NT>), dim3(pat.nfar_order), dim3(256),         \
                          lds_cpl, stream, rows, cols, pat, ip, kp,                          \
                          nrhs, mom, tlhs);                                                  \
     break;
@@ -389,11 +391,11 @@ hipError_t launch_mfma_nt(bool output, const DevTreeView<T> &rows, const DevTree
     FMCA_MFMA_CASE(KT_GAUSSIAN)
     default:
       if (output)
-        hipLaunchKernelGGL((k_leaf_output_mfma<T, D, -1, NT>), dim3(rows.nleaves), dim3(256),
+        hipLaunchKernelGGL((k_leaf_output_mfma<T, D, -1, NT>), dim3(pat.nnear_order), dim3(256),
                            lds_out, stream, rows, cols, pat, ip, kp, nrhs, xs, tlhs, out, ldo,
                            natural ? 1 : 0);
       else
-        hipLaunchKernelGGL((k_coupling_mfma<T, D, -1, NT>), dim3(rows.ncl), dim3(256),
+        hipLaunchKernelGGL((k_coupling_mfma<T, D, -1, NT>), dim3(pat.nfar_order), dim3(256),
                            lds_cpl, stream, rows, cols, pat, ip, kp, nrhs, mom, tlhs);
   }
 #undef FMCA_MFMA_CASE

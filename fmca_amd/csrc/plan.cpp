@@ -387,7 +387,6 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
   }
   for (int c = 0; c < R.ncl; ++c) {
     std::sort(pat.far_src.begin() + pat.far_ptr[c], pat.far_src.begin() + pat.far_ptr[c + 1]);
-    if (pat.far_ptr[c + 1] > pat.far_ptr[c]) pat.far_targets.push_back(c);
   }
   pat.far_entries = double(pat.n_far) * q * q;
   // near field CSR by target leaf; a dense block whose row cluster is not a
@@ -429,6 +428,41 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
   for (int k = 0; k < nl; ++k)
     std::sort(pat.near_src.begin() + pat.near_ptr[k], pat.near_src.begin() + pat.near_ptr[k + 1],
               [&](int a, int b) { return C.begin[a] < C.begin[b]; });
+  // launch orders: coupling work = far blocks (plus the zero fill of an empty
+  // list), deepest level first; leaf output work = near-field entries
+  std::vector<double> wf(R.ncl), wn(nl, 0.);
+  for (int c = 0; c < R.ncl; ++c) wf[c] = 0.05 + (pat.far_ptr[c + 1] - pat.far_ptr[c]);
+  std::vector<std::pair<int, int>> levels;
+  for (int L = R.depth; L >= 0; --L) levels.push_back({R.level_ptr[L], R.level_ptr[L + 1]});
+  pat.far_order = xcd_launch_order(wf, levels, kXcds);
+  for (int k = 0; k < nl; ++k)
+    for (int e = pat.near_ptr[k]; e < pat.near_ptr[k + 1]; ++e)
+      wn[k] += double(R.size[R.leaves[k]]) * C.size[pat.near_src[e]];
+  pat.near_order = xcd_launch_order(wn, {{0, nl}}, kXcds);
+}
+
+std::vector<int32_t> xcd_launch_order(const std::vector<double> &work,
+                                      const std::vector<std::pair<int, int>> &ranges, int nxcd) {
+  std::vector<std::vector<int32_t>> lists(nxcd);
+  for (const auto &rg : ranges) {
+    double total = 0;
+    for (int i = rg.first; i < rg.second; ++i) total += work[i];
+    double acc = 0;
+    for (int i = rg.first; i < rg.second; ++i) {
+      // the piece that holds the midpoint of item i's work
+      int x = total > 0 ? (int)((acc + 0.5 * work[i]) / total * nxcd)
+                        : (int)((int64_t)(i - rg.first) * nxcd / (rg.second - rg.first));
+      x = std::min(std::max(x, 0), nxcd - 1);
+      lists[x].push_back(i);
+      acc += work[i];
+    }
+  }
+  size_t len = 0;
+  for (const auto &l : lists) len = std::max(len, l.size());
+  std::vector<int32_t> order(len * nxcd, -1);
+  for (int x = 0; x < nxcd; ++x)
+    for (size_t k = 0; k < lists[x].size(); ++k) order[k * nxcd + x] = lists[x][k];
+  return order;
 }
 
 // ---------------------------------------------------------------------------

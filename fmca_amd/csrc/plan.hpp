@@ -34,6 +34,7 @@ int parse_kernel_type(const char *name);
 const char *kernel_type_name(int kt);
 
 constexpr int kMaxDim = 3;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 constexpr int kMaxP1 = 10;  // polynomial degree <= 9
 
 // Tensor-product Chebyshev interpolation on [0,1]^d
@@ -78,14 +79,27 @@ struct Pattern {
   std::vector<int8_t> block_lr;
   // far field: CSR over target clusters -> source clusters
   std::vector<int32_t> far_ptr, far_src;
-  std::vector<int32_t> far_targets;  // target clusters with a non-empty far list
   // near field: CSR over target leaves (index into Tree::leaves) -> source clusters
   std::vector<int32_t> near_ptr, near_src;
+  // launch orders (one workgroup per entry, -1 = nothing to do, see
+  // xcd_launch_order): target clusters of the coupling, target leaves of the
+  // leaf output
+  std::vector<int32_t> far_order, near_order;
   double near_entries = 0, far_entries = 0;
   int64_t n_far = 0, n_near = 0;
 };
 
 void compute_pattern(Pattern &pat, const Tree &rows, const Tree &cols, double eta, int q);
+
+// Launch order of one workgroup per item on MI355X, whose 8 XCDs (one L2
+// each) receive the workgroups of a launch round-robin (workgroup b on XCD
+// b % 8).  Each range [lo, hi) of items, taken in the order given, is cut into
+// nxcd contiguous pieces of about equal work, piece x for XCD x: every XCD
+// then works on tree neighbours, which read the same sources through its L2,
+// and the XCDs get equal work.  order[nxcd * k + x] is the k-th item of XCD x,
+// -1 where XCD x has fewer items.
+std::vector<int32_t> xcd_launch_order(const std::vector<double> &work,
+                                      const std::vector<std::pair<int, int>> &ranges, int nxcd);
 
 // Transfer factors: E_c = T_c,0 (x) T_c,1 (x) T_c,2 for every non-root cluster c
 // (FMCA/src/H2Matrix/compute_cluster_bases_impl.h:36-51), stored as

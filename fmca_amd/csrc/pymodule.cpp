@@ -118,6 +118,14 @@ py::tuple blocks(const Actor &a) {
   return py::make_tuple(row, col, lr);
 }
 
+py::array_t<int32_t> launch_order(const Actor &a, int which) {
+  int64_t len = 0;
+  check(fmca_h2_launch_order(a.h, which, nullptr, &len));
+  py::array_t<int32_t> out((py::ssize_t)len);
+  check(fmca_h2_launch_order(a.h, which, out.mutable_data(), &len));
+  return out;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_fmca_amd_py, m) {
@@ -242,7 +250,8 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
           py::arg("reset") = false)
       .def("info", &info)
       .def("clusters", &clusters)
-      .def("blocks", &blocks);
+      .def("blocks", &blocks)
+      .def("launch_order", &launch_order, py::arg("which"));
   m.def(
       "diag_math",
       [](int fn, uintptr_t x, uintptr_t y, int64_t n, uintptr_t stream) {
@@ -251,5 +260,11 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
                              reinterpret_cast<double *>(y), n, reinterpret_cast<void *>(stream)));
       },
       py::arg("fn"), py::arg("x"), py::arg("y"), py::arg("n"), py::arg("stream") = 0);
+  m.def("device_memory", [](bool reset_peak) {
+    int64_t live = 0, peak = 0;
+    check(fmca_device_memory(&live, &peak, reset_peak ? 1 : 0));
+    return py::make_tuple(live, peak);
+  }, py::arg("reset_peak") = false,
+        "(live, peak) device bytes held by the library's plans and workspaces");
   m.def("status_string", [](int s) { return std::string(fmca_status_string((fmca_status)s)); });
 }

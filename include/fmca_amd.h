@@ -183,6 +183,15 @@ fmca_status fmca_h2_info(const fmca_h2_actor *actor, double out[13]);
 fmca_status fmca_h2_clusters(const fmca_h2_actor *actor, int which, int64_t *begin,
                              int64_t *size, int32_t *level, int32_t *sons, double *bb);
 
+/* Launch order of the forward action's coupling (which = 0: target cluster
+ * per workgroup) or leaf output (which = 1: target leaf index into the row
+ * tree's leaves), -1 for an idle workgroup; the workgroups of a launch go to
+ * the 8 XCDs round-robin and each XCD gets a contiguous piece of every tree
+ * level of about equal work.  No FMCA counterpart (FMCA schedules with OpenMP,
+ * H2kernelActor.h:30-34).  out == NULL: *len receives the length. */
+fmca_status fmca_h2_launch_order(const fmca_h2_actor *actor, int which, int32_t *out,
+                                 int64_t *len);
+
 /* FMCA's leaf blocks: row / column block ids and low-rank flag; count = far + near. */
 fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *col,
                            int8_t *low_rank);
@@ -201,6 +210,12 @@ fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t coun
  * 2 FP32) or sqrt (fn 1 FP64, 3 FP32) that the kernel-entry loops use, on n
  * device values, so that their accuracy can be tested. */
 fmca_status fmca_diag_math(int fn, const double *d_x, double *d_y, int64_t n, void *stream);
+
+/* Device memory held by the library (every actor of the process): bytes
+ * allocated now and the high-water mark since load or the last reset_peak.
+ * No FMCA counterpart (FMCA allocates on the host heap); it states the device
+ * footprint of a plan and its workspaces. */
+fmca_status fmca_device_memory(int64_t *live_bytes, int64_t *peak_bytes, int reset_peak);
 
 /* Static string for a status code. */
 const char *fmca_status_string(fmca_status status);

@@ -155,3 +155,37 @@ def test_moment_order_table_matches_fmca():
         lin = [sum(a[i] * P1 ** i for i in range(d)) for a in mi]
         assert sorted(lin) == list(range(P1 ** d))
         assert lin[0] == 0 and lin[1:d + 1] == sorted(P1 ** i for i in range(d))[::-1]
+
+
+@pytest.mark.parametrize("points", ["uniform", "clustered"])
+def test_launch_orders_cover_every_item_once_with_balanced_xcd_pieces(points):
+    """The coupling / leaf-output launch orders: every target cluster (leaf)
+    exactly once, -1 padding only, 8 XCD pieces per tree level that are
+    contiguous, and per-level XCD work within 1.35x of the level mean where a
+    level has at least 256 items."""
+    from fmca_amd import workloads as W
+    n = 40000
+    P = W.uniform(n, 3, 11) if points == "uniform" else W.gaussian_mixture(n, 3, 16, 0.02, 11)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 3, 0.8)
+    cl = H.clusters(fmca_amd.ROWS)
+    ncl = len(cl["level"])
+    row, col, lr = H.blocks()
+    far = np.bincount(row[lr], minlength=ncl).astype(float) + 0.05
+    leaves = np.flatnonzero(cl["sons"][:, 0] < 0)
+    for which, nitems in (("coupling", ncl), ("leaf_output", len(leaves))):
+        order = H.launch_order(which)
+        assert len(order) % 8 == 0
+        items = order[order >= 0]
+        assert np.array_equal(np.sort(items), np.arange(nitems))
+        assert np.all(order[order < 0] == -1)
+        if which != "coupling":
+            continue
+        xcd = np.empty(nitems, dtype=int)
+        xcd[order[order >= 0]] = np.flatnonzero(order >= 0) % 8
+        for L in np.unique(cl["level"]):
+            ids = np.flatnonzero(cl["level"] == L)
+            # contiguous pieces: the XCD index never decreases along the level
+            assert np.all(np.diff(xcd[ids]) >= 0)
+            if len(ids) >= 256:
+                w = np.bincount(xcd[ids], weights=far[ids], minlength=8)
+                assert w.max() <= 1.35 * w.mean(), (L, w)
