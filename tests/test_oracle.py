@@ -249,3 +249,67 @@ def test_pcg_solves_regularised_system():
     x, it, err = A.pcg(b, ridge=1e-2, tol=1e-8)
     r = b - A.action(x)[:, 0] - 1e-2 * x
     assert err <= 1e-8 and np.linalg.norm(r) / np.linalg.norm(b) <= 1.1e-8 and it > 1
+
+
+def test_dense_action_matches_scipy_pairwise_distances():
+    """dense_action / dense_matrix against K built with scipy's cdist and the
+    closed-form kernels (FMCA's CovarianceKernel.h:64-70,93-94)."""
+    from scipy.spatial.distance import cdist
+    rng = np.random.default_rng(21)
+    P, Pe = rng.random((3, 700)), rng.random((3, 500))
+    x = rng.standard_normal((700, 2))
+    R = cdist(Pe.T, P.T)
+    forms = {"EXPONENTIAL": lambda r, l: np.exp(-r / l),
+             "MATERN32": lambda r, l: (1 + np.sqrt(3) * r / l) * np.exp(-np.sqrt(3) * r / l),
+             "GAUSSIAN": lambda r, l: np.exp(-r * r / (2 * l * l))}
+    for name, f in forms.items():
+        K = f(R, 0.4)
+        assert np.allclose(oracle.dense_matrix(P, Pe, name, 0.4), K, rtol=1e-13, atol=1e-15)
+        assert rel_err(oracle.dense_action(P, Pe, name, 0.4, x), K @ x) < 1e-13
+
+
+@pytest.mark.parametrize("dim,deg,kernel", [(2, 3, "MATERN32"), (3, 2, "EXPONENTIAL")])
+def test_action_equals_blockwise_assembly(dim, deg, kernel):
+    """The oracle's recursive action (upward transfers, coupling, downward
+    transfers, near field) equals the H2 matrix assembled block by block in
+    numpy from its own block list and boxes:
+        y = sum_dense K(t, s) x_s + sum_low-rank V_t^T K(xi_t, xi_s) V_s x_s
+    with V_c[a, p] = L_a((x_p - bbmin_c) / bbsize_c) evaluated directly for
+    every cluster (no transfer matrices) and xi_c = Xi * bbsize_c + bbmin_c
+    (unsymmetricNystromEvaluator.h:42-73, H2kernelActor.h:38-80)."""
+    from scipy.spatial.distance import cdist
+    rng = np.random.default_rng(22)
+    n, m = 2400, 2000
+    P, Pe = rng.random((dim, n)), rng.random((dim, m))
+    x = rng.standard_normal((n, 2))
+    l = 0.4
+    A = oracle.OracleH2(P, Pe, kernel, l, deg, 0.8)
+    f = {"EXPONENTIAL": lambda r: np.exp(-r / l),
+         "MATERN32": lambda r: (1 + np.sqrt(3) * r / l) * np.exp(-np.sqrt(3) * r / l)}[kernel]
+    _, _, _, Xi = oracle.interp(dim, deg)
+    rows, cols = A.clusters(0), A.clusters(1)
+    ri, ci = A.indices(0), A.indices(1)
+
+    def pts(tree, idx, Pm, c):
+        b, s = tree["begin"][c], tree["size"][c]
+        return Pm[:, idx[b:b + s]], idx[b:b + s]
+
+    def basis(tree, idx, Pm, c):
+        X, _ = pts(tree, idx, Pm, c)
+        U = (X - tree["bbmin"][c][:, None]) / tree["bbsize"][c][:, None]
+        return oracle.eval_polynomials(dim, deg, U)
+
+    y = np.zeros((m, 2))
+    row, col, lr = A.blocks()
+    assert lr.any() and (~lr).any()
+    for t, s, low in zip(row, col, lr):
+        Xt, it = pts(rows, ri, Pe, t)
+        Xs, is_ = pts(cols, ci, P, s)
+        if low:
+            xt = Xi * rows["bbsize"][t][:, None] + rows["bbmin"][t][:, None]
+            xs = Xi * cols["bbsize"][s][:, None] + cols["bbmin"][s][:, None]
+            S = f(cdist(xt.T, xs.T))
+            y[it] += basis(rows, ri, Pe, t).T @ (S @ (basis(cols, ci, P, s) @ x[is_]))
+        else:
+            y[it] += f(cdist(Xt.T, Xs.T)) @ x[is_]
+    assert rel_err(A.action(x), y) < 1e-12
