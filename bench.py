@@ -46,6 +46,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-solve", action="store_true")
     p.add_argument("--ridge", type=float, default=100.0)
+    p.add_argument("--workload", default="matvec",
+                   choices=["matvec", "small-dense", "clustered-cg", "autograd", "large"],
+                   help="matvec: the headline line (default); the others print a side line "
+                        "for the other configurations")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="also run one elementwise copy of this size (reads and writes "
                         "M MiB) as the reference kernel for HBM traffic counters")
@@ -146,6 +150,164 @@ def cpu_baseline(P, x, info, budget_seconds):
     }
 
 
+def _sync_time(fn, reps=1):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def side_small_dense(args):
+    """n = 1e4 uniform 3-D, EXPONENTIAL l = 0.5, deg 3, eta 0.8, one rhs
+    U[-1,1]: H2 error of the FP64 and FP32 actions against dense K x (FP64,
+    formed on the GPU with torch), GPU times, and the CPU oracle's full action."""
+    import torch
+    import fmca_amd
+    import oracle
+    n = 10000
+    rng = np.random.default_rng(0)
+    P = np.asfortranarray(rng.random((3, n)))
+    xh = np.asfortranarray(rng.uniform(-1, 1, (n, 1)))
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("EXPONENTIAL", 0.5), P, P, 3, 0.8)
+    x = torch.from_numpy(xh).cuda()
+    Pt = torch.from_numpy(np.ascontiguousarray(P.T)).cuda()
+    yd = torch.exp(-torch.cdist(Pt, Pt) / 0.5) @ x
+    res = {"workload": "small-dense: n=1e4 uniform 3D, EXPONENTIAL l=0.5, deg 3, eta 0.8, 1 rhs"}
+    for name, prec in (("fp64", fmca_amd.FP64), ("fp32", fmca_amd.FP32)):
+        H.action_device(x, precision=prec)
+        t, y = _sync_time(lambda: H.action_device(x, precision=prec), 10)
+        res[name] = {"ms": t * 1e3, "rel_err_vs_dense": float(torch.linalg.norm(y - yd)
+                                                              / torch.linalg.norm(yd))}
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 3, 0.8)
+    nthreads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    t0 = time.perf_counter()
+    yo = A.action(xh, nthreads)
+    res["cpu_oracle"] = {"ms": (time.perf_counter() - t0) * 1e3, "threads": nthreads,
+                         "fp64_gpu_vs_oracle": float(np.linalg.norm(
+                             H.action(xh) - yo) / np.linalg.norm(yo))}
+    res["info"] = H.info()
+    return res
+
+
+def side_clustered_cg(args):
+    """n = 1e6 points from a mixture of 16 isotropic Gaussians (sigma 0.02,
+    centres uniform in [0,1]^3), GAUSSIAN l = 0.1, deg 4, eta 0.8, 64 rhs
+    y = sin(2 pi x1) cos(2 pi x2) + 0.01 N(0,1): matvec times per phase, FP32
+    vs FP64 difference, and CG on (K + 1e-4 I) a = y (bounded iterations)."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    n, nrhs = 1_000_000, 64
+    P = W.gaussian_mixture(n, 3, 16, 0.02, 0)
+    t0 = time.perf_counter()
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("GAUSSIAN", 0.1), P, P, 4, 0.8)
+    plan = time.perf_counter() - t0
+    y = torch.from_numpy(W.smooth_rhs(P, nrhs)).cuda()
+    out = torch.empty((nrhs, n), dtype=torch.float64, device="cuda").t()
+    res = {"workload": "clustered-cg: n=1e6 16-Gaussian mixture (sigma 0.02), GAUSSIAN l=0.1, "
+                       "deg 4, eta 0.8, 64 rhs", "plan_s": plan, "info": H.info()}
+    ys = {}
+    for name, prec in (("fp32", fmca_amd.FP32), ("fp64", fmca_amd.FP64)):
+        t, ph = timed_steps(H, y, out, prec, 2, 1, True)
+        ys[name] = out.clone()
+        res[name] = {"ms": t * 1e3, "mpts": n * nrhs / t / 1e6,
+                     "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph.items()}}
+    res["fp32_vs_fp64_rel_err"] = float(torch.linalg.norm(ys["fp32"] - ys["fp64"])
+                                        / torch.linalg.norm(ys["fp64"]))
+    del ys, out
+    try:
+        a, info = H.cg(y, x0=torch.zeros_like(y), ridge=1e-4, tol=1e-8,
+                       precision=fmca_amd.FP32, max_iter=args.steps * 5)
+    except RuntimeError as e:
+        info = {"error": str(e)}
+    res["cg_fp32_refined_ridge_1e-4"] = info
+    return res
+
+
+def side_autograd(args):
+    """n = 2e5 uniform 2-D, MATERN32 l = 0.3, deg 5, eta 0.8, 8 rhs N(0,1):
+    L = 0.5 ||K(l) alpha - y||^2, forward and backward (d alpha, d l, d points)
+    times of the differentiable H2 matvec, plain FP64 form and FP32 form."""
+    import torch
+    from fmca_amd import FP32, FP64
+    from fmca_amd.autograd import H2Spec, h2_matvec
+    n, nrhs = 200_000, 8
+    rng = np.random.default_rng(0)
+    pts = torch.from_numpy(rng.random((2, n))).cuda().requires_grad_(True)
+    alpha = torch.from_numpy(rng.standard_normal((n, nrhs))).cuda().requires_grad_(True)
+    yt = torch.from_numpy(rng.standard_normal((n, nrhs))).cuda()
+    l = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    res = {"workload": "autograd: n=2e5 uniform 2D, MATERN32 l=0.3, deg 5, eta 0.8, 8 rhs"}
+    for name, prec in (("fp64", FP64), ("fp32", FP32)):
+        spec = H2Spec("MATERN32", deg=5, eta=0.8, precision=prec)
+        h2_matvec(alpha, l, pts, spec)  # plan
+        tf, tb = [], []
+        for _ in range(max(3, args.steps)):
+            for g in (alpha, l, pts):
+                g.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss = 0.5 * (h2_matvec(alpha, l, pts, spec) - yt).pow(2).sum()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            tf.append(t1 - t0)
+            tb.append(t2 - t1)
+        f, b = min(tf), min(tb)
+        res[name] = {"forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f}
+    return res
+
+
+def side_large(args):
+    """n = 8e6 uniform 3-D, EXPONENTIAL l = 0.25, deg 4, eta 0.8, 16 rhs:
+    plan time, FP32 forward, backward (d alpha, d l) with moments recomputed,
+    their ratio, and the device-memory high-water mark."""
+    import torch
+    import fmca_amd
+    from fmca_amd.autograd import H2Spec, h2_matvec
+    n, nrhs = 8_000_000, 16
+    rng = np.random.default_rng(0)
+    P = rng.random((3, n))
+    fmca_amd.device_memory(reset_peak=True)
+    torch.cuda.reset_peak_memory_stats()
+    pts = torch.from_numpy(P).cuda()
+    alpha = torch.from_numpy(rng.standard_normal((n, nrhs))).cuda().requires_grad_(True)
+    l = torch.tensor(0.25, dtype=torch.float64, requires_grad=True)
+    spec = H2Spec("EXPONENTIAL", deg=4, eta=0.8, precision=fmca_amd.FP32)
+    t0 = time.perf_counter()
+    y = h2_matvec(alpha, l, pts, spec)
+    torch.cuda.synchronize()
+    first = time.perf_counter() - t0
+    res = {"workload": "large: n=8e6 uniform 3D, EXPONENTIAL l=0.25, deg 4, eta 0.8, 16 rhs, "
+                       "FP32 form", "first_forward_incl_plan_s": first,
+           "info": spec._actor.info()}
+    tf, tb = [], []
+    for _ in range(max(2, args.steps // 2)):
+        alpha.grad = None
+        l.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y = h2_matvec(alpha, l, pts, spec)
+        loss = 0.5 * y.pow(2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        tf.append(t1 - t0)
+        tb.append(time.perf_counter() - t1)
+    f, b = min(tf), min(tb)
+    res.update({"forward_ms": f * 1e3, "mpts": n * nrhs / f / 1e6, "backward_ms": b * 1e3,
+                "backward_over_forward": b / f,
+                "device_memory": {"library_peak_bytes": fmca_amd.device_memory()[1],
+                                  "torch_peak_bytes": int(torch.cuda.max_memory_allocated())}})
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -154,6 +316,12 @@ def main():
         sys.exit(2)
     import torch
     import fmca_amd
+
+    side = {"small-dense": side_small_dense, "clustered-cg": side_clustered_cg,
+            "autograd": side_autograd, "large": side_large}.get(args.workload)
+    if side is not None:
+        print(json.dumps(side(args)), flush=True)
+        return
 
     n, nrhs = args.n, args.nrhs
     rng = np.random.default_rng(0)

@@ -6,10 +6,13 @@ The backward pass returns
   d points  = sum_r alpha_r (s (Phi^T g_r) - Phi^T (g_r * t))   per coordinate
   d targets = sum_r g_r   (t (Phi a_r)   - Phi (a_r * s))
 with Phi the H2 action of phi(r) = k'(r)/r on the same tree, bases and blocks
-(for targets = points both terms add up).  The H2 operator is linear in the
-kernel function, so each of these is one action with a derivative kernel; no
-moments are stored between forward and backward -- every action recomputes
-them from the points and the rhs.
+(for targets = points both terms add up and Phi^T = Phi).  The H2 operator is
+linear in the kernel function, so each of these is an action with a
+derivative kernel.  They run fused: one action whose 16-column groups carry
+different kernel variants (fmca_h2_action_variants_device), so that every
+kernel entry is evaluated once for K, dK/dl and Phi.  No moments are stored
+between forward and backward -- the backward action recomputes them from the
+points and its right-hand sides.
 
 FMCA has no differentiation code; these forms are checked against centred
 finite differences of the H2 forward and against dense FP64 autograd.
@@ -75,6 +78,47 @@ def _variant(actor: H2kernelActor, variant: int, x: torch.Tensor, transpose: boo
     return out
 
 
+def _variants(actor: H2kernelActor, blocks, transpose: bool, precision: int):
+    """Applies the H2 matrix of kernel variant v to every (v, X) of `blocks`
+    (X: rows_in x c) in fused actions: each block fills ceil(c / 16) column
+    groups of one action of up to 8 groups.  Returns the products in order,
+    as column-major views."""
+    if not blocks:
+        return []
+    dev = blocks[0][1].device
+    rows_in = actor.m if transpose else actor.n
+    rows_out = actor.n if transpose else actor.m
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    pieces = []  # (variant, block, first column, end column), 16 columns at most
+    for b, (v, X) in enumerate(blocks):
+        for c0 in range(0, X.shape[1], 16):
+            pieces.append((v, b, c0, min(c0 + 16, X.shape[1])))
+    parts = [[] for _ in blocks]
+    for p0 in range(0, len(pieces), 8):
+        call = pieces[p0:p0 + 8]
+        # the rhs of the action as (ncols, rows_in) row-major = column-major (rows_in, ncols)
+        tens = []
+        for g, (v, b, c0, c1) in enumerate(call):
+            tens.append(blocks[b][1][:, c0:c1].t())
+            if c1 - c0 < 16 and g < len(call) - 1:
+                tens.append(torch.zeros((16 - (c1 - c0), rows_in), dtype=torch.float64,
+                                        device=dev))
+        Ut = torch.cat(tens, dim=0).to(torch.float64)
+        ncols = Ut.shape[0]
+        Yt = torch.empty((ncols, rows_out), dtype=torch.float64, device=dev)
+        actor._a.action_variants_device([v for v, _, _, _ in call], transpose, Ut.data_ptr(),
+                                        rows_in, ncols, Yt.data_ptr(), rows_out, precision,
+                                        stream)
+        # the groups of one block are adjacent (padding only ends a block)
+        first = {}
+        for g, (v, b, c0, c1) in enumerate(call):
+            g0, w = first.get(b, (g, 0))
+            first[b] = (g0, 16 * (g - g0) + (c1 - c0))
+        for b, (g0, w) in first.items():
+            parts[b].append(Yt[16 * g0:16 * g0 + w])
+    return [(p[0] if len(p) == 1 else torch.cat(p, dim=0)).t() for p in parts]
+
+
 class _H2Matvec(torch.autograd.Function):
     @staticmethod
     def forward(ctx, alpha, lengthscale, points, targets, spec):
@@ -94,42 +138,66 @@ class _H2Matvec(torch.autograd.Function):
     def backward(ctx, grad):
         alpha, lengthscale, points, targets = ctx.saved_tensors
         actor, prec = ctx.actor, ctx.spec.precision
-        G = grad.reshape(-1, 1) if ctx.vector else grad
-        G = G.to(torch.float64)
-        A = alpha.reshape(-1, 1) if ctx.vector else alpha
-        d_alpha = d_l = d_points = d_targets = None
-        if ctx.needs_input_grad[0]:
-            d_alpha = _variant(actor, KERNEL_VALUE, G, True, prec)
-            d_alpha = d_alpha[:, 0] if ctx.vector else d_alpha
-        if ctx.needs_input_grad[1]:
-            Yl = _variant(actor, KERNEL_DL, A, False, prec)
-            d_l = (G * Yl).sum().reshape(lengthscale.shape).to(lengthscale.dtype)
+        G = (grad.reshape(-1, 1) if ctx.vector else grad).to(torch.float64)
+        A = (alpha.reshape(-1, 1) if ctx.vector else alpha).to(torch.float64)
+        need_a, need_l = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         need_s = ctx.needs_input_grad[2]
         need_t = ctx.needs_input_grad[3] or (ctx.symmetric and need_s)
-        if need_s or need_t:
-            dev = A.device
-            S = points.detach().to(dev, torch.float64)     # (d, n) sources
-            T = targets.detach().to(dev, torch.float64)    # (d, m) targets
-            d, r = S.shape[0], A.shape[1]
-            if need_t:
-                # targets: sum_r g_r (t (Phi a_r) - Phi (a_r * s))
-                U = torch.cat([A] + [A * S[k][:, None] for k in range(d)], dim=1)
-                V = _variant(actor, KERNEL_PHI, U, False, prec)
-                Va = V[:, :r]
-                gt = torch.stack([(G * (T[k][:, None] * Va - V[:, (k + 1) * r:(k + 2) * r])).sum(1)
-                                  for k in range(d)])
+        dev = A.device
+        r = A.shape[1]
+        S = points.detach().to(dev, torch.float64) if (need_s or need_t) else None  # (d, n)
+        T = targets.detach().to(dev, torch.float64) if (need_s or need_t) else None  # (d, m)
+        d = S.shape[0] if S is not None else 0
+        # blocks of the action on the forward pattern and of the transposed one
+        fwd, trn = [], []
+        if need_l:
+            fwd.append(("dl", KERNEL_DL, A))
+        if ctx.symmetric:
+            if need_a:
+                fwd.append(("alpha", KERNEL_VALUE, G))
             if need_s:
-                # sources: sum_r a_r (s (Phi^T g_r) - Phi^T (g_r * t))
-                U = torch.cat([G] + [G * T[k][:, None] for k in range(d)], dim=1)
-                W = _variant(actor, KERNEL_PHI, U, True, prec)
-                Wg = W[:, :r]
-                gs = torch.stack([(A * (S[k][:, None] * Wg - W[:, (k + 1) * r:(k + 2) * r])).sum(1)
+                fwd.append(("phi", KERNEL_PHI, torch.cat(
+                    [A, G] + [A * S[k][:, None] for k in range(d)]
+                    + [G * S[k][:, None] for k in range(d)], dim=1)))
+        else:
+            if need_a:
+                trn.append(("alpha", KERNEL_VALUE, G))
+            if need_t:
+                fwd.append(("phi_t", KERNEL_PHI, torch.cat(
+                    [A] + [A * S[k][:, None] for k in range(d)], dim=1)))
+            if need_s:
+                trn.append(("phi_s", KERNEL_PHI, torch.cat(
+                    [G] + [G * T[k][:, None] for k in range(d)], dim=1)))
+        res = {}
+        for blocks, transpose in ((fwd, False), (trn, True)):
+            outs = _variants(actor, [(v, X) for _, v, X in blocks], transpose, prec)
+            res.update({name: Y for (name, _, _), Y in zip(blocks, outs)})
+        d_alpha = d_l = d_points = d_targets = None
+        if need_a:
+            d_alpha = res["alpha"][:, 0] if ctx.vector else res["alpha"]
+        if need_l:
+            d_l = (G * res["dl"]).sum().reshape(lengthscale.shape).to(lengthscale.dtype)
+        if ctx.symmetric and need_s:
+            V = res["phi"]
+            PA, PG = V[:, :r], V[:, r:2 * r]
+            g = torch.stack([
+                (G * (S[k][:, None] * PA - V[:, (2 + k) * r:(3 + k) * r])).sum(1)
+                + (A * (S[k][:, None] * PG - V[:, (2 + d + k) * r:(3 + d + k) * r])).sum(1)
+                for k in range(d)])
+            d_points = g.to(points.device, points.dtype)
+        elif not ctx.symmetric:
+            if need_t and ctx.needs_input_grad[3]:
+                V = res["phi_t"]
+                gt = torch.stack([(G * (T[k][:, None] * V[:, :r]
+                                        - V[:, (1 + k) * r:(2 + k) * r])).sum(1)
                                   for k in range(d)])
-            if ctx.symmetric:
-                d_points = (gs + gt).to(points.device, points.dtype)
-            else:
-                d_points = gs.to(points.device, points.dtype) if need_s else None
-                d_targets = gt.to(targets.device, targets.dtype) if ctx.needs_input_grad[3] else None
+                d_targets = gt.to(targets.device, targets.dtype)
+            if need_s:
+                W = res["phi_s"]
+                gs = torch.stack([(A * (S[k][:, None] * W[:, :r]
+                                        - W[:, (1 + k) * r:(2 + k) * r])).sum(1)
+                                  for k in range(d)])
+                d_points = gs.to(points.device, points.dtype)
         return d_alpha, d_l, d_points, d_targets, None
 
 

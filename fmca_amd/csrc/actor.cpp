@@ -296,10 +296,31 @@ InterpParams<T> interp_params(const Interpolation &ip) {
   return p;
 }
 
+// Kernel variant of every 16-column group of the rhs of one action.
+struct VariantGroups {
+  int n = 1;
+  int v[kMaxVariantGroups] = {KV_VALUE};
+  static VariantGroups single(int variant) {
+    VariantGroups g;
+    g.v[0] = variant;
+    return g;
+  }
+  bool uniform() const {
+    for (int i = 1; i < n; ++i)
+      if (v[i] != v[0]) return false;
+    return true;
+  }
+};
+
 template <typename T>
-KernelParams<T> kernel_params(const fmca_h2_actor *a, int variant) {
+KernelParams<T> kernel_params(const fmca_h2_actor *a, const VariantGroups &g) {
   KernelParams<T> k;
-  k.kt = a->kt + KT_COUNT * variant;
+  k.kt = a->kt + KT_COUNT * g.v[0];
+  k.vmask = 0;
+  for (int i = 0; i < kMaxVariantGroups; ++i) {
+    k.tv[i] = g.v[std::min(i, g.n - 1)];
+    k.vmask |= 1 << k.tv[i];
+  }
   k.l = (T)a->l;
   k.c = (T)a->c;
   k.nu = (T)a->nu;
@@ -332,7 +353,7 @@ template <typename T>
 fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r_rows,
                     const DevTreeReals<T> &r_cols, const double *d_rhs, int64_t ld_rhs,
                     int64_t nrhs, double *d_out, int64_t ld_out, bool natural, bool transpose,
-                    hipStream_t stream, int variant) {
+                    hipStream_t stream, const VariantGroups &groups) {
   const Tree &tr_rows = a->rows();
   const Tree &tr_cols = a->cols();
   const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
@@ -364,14 +385,15 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
     pe = &a->pending.back();
   }
   hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
-                                  interp_params<T>(a->ip), kernel_params<T>(a, variant), d_rhs, ld_rhs,
+                                  interp_params<T>(a->ip), kernel_params<T>(a, groups), d_rhs, ld_rhs,
                                   (int)nrhs, d_out, ld_out, natural, buf, stream, pe);
   return from_hip(e, "launch_action");
 }
 
 fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
                         double *d_out, int64_t ld_out, int precision, bool natural,
-                        bool transpose, hipStream_t stream, int variant = KV_VALUE) {
+                        bool transpose, hipStream_t stream,
+                        const VariantGroups &groups = VariantGroups::single(KV_VALUE)) {
   if (transpose && !a->shared && !a->have_pat_t) {
     compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
     HIP_TRY(a->d_pat_t.upload(a->pat_t));
@@ -380,12 +402,22 @@ fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, i
   if (precision == FMCA_PRECISION_FP64)
     return enqueue<double>(a, a->ws64, a->d_rows_64, a->shared ? a->d_rows_64 : a->d_cols_64,
                            d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
-                           variant);
+                           groups);
   if (precision == FMCA_PRECISION_FP32)
     return enqueue<float>(a, a->ws32, a->d_rows_32, a->shared ? a->d_rows_32 : a->d_cols_32,
                           d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
-                          variant);
+                          groups);
   return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+}
+
+// One action whose 16-column groups carry different variants, when both
+// entry phases run on MFMA (the VALU forms evaluate one variant per launch).
+template <typename T>
+bool groups_in_one_action(const fmca_h2_actor *a, const VariantGroups &g, int64_t nrhs) {
+  if (g.uniform()) return true;
+  KernelParams<T> kp = kernel_params<T>(a, g);
+  return mfma_supported<T>(true, a->dim, a->ip.P1, a->ip.q, (int)nrhs, kp.kt, kp.vmask) &&
+         mfma_supported<T>(false, a->dim, a->ip.P1, a->ip.q, (int)nrhs, kp.kt, kp.vmask);
 }
 
 fmca_status host_action(fmca_h2_actor *a, const double *rhs, int64_t ld_rhs, int64_t nrhs,
@@ -579,7 +611,52 @@ fmca_status fmca_h2_action_variant_device(fmca_h2_actor *actor, int variant, int
   fmca_status st = ensure_device(actor);
   if (st != FMCA_OK) return st;
   return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, true, transpose != 0,
-                     (hipStream_t)stream, variant);
+                     (hipStream_t)stream, VariantGroups::single(variant));
+}
+
+fmca_status fmca_h2_action_variants_device(fmca_h2_actor *actor, const int *variants,
+                                           int ngroups, int transpose, const double *d_rhs,
+                                           int64_t ld_rhs, int64_t nrhs, double *d_out,
+                                           int64_t ld_out, int precision, void *stream) {
+  if (!actor || !variants || nrhs < 0 || (nrhs && (!d_rhs || !d_out)))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or negative nrhs");
+  if (ngroups < 1 || ngroups > kMaxVariantGroups)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "ngroups must be in [1, 8]");
+  if (nrhs > 16 * (int64_t)ngroups || nrhs <= 16 * (int64_t)(ngroups - 1))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "nrhs must fill the last 16-column group");
+  VariantGroups g;
+  g.n = ngroups;
+  for (int i = 0; i < ngroups; ++i) {
+    if (variants[i] < KV_VALUE || variants[i] > KV_PHI)
+      return fail(FMCA_ERR_INVALID_ARGUMENT, "variant");
+    g.v[i] = variants[i];
+  }
+  const int64_t rows_in = transpose ? actor->m : actor->n;
+  const int64_t rows_out = transpose ? actor->n : actor->m;
+  if (ld_rhs < rows_in || ld_out < rows_out)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  fmca_status st = ensure_device(actor);
+  if (st != FMCA_OK) return st;
+  const bool one = precision == FMCA_PRECISION_FP64 ? groups_in_one_action<double>(actor, g, nrhs)
+                                                    : groups_in_one_action<float>(actor, g, nrhs);
+  if (one)
+    return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, true,
+                       transpose != 0, (hipStream_t)stream, g);
+  // one action per run of equal variants
+  for (int g0 = 0; g0 < ngroups;) {
+    int g1 = g0 + 1;
+    while (g1 < ngroups && g.v[g1] == g.v[g0]) ++g1;
+    const int64_t c0 = 16 * (int64_t)g0, c1 = std::min<int64_t>(16 * (int64_t)g1, nrhs);
+    st = enqueue_any(actor, d_rhs + c0 * ld_rhs, ld_rhs, c1 - c0, d_out + c0 * ld_out, ld_out,
+                     precision, true, transpose != 0, (hipStream_t)stream,
+                     VariantGroups::single(g.v[g0]));
+    if (st != FMCA_OK) return st;
+    g0 = g1;
+  }
+  return FMCA_OK;
 }
 
 fmca_status fmca_h2_indices(const fmca_h2_actor *actor, int which, uint32_t *out) {
@@ -770,7 +847,24 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
     if ((st = matvec(Pv, Q, prec)) != FMCA_OK) return st;
     CG_TRY(blas1_add_ridge(n, nr, ridge, Pv, n, Q, n, sm));
     if ((st = dot(Pv, Q, h_dot)) != FMCA_OK) return st;
-    for (int j = 0; j < nr; ++j) h_alpha[j] = active[j] && h_dot[j] != 0 ? rho[j] / h_dot[j] : 0.0;
+    for (int j = 0; j < nr; ++j) {
+      if (active[j] && !(h_dot[j] > 0)) {
+        // negative or zero curvature: CG is undefined on an indefinite system
+        res->indefinite = 1;
+        res->converged = 0;
+        res->seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (cl)
+          CG_TRY(blas1_permute(n, nr, perm, false, X, n, d_x, ld_x, sm));
+        else
+          CG_TRY(hipMemcpy2DAsync(d_x, ld_x * sizeof(double), X, n * sizeof(double),
+                                  n * sizeof(double), nr, hipMemcpyDeviceToDevice, sm));
+        CG_TRY(hipStreamSynchronize(sm));
+        return fail(FMCA_ERR_NOT_CONVERGED,
+                    "CG met p.(K + ridge I)p <= 0: the H2 matrix plus ridge is indefinite");
+      }
+      h_alpha[j] = active[j] ? rho[j] / h_dot[j] : 0.0;
+    }
     CG_TRY(upload(d_alpha, h_alpha));
     CG_TRY(blas1_axpy(n, nr, 1.0, d_alpha, Pv, n, X, n, sm));
     CG_TRY(blas1_axpy(n, nr, -1.0, d_alpha, Q, n, R, n, sm));

@@ -58,6 +58,12 @@ struct InterpParams {
 // kt >= KT_COUNT selects a derivative of kernel kt % KT_COUNT (variant
 // kt / KT_COUNT): 1 = dk/dl, 2 = phi(r) = k'(r) / r (the radial factor of the
 // gradient with respect to a point, phi(0) = 0 where k' / r is singular).
+//
+// Several variants in one action (MFMA forms): tv[g] is the variant of rhs
+// columns [16 g, 16 g + 16) and vmask the OR of 1 << tv[g]; the kernel entry
+// of every (target, source) pair is then evaluated once for all variants.
+constexpr int kMaxVariantGroups = 8;
+
 template <typename T>
 struct KernelParams {
   int kt = 0;
@@ -65,6 +71,8 @@ struct KernelParams {
   T s = 1;      // radial scale (see above)
   T inv_l = 1;  // 1 / l
   T c2 = 1;     // c * c
+  int vmask = 1;
+  int tv[kMaxVariantGroups] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 enum KernelVariant : int { KV_VALUE = 0, KV_DL = 1, KV_PHI = 2 };
@@ -115,8 +123,10 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
 // output (output = true), used from kMfmaMinRhs right-hand sides on.
 constexpr int kMfmaMinRhs = 8;
 
+// kt, vmask: the kernel parameters of the launch (several variants take
+// 64-column passes, also in FP64).
 template <typename T>
-bool mfma_supported(bool output, int D, int P1, int q, int nrhs);
+bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask);
 
 template <typename T>
 hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,

@@ -202,6 +202,58 @@ __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
   return var == KV_DL ? -r2 * il * phi : phi;
 }
 
+// Value, dk/dl and phi of one entry (ev[KV_VALUE], ev[KV_DL], ev[KV_PHI]) for
+// the base kernel KT (kp.kt % KT_COUNT at run time when KT < 0).  Evaluated
+// together, the three share the distance, square root and exponential.
+template <int KT, typename T>
+__device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T ev[3]) {
+  using M = Math<T>;
+  if constexpr (KT == KT_EXPONENTIAL) {
+    // s = 1 / l: value e, dk/dl = r e / l^2, phi = -e / (l r)
+    const T r = M::sqrt_(r2), e = M::exp_(-r * kp.s);
+    ev[KV_VALUE] = e;
+    ev[KV_DL] = r * (kp.s * kp.s) * e;
+    ev[KV_PHI] = r > T(0) ? -(kp.s / r) * e : T(0);
+  } else if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN) {
+    // the shared exponential; dk/dl = -(r^2 / l) phi for these f(r / l)
+    T phi;
+    if constexpr (KT == KT_GAUSSIAN) {
+      const T e = M::exp_(r2 * kp.s);
+      ev[KV_VALUE] = e;
+      phi = -(kp.inv_l * kp.inv_l) * e;
+    } else {
+      const T x = M::sqrt_(r2) * kp.s, e = M::exp_(-x);
+      if constexpr (KT == KT_MATERN32) {
+        ev[KV_VALUE] = (T(1) + x) * e;
+        phi = -kp.s * kp.s * e;
+      } else {
+        ev[KV_VALUE] = (T(1) + x + x * x * T(1. / 3.)) * e;
+        phi = -kp.s * kp.s * T(1. / 3.) * (T(1) + x) * e;
+      }
+    }
+    ev[KV_PHI] = phi;
+    ev[KV_DL] = -r2 * kp.inv_l * phi;
+  } else if constexpr (KT >= 0) {
+    ev[KV_VALUE] = kernel_r2<KT, T>(r2, kp);
+    ev[KV_DL] = kernel_r2<KT + KT_COUNT * KV_DL, T>(r2, kp);
+    ev[KV_PHI] = kernel_r2<KT + KT_COUNT * KV_PHI, T>(r2, kp);
+  } else {
+    KernelParams<T> k = kp;
+    const int base = kp.kt % KT_COUNT;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      k.kt = base + KT_COUNT * v;
+      ev[v] = kernel_r2<-1, T>(r2, k);
+    }
+  }
+}
+
+// ev[v] for a uniform variant index v
+template <typename T>
+__device__ __forceinline__ T pick_variant(const T ev[3], int v) {
+  return v == KV_VALUE ? ev[KV_VALUE] : (v == KV_DL ? ev[KV_DL] : ev[KV_PHI]);
+}
+
 // Barycentric Lagrange factors l_a(u), a < P1, written with a stride
 // (TensorProductInterpolator.h:64-81; a factor within eps of its node is 1).
 template <typename T>

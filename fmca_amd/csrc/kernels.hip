@@ -521,9 +521,11 @@ hipError_t dispatch_nr(bool output, const DevTreeView<T> &rows, const DevTreeVie
                        const DevPatternView &pat, const InterpParams<T> &ip,
                        const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
                        int64_t ldo, bool natural, hipStream_t stream) {
-  if (mfma_supported<T>(output, D, ip.P1, ip.q, nrhs))
+  if (mfma_supported<T>(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask))
     return launch_mfma<T>(output, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.mom, buf.tlhs, out,
                           ldo, natural, stream);
+  // the VALU forms evaluate one variant per launch (kp.kt)
+  if ((kp.vmask & (kp.vmask - 1)) != 0) return hipErrorInvalidValue;
   const int nr = pick_nr<T>(output, D, ip.P1, ip.q, nrhs);
   if (output ? output_lds<T, 1>(D, ip.P1) > kLdsLimit : coupling_lds<T, 1>(D, ip.q) > kLdsLimit)
     return hipErrorInvalidConfiguration;

@@ -53,13 +53,17 @@ constexpr int kRows = kWaves * kRT * 16;
 // read rows k..k+3 of a staged operand, and a stride = 16 (mod 32) puts them
 // on disjoint banks (for 32 and for 64 banks)
 constexpr int kRowsP = kRows + 16;
-constexpr int kChunk = 64;        // source points / nodes staged per LDS chunk
+// source points / nodes staged per LDS chunk (32 for FP64 with 64 rhs columns,
+// to keep the staging within 64 KB of LDS)
+template <typename T, int NT>
+constexpr int kChunkOf = (sizeof(T) == 8 && NT >= 4) ? 32 : 64;
 
 // Occupancy target (waves per SIMD) of the MFMA kernels: FP32 with up to 32
 // rhs columns fits 4 waves without spilling; FP64 (twice the registers per
-// value) and 64-column tiles keep the default.
-template <typename T, int NT>
-constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2) ? 4 : 1;
+// value), 64-column tiles and the three-variant run-time switch keep the
+// default.
+template <typename T, int NT, int KT, bool MULTI>
+constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2 && !(MULTI && KT < 0)) ? 4 : 1;
 
 // Packed tensor index of every moment: a0 | a1 << 8 | a2 << 16.
 __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0xff; }
@@ -68,8 +72,10 @@ __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0x
 // Leaf output on MFMA: one workgroup per target leaf, 4 waves, each wave owns
 // row tiles {wave, wave+4, wave+8, wave+12} of the current 256-row pass and
 // 16*NT rhs columns.
-template <typename T, int D, int KT, int NT>
-__global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
+// MULTI: the rhs columns carry different kernel variants (kp.tv per 16-column
+// tile); every entry is evaluated once for all three variants.
+template <typename T, int D, int KT, int NT, bool MULTI>
+__global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_output_mfma(
     DevTreeView<T> rows, DevTreeView<T> cols, DevPatternView pat, InterpParams<T> ip,
     KernelParams<T> kp, int nrhs, const T *__restrict__ xs, const T *__restrict__ tlhs,
     double *__restrict__ out, int64_t ldo, int natural) {
@@ -77,6 +83,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
   using acc_t = typename M::acc_t;
   constexpr int NC = 16 * NT;
   constexpr int NCP = NC + 16;  // padded LDS row stride
+  constexpr int kChunk = kChunkOf<T, NT>;
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
@@ -137,6 +144,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
       for (int rt = 0; rt < kRT; ++rt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = acc_t{0, 0, 0, 0};
+      int tv[NT];  // kernel variant of each column tile of this pass (MULTI)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) tv[nt] = kp.tv[min((rc >> 4) + nt, kMaxVariantGroups - 1)];
+      // column tiles of this pass that hold rhs columns (the rest are skipped)
+      const int ntl = min(NT, (nrhs - rc + 15) >> 4);
       // far field: A = V^T (rows x q), B = l_t (q x NC), K-steps of 4 moments
       for (int a0 = 0; a0 < qpad; a0 += kChunk) {
         const int na = min(kChunk, qpad - a0);
@@ -163,7 +175,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
 #pragma unroll
             for (int d = 1; d < D; ++d) v *= F[(d * P1 + unpack(pk, d)) * kRowsP + row];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(v, b[nt], acc[rt][nt]);
+            for (int nt = 0; nt < NT; ++nt)
+              if (nt < ntl) acc[rt][nt] = M::mma(v, b[nt], acc[rt][nt]);
           }
         }
       }
@@ -204,9 +217,18 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
                 const T dd = x[rt][d] - y[d];
                 r2 = fmat(dd, dd, r2);
               }
-              const T e = kernel_r2<KT, T>(r2, kp);
+              if constexpr (MULTI) {
+                T ev[3];
+                kernel_r2_all<KT, T>(r2, kp, ev);
 #pragma unroll
-              for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+                for (int nt = 0; nt < NT; ++nt)
+                  if (nt < ntl) acc[rt][nt] = M::mma(pick_variant(ev, tv[nt]), b[nt], acc[rt][nt]);
+              } else {
+                const T e = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                  if (nt < ntl) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+              }
             }
           }
         }
@@ -235,9 +257,12 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_leaf_output_mfma(
 // ---------------------------------------------------------------------------
 // Coupling on MFMA: one workgroup per target cluster; rows = the q target
 // interpolation nodes (tiles of 16, kRT per wave, so q <= 256), K = the q
-// source nodes of each far cluster, N = 16*NT rhs columns.
-template <typename T, int D, int KT, int NT>
-__global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
+// source nodes of each far cluster, N = 16*NT rhs columns.  For q <= 64 (at
+// most kRT row tiles) every wave takes all row tiles and a quarter of the
+// K-steps, and the four partial sums are added in LDS (in wave order, so the
+// result does not depend on scheduling): all waves stay busy.
+template <typename T, int D, int KT, int NT, bool MULTI>
+__global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
                                                        DevPatternView pat, InterpParams<T> ip,
                                                        KernelParams<T> kp, int nrhs,
                                                        const T *__restrict__ mom,
@@ -246,12 +271,14 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
   using acc_t = typename M::acc_t;
   constexpr int NC = 16 * NT;
   constexpr int NCP = NC + 16;  // padded LDS row stride
+  constexpr int kChunk = kChunkOf<T, NT>;
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
   int *bidx = reinterpret_cast<int *>(smem_raw);                     // [qpad]
   T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][kMaxP1]
   T *Ms = Y + D * kMaxP1;                                            // [kChunk][NCP]
+  T *Red = Ms + kChunk * NCP;  // [kRT * 16][NC] partial sums of the K-split (q <= 64)
   const int lane = threadIdx.x & 63;
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -259,6 +286,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
   const int t = pat.far_order[blockIdx.x];
   if (t < 0) return;
   const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
+  const bool ksplit = q <= kRT * 16;
+  const int ks0 = ksplit ? 4 * wave : 0, kstep = ksplit ? 4 * kWaves : 4;
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
     int rem = e < q ? e : 0, pk = 0;
 #pragma unroll
@@ -273,7 +302,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
   bool tile_on[kRT];
 #pragma unroll
   for (int rt = 0; rt < kRT; ++rt) {
-    const int tile_row0 = (wave + kWaves * rt) * 16;
+    const int tile_row0 = (ksplit ? rt : wave + kWaves * rt) * 16;
     tile_on[rt] = tile_row0 < q;
     int rem = tile_row0 + lr < q ? tile_row0 + lr : 0;
 #pragma unroll
@@ -289,6 +318,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
     for (int rt = 0; rt < kRT; ++rt)
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = acc_t{0, 0, 0, 0};
+    int tv[NT];  // kernel variant of each column tile of this pass (MULTI)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) tv[nt] = kp.tv[min((rc >> 4) + nt, kMaxVariantGroups - 1)];
+    // column tiles of this pass that hold rhs columns (the rest are skipped)
+    const int ntl = min(NT, (nrhs - rc + 15) >> 4);
     for (int f = f0; f < f1; ++f) {
       const int s = pat.far_src[f];
       for (int be = 0; be < qpad; be += kChunk) {
@@ -308,7 +342,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
         }
         __syncthreads();
 #pragma unroll 2
-        for (int ks = 0; ks < nbeta; ks += 4) {
+        for (int ks = ks0; ks < nbeta; ks += kstep) {
           const int bl = ks + lk;
           const int pk = bidx[be + bl];
           T y[D];
@@ -326,24 +360,62 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
               const T dd = x[rt][d] - y[d];
               r2 = fmat(dd, dd, r2);
             }
-            const T e = kernel_r2<KT, T>(r2, kp);
+            if constexpr (MULTI) {
+              T ev[3];
+              kernel_r2_all<KT, T>(r2, kp, ev);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+              for (int nt = 0; nt < NT; ++nt)
+                if (nt < ntl) acc[rt][nt] = M::mma(pick_variant(ev, tv[nt]), b[nt], acc[rt][nt]);
+            } else {
+              const T e = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+              for (int nt = 0; nt < NT; ++nt)
+                if (nt < ntl) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+            }
           }
         }
       }
     }
+    if (ksplit) {
+      // sum the waves' partials in LDS in wave order, then store row-contiguous
+      for (int w = 0; w < kWaves; ++w) {
+        if (wave == w) {
 #pragma unroll
-    for (int rt = 0; rt < kRT; ++rt) {
-      if (!tile_on[rt]) continue;
+          for (int rt = 0; rt < kRT; ++rt) {
+            if (!tile_on[rt]) continue;
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const int col = rc + nt * 16 + lr;
+            for (int nt = 0; nt < NT; ++nt) {
+              if (nt >= ntl) continue;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int alpha = (wave + kWaves * rt) * 16 + M::crow(lk, v);
-          if (alpha < q && col < nrhs)
-            tlhs[((int64_t)t * q + alpha) * nrhs + col] = acc[rt][nt][v];
+              for (int v = 0; v < 4; ++v) {
+                const int alpha = rt * 16 + M::crow(lk, v);
+                T *p = Red + alpha * NC + nt * 16 + lr;
+                *p = (w == 0 ? T(0) : *p) + acc[rt][nt][v];
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      for (int e = threadIdx.x; e < q * NC; e += blockDim.x) {
+        const int alpha = e / NC, c = e - alpha * NC;
+        if (c < 16 * ntl && rc + c < nrhs)
+          tlhs[((int64_t)t * q + alpha) * nrhs + rc + c] = Red[e];
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int rt = 0; rt < kRT; ++rt) {
+        if (!tile_on[rt]) continue;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int col = rc + nt * 16 + lr;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int alpha = (wave + kWaves * rt) * 16 + M::crow(lk, v);
+            if (alpha < q && col < nrhs)
+              tlhs[((int64_t)t * q + alpha) * nrhs + col] = acc[rt][nt][v];
+          }
         }
       }
     }
@@ -352,6 +424,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT>)) void k_coupling_mfma(DevTr
 
 template <typename T, int NT>
 size_t leaf_output_mfma_lds(int D, int P1, int q) {
+  constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
   const size_t u1 = (size_t)kChunk * (16 * NT + 16);
   const size_t u2 = (size_t)D * kChunk + (size_t)kChunk * (16 * NT + 16);
@@ -361,53 +434,93 @@ size_t leaf_output_mfma_lds(int D, int P1, int q) {
 
 template <typename T, int NT>
 size_t coupling_mfma_lds(int D, int q) {
+  constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
+  const size_t red = q <= kRT * 16 ? (size_t)kRT * 16 * 16 * NT : 0;  // K-split partials
   return (size_t)((qpad + 3) & ~3) * sizeof(int) +
-         ((size_t)D * kMaxP1 + (size_t)kChunk * (16 * NT + 16)) * sizeof(T);
+         ((size_t)D * kMaxP1 + (size_t)kChunk * (16 * NT + 16) + red) * sizeof(T);
 }
 
-template <typename T, int D, int NT>
+// Specialised entry code for the common kernels, a run-time switch otherwise.
+inline bool kt_specialised(int base) {
+  return base == KT_EXPONENTIAL || base == KT_MATERN32 || base == KT_MATERN52 ||
+         base == KT_GAUSSIAN;
+}
+
+// True when the launch needs the MULTI kernels: several variants, or one
+// derivative variant of a specialised kernel (evaluated with the shared code
+// instead of the generic switch).
+inline bool mfma_multi(int kt, int vmask) {
+  const int base = kt % KT_COUNT;
+  return (vmask & (vmask - 1)) != 0 || (vmask != (1 << KV_VALUE) && kt_specialised(base));
+}
+
+template <typename T, int D, int NT, bool MULTI>
 hipError_t launch_mfma_nt(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                           const DevPatternView &pat, const InterpParams<T> &ip,
                           const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom,
                           T *tlhs, double *out, int64_t ldo, bool natural, hipStream_t stream) {
   const size_t lds_out = leaf_output_mfma_lds<T, NT>(D, ip.P1, ip.q);
   const size_t lds_cpl = coupling_mfma_lds<T, NT>(D, ip.q);
-#define FMCA_MFMA_CASE(K)                                                                   \
-  case K:                                                                                   \
-    if (output)                                                                             \
-      hipLaunchKernelGGL((k_leaf_output_mfma<T, D, K, NT>), dim3(pat.nnear_order), dim3(256),  \
-                         lds_out, stream, rows, cols,                                       \
-                         pat, ip, kp, nrhs, xs, tlhs, out, ldo, natural ? 1 : 0);           \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_coupling_mfma<T, D, K, NT>), dim3(pat.nfar_order), dim3(256),         \
-                         lds_cpl, stream, rows, cols, pat, ip, kp,                          \
-                         nrhs, mom, tlhs);                                                  \
+  const int base = kp.kt % KT_COUNT;
+  // MULTI kernels take the base kernel; single-variant kernels take kp.kt
+  // (a derivative variant there goes to the run-time switch, K = -1)
+  const int key = MULTI ? base : kp.kt;
+#define FMCA_MFMA_CASE(K)                                                                  \
+  case K:                                                                                  \
+    if (output)                                                                            \
+      hipLaunchKernelGGL((k_leaf_output_mfma<T, D, K, NT, MULTI>), dim3(pat.nnear_order),  \
+                         dim3(256), lds_out, stream, rows, cols, pat, ip, kp, nrhs, xs,    \
+                         tlhs, out, ldo, natural ? 1 : 0);                                 \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_coupling_mfma<T, D, K, NT, MULTI>), dim3(pat.nfar_order),      \
+                         dim3(256), lds_cpl, stream, rows, cols, pat, ip, kp, nrhs, mom,   \
+                         tlhs);                                                            \
     break;
-  switch (kp.kt) {
+  switch (key) {
     FMCA_MFMA_CASE(KT_EXPONENTIAL)
     FMCA_MFMA_CASE(KT_MATERN32)
     FMCA_MFMA_CASE(KT_MATERN52)
     FMCA_MFMA_CASE(KT_GAUSSIAN)
     default:
       if (output)
-        hipLaunchKernelGGL((k_leaf_output_mfma<T, D, -1, NT>), dim3(pat.nnear_order), dim3(256),
-                           lds_out, stream, rows, cols, pat, ip, kp, nrhs, xs, tlhs, out, ldo,
-                           natural ? 1 : 0);
+        hipLaunchKernelGGL((k_leaf_output_mfma<T, D, -1, NT, MULTI>), dim3(pat.nnear_order),
+                           dim3(256), lds_out, stream, rows, cols, pat, ip, kp, nrhs, xs, tlhs,
+                           out, ldo, natural ? 1 : 0);
       else
-        hipLaunchKernelGGL((k_coupling_mfma<T, D, -1, NT>), dim3(pat.nfar_order), dim3(256),
-                           lds_cpl, stream, rows, cols, pat, ip, kp, nrhs, mom, tlhs);
+        hipLaunchKernelGGL((k_coupling_mfma<T, D, -1, NT, MULTI>), dim3(pat.nfar_order),
+                           dim3(256), lds_cpl, stream, rows, cols, pat, ip, kp, nrhs, mom, tlhs);
   }
 #undef FMCA_MFMA_CASE
   return hipGetLastError();
 }
 
-// rhs columns per pass: 16, 32 or (FP32 only) 64
+// rhs columns per pass: 16, 32 or (FP32) 64.  Several variants take 32:
+// the staging latency per source cluster, hidden by occupancy, costs more
+// than evaluating the entries again in a further pass.
 template <typename T>
-int mfma_nt(int nrhs) {
+int mfma_nt(int nrhs, bool multi) {
   if (nrhs <= 16) return 1;
-  if (nrhs <= 32 || sizeof(T) == 8) return 2;
+  if (nrhs <= 32 || sizeof(T) == 8 || multi) return 2;
   return 4;
+}
+
+template <typename T, int D, bool MULTI>
+hipError_t launch_mfma_dm(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                          const DevPatternView &pat, const InterpParams<T> &ip,
+                          const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom,
+                          T *tlhs, double *out, int64_t ldo, bool natural, hipStream_t stream) {
+  switch (mfma_nt<T>(nrhs, MULTI)) {
+    case 1:
+      return launch_mfma_nt<T, D, 1, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom,
+                                            tlhs, out, ldo, natural, stream);
+    case 2:
+      return launch_mfma_nt<T, D, 2, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom,
+                                            tlhs, out, ldo, natural, stream);
+    default:
+      return launch_mfma_nt<T, D, 4, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom,
+                                            tlhs, out, ldo, natural, stream);
+  }
 }
 
 template <typename T, int D>
@@ -415,26 +528,20 @@ hipError_t launch_mfma_d(bool output, const DevTreeView<T> &rows, const DevTreeV
                          const DevPatternView &pat, const InterpParams<T> &ip,
                          const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom, T *tlhs,
                          double *out, int64_t ldo, bool natural, hipStream_t stream) {
-  switch (mfma_nt<T>(nrhs)) {
-    case 1:
-      return launch_mfma_nt<T, D, 1>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
+  if (mfma_multi(kp.kt, kp.vmask))
+    return launch_mfma_dm<T, D, true>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
+                                      ldo, natural, stream);
+  return launch_mfma_dm<T, D, false>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
                                      ldo, natural, stream);
-    case 2:
-      return launch_mfma_nt<T, D, 2>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
-                                     ldo, natural, stream);
-    default:
-      return launch_mfma_nt<T, D, 4>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out,
-                                     ldo, natural, stream);
-  }
 }
 
 }  // namespace
 
 template <typename T>
-bool mfma_supported(bool output, int D, int P1, int q, int nrhs) {
+bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask) {
   if (nrhs < kMfmaMinRhs) return false;
   const size_t limit = 64 * 1024;
-  const int nt = mfma_nt<T>(nrhs);
+  const int nt = mfma_nt<T>(nrhs, mfma_multi(kt, vmask));
   if (output) {
     const size_t b = nt == 1   ? leaf_output_mfma_lds<T, 1>(D, P1, q)
                      : nt == 2 ? leaf_output_mfma_lds<T, 2>(D, P1, q)
@@ -467,8 +574,8 @@ hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeVie
   return hipErrorInvalidValue;
 }
 
-template bool mfma_supported<float>(bool, int, int, int, int);
-template bool mfma_supported<double>(bool, int, int, int, int);
+template bool mfma_supported<float>(bool, int, int, int, int, int, int);
+template bool mfma_supported<double>(bool, int, int, int, int, int, int);
 template hipError_t launch_mfma<float>(bool, const DevTreeView<float> &,
                                        const DevTreeView<float> &, const DevPatternView &,
                                        const InterpParams<float> &, const KernelParams<float> &,

@@ -195,6 +195,7 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
             d["matvecs_fp64"] = r.matvecs_fp64;
             d["matvecs_fp32"] = r.matvecs_fp32;
             d["rel_residual"] = r.rel_residual;
+            d["indefinite"] = r.indefinite != 0;
             d["seconds"] = r.seconds;
             return d;
           },
@@ -206,6 +207,20 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
              fmca_kernel_desc k{ktype.c_str(), l, c, nu};
              check(fmca_h2_set_kernel(a.h, &k));
            })
+      .def(
+          "action_variants_device",
+          [](Actor &a, std::vector<int> variants, bool transpose, uintptr_t rhs, int64_t ld_rhs,
+             int64_t nrhs, uintptr_t out, int64_t ld_out, int precision, uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            check(fmca_h2_action_variants_device(
+                a.h, variants.data(), (int)variants.size(), transpose ? 1 : 0,
+                reinterpret_cast<const double *>(rhs), ld_rhs, nrhs,
+                reinterpret_cast<double *>(out), ld_out, precision,
+                reinterpret_cast<void *>(stream)));
+          },
+          py::arg("variants"), py::arg("transpose"), py::arg("rhs"), py::arg("ld_rhs"),
+          py::arg("nrhs"), py::arg("out"), py::arg("ld_out"), py::arg("precision"),
+          py::arg("stream") = 0)
       .def(
           "action_variant_device",
           [](Actor &a, int variant, bool transpose, uintptr_t rhs, int64_t ld_rhs, int64_t nrhs,

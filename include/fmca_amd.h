@@ -127,6 +127,8 @@ typedef struct {
   double rel_residual;  /* max over columns of ||b - (K + ridge I) x|| / ||b||, FP64 action,
                            at the last replacement */
   double seconds;       /* wall time of the solve */
+  int indefinite;       /* 1 if a column met p.(K + ridge I)p <= 0 (the H2 matrix plus ridge
+                           is not positive definite); the solve then stops */
 } fmca_cg_result;
 
 /* CG solve of (K(P, P) + ridge I) x = b for nrhs independent columns
@@ -161,6 +163,17 @@ fmca_status fmca_h2_action_variant_device(fmca_h2_actor *actor, int variant, int
                                           const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
                                           double *d_out, int64_t ld_out, int precision,
                                           void *stream);
+
+/* Several kernel variants in one action: rhs columns [16 g, 16 g + 16) are
+ * multiplied by the H2 matrix of variant variants[g] (FMCA_KERNEL_VALUE /
+ * _DL / _PHI), g < ngroups <= 8, and 16 (ngroups - 1) < nrhs <= 16 ngroups.
+ * On the matrix cores every kernel entry is evaluated once for all groups
+ * (the fused backward of the differentiable matvec); where a phase runs on
+ * VALU the groups are applied as separate actions.  No FMCA counterpart. */
+fmca_status fmca_h2_action_variants_device(fmca_h2_actor *actor, const int *variants,
+                                           int ngroups, int transpose, const double *d_rhs,
+                                           int64_t ld_rhs, int64_t nrhs, double *d_out,
+                                           int64_t ld_out, int precision, void *stream);
 
 /* Permutation of a tree (cluster order position -> point index), as
  * pyH2Matrix::rindices / cindices (py/FMCA.cpp:224-229) and

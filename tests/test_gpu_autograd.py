@@ -195,3 +195,39 @@ def test_gradients_rectangular_targets_vs_dense():
     assert rel_err(Sg.grad.cpu().numpy(), Sd.grad.cpu().numpy()) < 1e-5
     assert rel_err(Tg.grad.cpu().numpy(), Td.grad.cpu().numpy()) < 1e-5
     assert abs(lt.grad.item() - ld.grad.item()) / abs(ld.grad.item()) < 1e-5
+
+
+@pytest.mark.parametrize("kernel,dim,deg,transpose,groups", [
+    ("MATERN32", 2, 5, False, [2, 0, 1, 2, 2]),   # fused backward layout, one MFMA pass of 4 + 1
+    ("MATERN32", 3, 4, True, [0, 2]),
+    ("GAUSSIAN", 3, 3, False, [1, 0, 1]),
+    ("INVMULTIQUADRIC", 2, 4, False, [0, 1, 2]),  # run-time kernel switch
+    ("EXPONENTIAL", 3, 6, False, [0, 2, 2]),      # q = 343: coupling on VALU, split per group
+])
+def test_variant_groups_equal_separate_actions(kernel, dim, deg, transpose, groups):
+    """fmca_h2_action_variants_device: the 16-column groups of one action carry
+    different kernel variants; each group must equal the single-variant action
+    of its columns (FP64 within 1e-13, FP32 within 1e-6 relative)."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print(f"[case] variant groups {kernel} d={dim} deg={deg} T={transpose} {groups}", flush=True)
+    n, m = 6000, 5000
+    P, Pe = W.uniform(n, dim, 31), W.uniform(m, dim, 32)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel(kernel, 0.4), Pe, P, deg, 0.8)
+    rows_in, rows_out = (m, n) if transpose else (n, m)
+    ncols = 16 * (len(groups) - 1) + 11
+    X = torch.randn(ncols, rows_in, dtype=torch.float64, device="cuda").t()
+    stream = torch.cuda.current_stream().cuda_stream
+    for prec, tol in ((fmca_amd.FP64, 1e-13), (fmca_amd.FP32, 1e-6)):
+        Y = torch.empty(ncols, rows_out, dtype=torch.float64, device="cuda").t()
+        H._a.action_variants_device(groups, transpose, X.data_ptr(), rows_in, ncols,
+                                    Y.data_ptr(), rows_out, prec, stream)
+        for g, v in enumerate(groups):
+            c0, c1 = 16 * g, min(16 * g + 16, ncols)
+            Xg = X[:, c0:c1].t().contiguous().t()
+            Yg = torch.empty(c1 - c0, rows_out, dtype=torch.float64, device="cuda").t()
+            H._a.action_variant_device(v, transpose, Xg.data_ptr(), rows_in, c1 - c0,
+                                       Yg.data_ptr(), rows_out, prec, stream)
+            err = (torch.linalg.norm(Y[:, c0:c1] - Yg) / torch.linalg.norm(Yg)).item()
+            assert err <= tol, (prec, g, v, err)

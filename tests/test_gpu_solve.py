@@ -82,3 +82,22 @@ def test_cg_single_column_and_zero_rhs():
     x1, info1 = H.cg(b[:, 1].contiguous(), ridge=1.0, tol=1e-9, precision=fmca_amd.FP64)
     assert info1["converged"] and x1.dim() == 1
     assert _resid(H, x1.reshape(-1, 1), b[:, 1:].contiguous(), 1.0) <= 1e-9
+
+
+def test_cg_reports_indefinite_system():
+    """The multiquadric kernel sqrt(r^2 + c^2) gives a matrix with one positive
+    eigenvalue and the rest negative (conditionally negative definite): CG
+    must stop with the indefinite flag instead of iterating on."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print("[case] cg indefinite multiquadric", flush=True)
+    n = 3000
+    P = W.uniform(n, 2, 13)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MULTIQUADRIC", 1.0, 0.5), P, P, 4,
+                               0.8)
+    b = torch.from_numpy(W.normal_rhs(n, 2, 14)).cuda()
+    for prec in (fmca_amd.FP64, fmca_amd.FP32):
+        x, info = H.cg(b, ridge=0.0, tol=1e-8, precision=prec, max_iter=500)
+        assert info["indefinite"] and not info["converged"], info
+        assert info["iterations"] < 500

@@ -189,3 +189,34 @@ def test_launch_orders_cover_every_item_once_with_balanced_xcd_pieces(points):
             if len(ids) >= 256:
                 w = np.bincount(xcd[ids], weights=far[ids], minlength=8)
                 assert w.max() <= 1.35 * w.mean(), (L, w)
+
+
+def test_variant_groups_argument_checks_without_device():
+    """fmca_h2_action_variants_device validates its arguments before it needs
+    a device: group count in [1, 8], variants in {0, 1, 2}, and nrhs filling
+    the last 16-column group."""
+    lib = ctypes.CDLL(LIB)
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("ktype", ctypes.c_char_p), ("l", ctypes.c_double), ("c", ctypes.c_double),
+                    ("nu", ctypes.c_double)]
+    P = uniform_points(500, 2, seed=1)
+    h = ctypes.c_void_p()
+    desc = Desc(b"MATERN32", 0.5, 1.0, 1.0)
+    ptr = P.ctypes.data_as(ctypes.c_void_p)
+    assert lib.fmca_h2_create(ctypes.byref(desc), 2, ctypes.c_int64(500), ptr,
+                              ctypes.c_int64(500), ptr, 3, ctypes.c_double(0.8),
+                              ctypes.byref(h)) == 0
+    buf = (ctypes.c_double * 1)()
+
+    def call(variants, nrhs):
+        v = (ctypes.c_int * max(1, len(variants)))(*variants)
+        return lib.fmca_h2_action_variants_device(h, v, len(variants), 0, buf, ctypes.c_int64(500),
+                                                  ctypes.c_int64(nrhs), buf, ctypes.c_int64(500),
+                                                  0, None)
+    assert call([0] * 9, 130) == 1       # more than 8 groups
+    assert call([], 1) == 1              # no group
+    assert call([0, 3], 20) == 1         # unknown variant
+    assert call([0, 1], 33) == 1         # more columns than groups
+    assert call([0, 1], 16) == 1         # last group empty
+    assert lib.fmca_h2_destroy(h) == 0
