@@ -213,8 +213,9 @@ class H2kernelActor:
         return r, c, lr.astype(bool)
 
     def launch_order(self, which: str) -> np.ndarray:
-        """Workgroup -> work item of the coupling ("coupling": target cluster)
-        or of the leaf output ("leaf_output": target leaf index); -1 = idle."""
+        """(workgroups, 4): target (cluster id for "coupling", leaf index for
+        "leaf_output"), list begin, list end, flags (1 split, 2 far field) of
+        each workgroup; rows of -1 are idle workgroups."""
         return self._a.launch_order({"coupling": 0, "leaf_output": 1}[which])
 
 

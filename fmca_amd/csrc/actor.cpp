@@ -107,18 +107,25 @@ struct DevTreeReals {
 };
 
 struct DevPattern {
-  DevBuf far_ptr, far_src, far_order, near_ptr, near_src, near_order;
+  DevBuf far_ptr, far_src, far_order, far_task, far_split;
+  DevBuf near_ptr, near_src, near_order, near_task, near_split;
   DevPatternView view(const Pattern &p) const {
     DevPatternView v;
     v.far_ptr = far_ptr.as<int32_t>();
     v.far_src = far_src.as<int32_t>();
     v.nfar_order = (int)p.far_order.size();
     v.far_order = far_order.as<int32_t>();
+    v.far_task = far_task.as<int32_t>();
+    v.nfar_split = (int)p.far_split.size();
+    v.far_split = far_split.as<int32_t>();
     v.nnear_leaves = (int)p.near_ptr.size() - 1;
     v.near_ptr = near_ptr.as<int32_t>();
     v.near_src = near_src.as<int32_t>();
     v.nnear_order = (int)p.near_order.size();
     v.near_order = near_order.as<int32_t>();
+    v.near_task = near_task.as<int32_t>();
+    v.nnear_split = (int)p.near_split.size();
+    v.near_split = near_split.as<int32_t>();
     return v;
   }
   hipError_t upload(const Pattern &p) {
@@ -126,9 +133,13 @@ struct DevPattern {
     if ((e = far_ptr.upload(p.far_ptr)) != hipSuccess) return e;
     if ((e = far_src.upload(p.far_src)) != hipSuccess) return e;
     if ((e = far_order.upload(p.far_order)) != hipSuccess) return e;
+    if ((e = far_task.upload(p.far_task)) != hipSuccess) return e;
+    if ((e = far_split.upload(p.far_split)) != hipSuccess) return e;
     if ((e = near_ptr.upload(p.near_ptr)) != hipSuccess) return e;
     if ((e = near_src.upload(p.near_src)) != hipSuccess) return e;
-    return near_order.upload(p.near_order);
+    if ((e = near_order.upload(p.near_order)) != hipSuccess) return e;
+    if ((e = near_task.upload(p.near_task)) != hipSuccess) return e;
+    return near_split.upload(p.near_split);
   }
 };
 
@@ -724,8 +735,11 @@ fmca_status fmca_h2_launch_order(const fmca_h2_actor *actor, int which, int32_t 
   if (!actor || !len) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
   if (which != 0 && which != 1) return fail(FMCA_ERR_INVALID_ARGUMENT, "which must be 0 or 1");
   const std::vector<int32_t> &o = which == 0 ? actor->pat.far_order : actor->pat.near_order;
+  const std::vector<int32_t> &tk = which == 0 ? actor->pat.far_task : actor->pat.near_task;
   *len = (int64_t)o.size();
-  if (out) std::copy(o.begin(), o.end(), out);
+  if (out)
+    for (size_t b = 0; b < o.size(); ++b)
+      for (int k = 0; k < 4; ++k) out[4 * b + k] = o[b] < 0 ? -1 : tk[4 * (size_t)o[b] + k];
   return FMCA_OK;
 }
 

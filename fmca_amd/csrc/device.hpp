@@ -31,16 +31,24 @@ struct DevTreeView {
 };
 
 // Interaction lists of one (target tree, source tree) pair.
+// Tasks hold 4 ints (target, list begin, list end, flags kTaskSplit /
+// kTaskFar); a split target's tasks add atomically into a zeroed output.
 struct DevPatternView {
   const int32_t *far_ptr = nullptr;     // ncl_target + 1
   const int32_t *far_src = nullptr;
   int nfar_order = 0;                   // coupling grid size
-  const int32_t *far_order = nullptr;   // workgroup -> target cluster (-1: none)
+  const int32_t *far_order = nullptr;   // workgroup -> coupling task (-1: none)
+  const int32_t *far_task = nullptr;    // target cluster, far list range, flags
+  int nfar_split = 0;
+  const int32_t *far_split = nullptr;   // target clusters with several tasks
   int nnear_leaves = 0;
   const int32_t *near_ptr = nullptr;    // nleaves_target + 1
   const int32_t *near_src = nullptr;
   int nnear_order = 0;                  // leaf output grid size
-  const int32_t *near_order = nullptr;  // workgroup -> target leaf index (-1: none)
+  const int32_t *near_order = nullptr;  // workgroup -> leaf output task (-1: none)
+  const int32_t *near_task = nullptr;   // target leaf index, near list range, flags
+  int nnear_split = 0;
+  const int32_t *near_split = nullptr;  // target leaf indices with several tasks
 };
 
 template <typename T>

@@ -248,6 +248,13 @@ __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T
   }
 }
 
+// Adds v to *p with a relaxed device-scope hardware atomic: the tasks of a
+// split target accumulate into one (zeroed) output.
+template <typename T>
+__device__ __forceinline__ void atomic_accumulate(T *p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
 // ev[v] for a uniform variant index v
 template <typename T>
 __device__ __forceinline__ T pick_variant(const T ev[3], int v) {

@@ -44,6 +44,32 @@ __global__ __launch_bounds__(kBlock) void k_permute_in(const double *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Zero fill of the outputs that split targets accumulate into (a kernel, not
+// a memset, so that the action can be captured in a graph): the moments
+// tlhs[t] of the listed clusters, or the output rows of the listed leaves.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_clusters(const int32_t *__restrict__ list,
+                                                          int64_t per_cluster,
+                                                          T *__restrict__ tlhs) {
+  const int64_t base = (int64_t)list[blockIdx.x] * per_cluster;
+  for (int64_t e = threadIdx.x; e < per_cluster; e += blockDim.x) tlhs[base + e] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_leaf_rows(DevTreeView<T> rows,
+                                                           const int32_t *__restrict__ list,
+                                                           int nrhs, double *__restrict__ out,
+                                                           int64_t ldo, int natural) {
+  const int c = rows.leaves[list[blockIdx.x]];
+  const int b0 = rows.begin[c], sz = rows.size[c];
+  for (int e = threadIdx.x; e < sz * nrhs; e += blockDim.x) {
+    const int i = e % sz, r = e / sz;
+    const int64_t row = natural ? (int64_t)rows.perm[b0 + i] : (int64_t)(b0 + i);
+    out[row + (int64_t)r * ldo] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2. leaf moments: mom[t][alpha][r] = sum_p V_t[alpha][p] xs[p][r]
 // One workgroup per leaf; the Lagrange factors of 64 points at a time are
 // staged in LDS and the tensor-product basis value is formed on the fly.
@@ -189,9 +215,11 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
   const int P1 = ip.P1, q = ip.q;
   T *Y = reinterpret_cast<T *>(smem_raw);  // [D][P1] source nodes
   T *Mv = Y + D * kMaxP1;                  // [q][NR] source moments
-  const int t = pat.far_order[blockIdx.x];
-  if (t < 0) return;
-  const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
+  const int task = pat.far_order[blockIdx.x];
+  if (task < 0) return;
+  const int t = pat.far_task[4 * task], f0 = pat.far_task[4 * task + 1],
+            f1 = pat.far_task[4 * task + 2];
+  const bool split = (pat.far_task[4 * task + 3] & kTaskSplit) != 0;
   for (int rc = 0; rc < nrhs; rc += NR) {
     const int nr = min(NR, nrhs - rc);
     for (int abase = 0; abase < q; abase += blockDim.x) {
@@ -264,7 +292,13 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
       if (alpha < q) {
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-          if (r < nr) tlhs[((int64_t)t * q + alpha) * nrhs + rc + r] = acc[r];
+          if (r < nr) {
+            T *p = tlhs + ((int64_t)t * q + alpha) * nrhs + rc + r;
+            if (split)
+              atomic_accumulate(p, acc[r]);
+            else
+              *p = acc[r];
+          }
       }
     }
   }
@@ -344,11 +378,13 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
   T *Mv = U;                               // [kAlphaTile][NR] far-field coefficients
   T *Ys = U;                               // [D][kSrcTile] source points
   T *Xs = U + D * kSrcTile;                // [kSrcTile][NR] source rhs
-  const int li = pat.near_order[blockIdx.x];
-  if (li < 0) return;
+  const int task = pat.near_order[blockIdx.x];
+  if (task < 0) return;
+  const int li = pat.near_task[4 * task], n0 = pat.near_task[4 * task + 1],
+            n1 = pat.near_task[4 * task + 2], flags = pat.near_task[4 * task + 3];
+  const bool split = (flags & kTaskSplit) != 0, do_far = (flags & kTaskFar) != 0;
   const int c = rows.leaves[li];
   const int b0 = rows.begin[c], sz = rows.size[c];
-  const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
   for (int rc = 0; rc < nrhs; rc += NR) {
     const int nr = min(NR, nrhs - rc);
     for (int pbase = 0; pbase < sz; pbase += kBlock) {
@@ -361,9 +397,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
       T acc[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) acc[r] = 0;
-      // far field through the leaf basis (backward_transform_impl.h:30-31)
+      // far field through the leaf basis (backward_transform_impl.h:30-31),
+      // applied by the first task of the leaf
       __syncthreads();
-      if (act) {
+      if (act && do_far) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
           lagrange_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
@@ -372,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
       int a[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) a[d] = 0;
-      for (int at = 0; at < q; at += kAlphaTile) {
+      for (int at = 0; do_far && at < q; at += kAlphaTile) {
         const int na = min(kAlphaTile, q - at);
         __syncthreads();
         for (int e = threadIdx.x; e < kAlphaTile * NR; e += blockDim.x) {
@@ -430,7 +467,13 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
         const int64_t row = natural ? (int64_t)rows.perm[b0 + i] : (int64_t)(b0 + i);
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-          if (r < nr) out[row + (int64_t)(rc + r) * ldo] = (double)acc[r];
+          if (r < nr) {
+            double *p = out + row + (int64_t)(rc + r) * ldo;
+            if (split)
+              atomic_accumulate(p, (double)acc[r]);
+            else
+              *p = (double)acc[r];
+          }
       }
     }
   }
@@ -586,6 +629,9 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 4. coupling (every target cluster is written, empty far lists give zeros)
   mark(PH_COUPLING);
+  if (pat.nfar_split > 0)
+    hipLaunchKernelGGL((k_zero_clusters<T>), dim3(pat.nfar_split), dim3(kBlock), 0, stream,
+                       pat.far_split, (int64_t)q * nrhs, buf.tlhs);
   if ((err = dispatch_nr<T, D>(false, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
                                stream)) != hipSuccess)
     return err;
@@ -600,6 +646,9 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 6. leaf output: far field through V^T plus the near field
   mark(PH_LEAF_OUTPUT);
+  if (pat.nnear_split > 0)
+    hipLaunchKernelGGL((k_zero_leaf_rows<T>), dim3(pat.nnear_split), dim3(kBlock), 0, stream,
+                       rows, pat.near_split, nrhs, out, ld_out, natural ? 1 : 0);
   err = dispatch_nr<T, D>(true, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural, stream);
   mark(PH_COUNT);
   return err;

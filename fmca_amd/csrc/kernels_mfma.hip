@@ -97,11 +97,13 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int li = pat.near_order[blockIdx.x];
-  if (li < 0) return;
+  const int task = pat.near_order[blockIdx.x];
+  if (task < 0) return;
+  const int li = pat.near_task[4 * task], n0 = pat.near_task[4 * task + 1],
+            n1 = pat.near_task[4 * task + 2], flags = pat.near_task[4 * task + 3];
+  const bool split = (flags & kTaskSplit) != 0, do_far = (flags & kTaskFar) != 0;
   const int c = rows.leaves[li];
   const int b0 = rows.begin[c], sz = rows.size[c];
-  const int n0 = pat.near_ptr[li], n1 = pat.near_ptr[li + 1];
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
     int rem = e < q ? e : 0, pk = 0;
 #pragma unroll
@@ -150,7 +152,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
       // column tiles of this pass that hold rhs columns (the rest are skipped)
       const int ntl = min(NT, (nrhs - rc + 15) >> 4);
       // far field: A = V^T (rows x q), B = l_t (q x NC), K-steps of 4 moments
-      for (int a0 = 0; a0 < qpad; a0 += kChunk) {
+      // (applied by the first task of the leaf)
+      for (int a0 = 0; do_far && a0 < qpad; a0 += kChunk) {
         const int na = min(kChunk, qpad - a0);
         __syncthreads();
         for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
@@ -245,7 +248,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
             const int i = pbase + (wave + kWaves * rt) * 16 + M::crow(lk, v);
             if (i < sz && col < nrhs) {
               const int64_t row = natural ? (int64_t)rows.perm[b0 + i] : (int64_t)(b0 + i);
-              out[row + (int64_t)col * ldo] = (double)acc[rt][nt][v];
+              double *p = out + row + (int64_t)col * ldo;
+              if (split)
+                atomic_accumulate(p, (double)acc[rt][nt][v]);
+              else
+                *p = (double)acc[rt][nt][v];
             }
           }
         }
@@ -283,9 +290,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int t = pat.far_order[blockIdx.x];
-  if (t < 0) return;
-  const int f0 = pat.far_ptr[t], f1 = pat.far_ptr[t + 1];
+  const int task = pat.far_order[blockIdx.x];
+  if (task < 0) return;
+  const int t = pat.far_task[4 * task], f0 = pat.far_task[4 * task + 1],
+            f1 = pat.far_task[4 * task + 2];
+  const bool split = (pat.far_task[4 * task + 3] & kTaskSplit) != 0;
   const bool ksplit = q <= kRT * 16;
   const int ks0 = ksplit ? 4 * wave : 0, kstep = ksplit ? 4 * kWaves : 4;
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
@@ -399,8 +408,13 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
       }
       for (int e = threadIdx.x; e < q * NC; e += blockDim.x) {
         const int alpha = e / NC, c = e - alpha * NC;
-        if (c < 16 * ntl && rc + c < nrhs)
-          tlhs[((int64_t)t * q + alpha) * nrhs + rc + c] = Red[e];
+        if (c < 16 * ntl && rc + c < nrhs) {
+          T *p = tlhs + ((int64_t)t * q + alpha) * nrhs + rc + c;
+          if (split)
+            atomic_accumulate(p, Red[e]);
+          else
+            *p = Red[e];
+        }
       }
       __syncthreads();
     } else {
@@ -413,8 +427,13 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int alpha = (wave + kWaves * rt) * 16 + M::crow(lk, v);
-            if (alpha < q && col < nrhs)
-              tlhs[((int64_t)t * q + alpha) * nrhs + col] = acc[rt][nt][v];
+            if (alpha < q && col < nrhs) {
+              T *p = tlhs + ((int64_t)t * q + alpha) * nrhs + col;
+              if (split)
+                atomic_accumulate(p, acc[rt][nt][v]);
+              else
+                *p = acc[rt][nt][v];
+            }
           }
         }
       }

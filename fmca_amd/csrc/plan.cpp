@@ -428,17 +428,74 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
   for (int k = 0; k < nl; ++k)
     std::sort(pat.near_src.begin() + pat.near_ptr[k], pat.near_src.begin() + pat.near_ptr[k + 1],
               [&](int a, int b) { return C.begin[a] < C.begin[b]; });
-  // launch orders: coupling work = far blocks (plus the zero fill of an empty
-  // list), deepest level first; leaf output work = near-field entries
-  std::vector<double> wf(R.ncl), wn(nl, 0.);
-  for (int c = 0; c < R.ncl; ++c) wf[c] = 0.05 + (pat.far_ptr[c + 1] - pat.far_ptr[c]);
-  std::vector<std::pair<int, int>> levels;
-  for (int L = R.depth; L >= 0; --L) levels.push_back({R.level_ptr[L], R.level_ptr[L + 1]});
-  pat.far_order = xcd_launch_order(wf, levels, kXcds);
-  for (int k = 0; k < nl; ++k)
-    for (int e = pat.near_ptr[k]; e < pat.near_ptr[k + 1]; ++e)
-      wn[k] += double(R.size[R.leaves[k]]) * C.size[pat.near_src[e]];
-  pat.near_order = xcd_launch_order(wn, {{0, nl}}, kXcds);
+  // Tasks and launch orders.  Coupling work = far blocks (plus the zero fill
+  // of an empty list); leaf output work = near-field entries (plus the far
+  // field V^T l_t).  Targets above three times the mean work are cut into
+  // pieces of about that size.
+  {
+    double fsum = 0;
+    int fcnt = 0;
+    for (int c = 0; c < R.ncl; ++c)
+      if (pat.far_ptr[c + 1] > pat.far_ptr[c]) {
+        fsum += pat.far_ptr[c + 1] - pat.far_ptr[c];
+        ++fcnt;
+      }
+    const int cap = std::max(16, (int)std::ceil(3.0 * fsum / std::max(1, fcnt)));
+    std::vector<double> wt;
+    std::vector<std::pair<int, int>> levels;
+    for (int L = R.depth; L >= 0; --L) {
+      const int first = (int)(pat.far_task.size() / 4);
+      for (int c = R.level_ptr[L]; c < R.level_ptr[L + 1]; ++c) {
+        const int lo = pat.far_ptr[c], hi = pat.far_ptr[c + 1], f = hi - lo;
+        const int m = std::max(1, (f + cap - 1) / cap);
+        if (m > 1) pat.far_split.push_back(c);
+        for (int k = 0; k < m; ++k) {
+          const int a = lo + (int)((int64_t)f * k / m), b = lo + (int)((int64_t)f * (k + 1) / m);
+          pat.far_task.insert(pat.far_task.end(), {c, a, b, m > 1 ? kTaskSplit : 0});
+          wt.push_back(0.05 + (b - a));
+        }
+      }
+      levels.push_back({first, (int)(pat.far_task.size() / 4)});
+    }
+    pat.far_order = xcd_launch_order(wt, levels, kXcds);
+  }
+  {
+    std::vector<double> wl(nl, 0.);
+    for (int k = 0; k < nl; ++k)
+      for (int e = pat.near_ptr[k]; e < pat.near_ptr[k + 1]; ++e)
+        wl[k] += double(R.size[R.leaves[k]]) * C.size[pat.near_src[e]];
+    double mean = 0;
+    for (double w : wl) mean += w;
+    mean /= std::max(1, nl);
+    const double cap = std::max(3.0 * mean, 1.0);
+    std::vector<double> wt;
+    for (int k = 0; k < nl; ++k) {
+      const int rows = (int)R.size[R.leaves[k]];
+      const double wfar = double(rows) * q;  // V^T l_t
+      const size_t first = pat.near_task.size();
+      int a = pat.near_ptr[k];
+      double acc = wfar;
+      for (int e = pat.near_ptr[k]; e < pat.near_ptr[k + 1]; ++e) {
+        const double w = double(rows) * C.size[pat.near_src[e]];
+        if (acc + w > cap && e > a) {
+          pat.near_task.insert(pat.near_task.end(), {k, a, e, 0});
+          wt.push_back(acc);
+          a = e;
+          acc = 0;
+        }
+        acc += w;
+      }
+      pat.near_task.insert(pat.near_task.end(), {k, a, pat.near_ptr[k + 1], 0});
+      wt.push_back(acc);
+      const size_t ntasks = (pat.near_task.size() - first) / 4;
+      pat.near_task[first + 3] |= kTaskFar;
+      if (ntasks > 1) {
+        pat.near_split.push_back(k);
+        for (size_t i = first; i < pat.near_task.size(); i += 4) pat.near_task[i + 3] |= kTaskSplit;
+      }
+    }
+    pat.near_order = xcd_launch_order(wt, {{0, (int)wt.size()}}, kXcds);
+  }
 }
 
 std::vector<int32_t> xcd_launch_order(const std::vector<double> &work,

@@ -35,6 +35,7 @@ const char *kernel_type_name(int kt);
 
 constexpr int kMaxDim = 3;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
+constexpr int kTaskSplit = 1, kTaskFar = 2;  // task flags
 constexpr int kMaxP1 = 10;  // polynomial degree <= 9
 
 // Tensor-product Chebyshev interpolation on [0,1]^d
@@ -81,9 +82,16 @@ struct Pattern {
   std::vector<int32_t> far_ptr, far_src;
   // near field: CSR over target leaves (index into Tree::leaves) -> source clusters
   std::vector<int32_t> near_ptr, near_src;
-  // launch orders (one workgroup per entry, -1 = nothing to do, see
-  // xcd_launch_order): target clusters of the coupling, target leaves of the
-  // leaf output
+  // Work items of the two entry phases, 4 ints each: target (cluster id for
+  // the coupling, leaf index for the leaf output), begin and end in its far /
+  // near list, flags.  A target with more than three times the mean work is cut
+  // into contiguous pieces of its list (flag kTaskSplit: its pieces add into
+  // an output that is zeroed first); the first piece of a leaf also applies
+  // the far field (kTaskFar).
+  std::vector<int32_t> far_task, near_task;
+  std::vector<int32_t> far_split, near_split;  // targets cut into several tasks
+  // launch orders over the tasks (one workgroup per entry, -1 = nothing to
+  // do, see xcd_launch_order)
   std::vector<int32_t> far_order, near_order;
   double near_entries = 0, far_entries = 0;
   int64_t n_far = 0, n_near = 0;

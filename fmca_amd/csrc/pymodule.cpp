@@ -121,7 +121,7 @@ py::tuple blocks(const Actor &a) {
 py::array_t<int32_t> launch_order(const Actor &a, int which) {
   int64_t len = 0;
   check(fmca_h2_launch_order(a.h, which, nullptr, &len));
-  py::array_t<int32_t> out((py::ssize_t)len);
+  py::array_t<int32_t> out({(py::ssize_t)len, (py::ssize_t)4});
   check(fmca_h2_launch_order(a.h, which, out.mutable_data(), &len));
   return out;
 }

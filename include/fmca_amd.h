@@ -196,12 +196,16 @@ fmca_status fmca_h2_info(const fmca_h2_actor *actor, double out[13]);
 fmca_status fmca_h2_clusters(const fmca_h2_actor *actor, int which, int64_t *begin,
                              int64_t *size, int32_t *level, int32_t *sons, double *bb);
 
-/* Launch order of the forward action's coupling (which = 0: target cluster
- * per workgroup) or leaf output (which = 1: target leaf index into the row
- * tree's leaves), -1 for an idle workgroup; the workgroups of a launch go to
- * the 8 XCDs round-robin and each XCD gets a contiguous piece of every tree
- * level of about equal work.  No FMCA counterpart (FMCA schedules with OpenMP,
- * H2kernelActor.h:30-34).  out == NULL: *len receives the length. */
+/* Work of each workgroup of the forward action's coupling (which = 0) or
+ * leaf output (which = 1), 4 ints per workgroup: target (cluster id / index
+ * into the row tree's leaves), begin and end in its far / near list, flags
+ * (1: the target is split over several workgroups that add into a zeroed
+ * output, 2: this workgroup applies the leaf's far field); all -1 for an
+ * idle workgroup.  Targets above three times the mean work are split; the
+ * workgroups of a launch go to the 8 XCDs round-robin and each XCD gets a
+ * contiguous piece of every tree level of about equal work.  No FMCA
+ * counterpart (FMCA schedules with OpenMP, H2kernelActor.h:30-34).
+ * out == NULL: *len receives the number of workgroups. */
 fmca_status fmca_h2_launch_order(const fmca_h2_actor *actor, int which, int32_t *out,
                                  int64_t *len);
 

@@ -159,10 +159,11 @@ def test_moment_order_table_matches_fmca():
 
 @pytest.mark.parametrize("points", ["uniform", "clustered"])
 def test_launch_orders_cover_every_item_once_with_balanced_xcd_pieces(points):
-    """The coupling / leaf-output launch orders: every target cluster (leaf)
-    exactly once, -1 padding only, 8 XCD pieces per tree level that are
-    contiguous, and per-level XCD work within 1.35x of the level mean where a
-    level has at least 256 items."""
+    """The coupling / leaf-output work lists: every target's far / near list
+    covered exactly once by contiguous pieces, a target split only above three
+    times the mean work (and then flagged), one far-field piece per leaf, -1 rows
+    only for idle workgroups, and per tree level 8 contiguous XCD pieces of
+    work within 1.35x of the level mean where a level has 256 items or more."""
     from fmca_amd import workloads as W
     n = 40000
     P = W.uniform(n, 3, 11) if points == "uniform" else W.gaussian_mixture(n, 3, 16, 0.02, 11)
@@ -170,24 +171,43 @@ def test_launch_orders_cover_every_item_once_with_balanced_xcd_pieces(points):
     cl = H.clusters(fmca_amd.ROWS)
     ncl = len(cl["level"])
     row, col, lr = H.blocks()
-    far = np.bincount(row[lr], minlength=ncl).astype(float) + 0.05
+    nfar = np.bincount(row[lr], minlength=ncl)
     leaves = np.flatnonzero(cl["sons"][:, 0] < 0)
-    for which, nitems in (("coupling", ncl), ("leaf_output", len(leaves))):
+    for which in ("coupling", "leaf_output"):
         order = H.launch_order(which)
         assert len(order) % 8 == 0
-        items = order[order >= 0]
-        assert np.array_equal(np.sort(items), np.arange(nitems))
-        assert np.all(order[order < 0] == -1)
+        idle = order[:, 0] < 0
+        assert np.all(order[idle] == -1)
+        tasks = order[~idle]
+        ntarget = ncl if which == "coupling" else len(leaves)
+        assert np.array_equal(np.unique(tasks[:, 0]), np.arange(ntarget))
+        # per target: pieces contiguous and covering its list; split flag iff several
+        by = {}
+        for t, b, e, f in tasks:
+            by.setdefault(int(t), []).append((int(b), int(e), int(f)))
+        for t, pieces in by.items():
+            pieces.sort()
+            assert all(pieces[k][1] == pieces[k + 1][0] for k in range(len(pieces) - 1))
+            assert all(((f & 1) == 1) == (len(pieces) > 1) for _, _, f in pieces)
+            if which == "coupling":
+                assert pieces[-1][1] - pieces[0][0] == nfar[t]
+            else:
+                assert sum((f & 2) == 2 for _, _, f in pieces) == 1
+        if points == "uniform":
+            assert not (tasks[:, 3] & 1).any()  # uniform points: nothing to split
         if which != "coupling":
             continue
-        xcd = np.empty(nitems, dtype=int)
-        xcd[order[order >= 0]] = np.flatnonzero(order >= 0) % 8
-        for L in np.unique(cl["level"]):
-            ids = np.flatnonzero(cl["level"] == L)
+        xcd = np.flatnonzero(~idle) % 8
+        work = tasks[:, 2] - tasks[:, 1] + 0.05
+        lev = cl["level"][tasks[:, 0]]
+        for L in np.unique(lev):
+            sel = lev == L
             # contiguous pieces: the XCD index never decreases along the level
-            assert np.all(np.diff(xcd[ids]) >= 0)
-            if len(ids) >= 256:
-                w = np.bincount(xcd[ids], weights=far[ids], minlength=8)
+            pos = np.flatnonzero(sel)
+            order_in_level = np.argsort(tasks[sel, 0] * 1000000 + tasks[sel, 1], kind="stable")
+            assert np.all(np.diff(xcd[pos][order_in_level]) >= 0)
+            if sel.sum() >= 256:
+                w = np.bincount(xcd[sel], weights=work[sel], minlength=8)
                 assert w.max() <= 1.35 * w.mean(), (L, w)
 
 
