@@ -380,11 +380,11 @@ def main():
     if dom == "leaf_output":
         flops = 2.0 * E_near * nrhs + 2.0 * q * n * nrhs
         entries = E_near
-        kname = f"k_leaf_output_mfma<float, 3, 1, {nt}>"
+        kname = f"k_leaf_output_mfma<float, 3, 1, {nt}, false>"
     else:
         flops = 2.0 * E_far * nrhs
         entries = E_far
-        kname = f"k_coupling_mfma<float, 3, 1, {nt}>"
+        kname = f"k_coupling_mfma<float, 3, 1, {nt}, false>"
     achieved = flops / t_dom / 1e12
     roofline = {
         "bound": "mfma",

@@ -53,15 +53,13 @@ constexpr int kRows = kWaves * kRT * 16;
 // read rows k..k+3 of a staged operand, and a stride = 16 (mod 32) puts them
 // on disjoint banks (for 32 and for 64 banks)
 constexpr int kRowsP = kRows + 16;
-// source points / nodes staged per LDS chunk (32 for FP64 with 64 rhs columns,
-// to keep the staging within 64 KB of LDS)
+// source points / nodes staged per LDS chunk
 template <typename T, int NT>
-constexpr int kChunkOf = (sizeof(T) == 8 && NT >= 4) ? 32 : 64;
+constexpr int kChunkOf = 64;
 
-// Occupancy target (waves per SIMD) of the MFMA kernels: FP32 with up to 32
-// rhs columns fits 4 waves without spilling; FP64 (twice the registers per
-// value), 64-column tiles and the three-variant run-time switch keep the
-// default.
+// Occupancy target (waves per SIMD) of the MFMA kernels: FP32 fits 4 waves
+// without spilling; FP64 (twice the registers per value) and the
+// three-variant run-time switch keep the default.
 template <typename T, int NT, int KT, bool MULTI>
 constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2 && !(MULTI && KT < 0)) ? 4 : 1;
 
@@ -514,14 +512,14 @@ hipError_t launch_mfma_nt(bool output, const DevTreeView<T> &rows, const DevTree
   return hipGetLastError();
 }
 
-// rhs columns per pass: 16, 32 or (FP32) 64.  Several variants take 32:
-// the staging latency per source cluster, hidden by occupancy, costs more
-// than evaluating the entries again in a further pass.
+// rhs columns per pass: 16 or 32.  Wider passes (64 columns) take fewer
+// entry evaluations but drop to one wave per SIMD, and the staging latency per
+// source cluster, hidden by occupancy, then costs more (measured 7-8 % slower
+// at 64 rhs on uniform and on clustered points).
 template <typename T>
 int mfma_nt(int nrhs, bool multi) {
-  if (nrhs <= 16) return 1;
-  if (nrhs <= 32 || sizeof(T) == 8 || multi) return 2;
-  return 4;
+  (void)multi;
+  return nrhs <= 16 ? 1 : 2;
 }
 
 template <typename T, int D, bool MULTI>
@@ -529,17 +527,11 @@ hipError_t launch_mfma_dm(bool output, const DevTreeView<T> &rows, const DevTree
                           const DevPatternView &pat, const InterpParams<T> &ip,
                           const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom,
                           T *tlhs, double *out, int64_t ldo, bool natural, hipStream_t stream) {
-  switch (mfma_nt<T>(nrhs, MULTI)) {
-    case 1:
-      return launch_mfma_nt<T, D, 1, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom,
-                                            tlhs, out, ldo, natural, stream);
-    case 2:
-      return launch_mfma_nt<T, D, 2, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom,
-                                            tlhs, out, ldo, natural, stream);
-    default:
-      return launch_mfma_nt<T, D, 4, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom,
-                                            tlhs, out, ldo, natural, stream);
-  }
+  if (mfma_nt<T>(nrhs, MULTI) == 1)
+    return launch_mfma_nt<T, D, 1, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs,
+                                          out, ldo, natural, stream);
+  return launch_mfma_nt<T, D, 2, MULTI>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs,
+                                        out, ldo, natural, stream);
 }
 
 template <typename T, int D>
@@ -561,17 +553,11 @@ bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vma
   if (nrhs < kMfmaMinRhs) return false;
   const size_t limit = 64 * 1024;
   const int nt = mfma_nt<T>(nrhs, mfma_multi(kt, vmask));
-  if (output) {
-    const size_t b = nt == 1   ? leaf_output_mfma_lds<T, 1>(D, P1, q)
-                     : nt == 2 ? leaf_output_mfma_lds<T, 2>(D, P1, q)
-                               : leaf_output_mfma_lds<T, 4>(D, P1, q);
-    return b <= limit;
-  }
+  if (output)
+    return (nt == 1 ? leaf_output_mfma_lds<T, 1>(D, P1, q)
+                    : leaf_output_mfma_lds<T, 2>(D, P1, q)) <= limit;
   if (q > kWaves * kRT * 16) return false;
-  const size_t b = nt == 1   ? coupling_mfma_lds<T, 1>(D, q)
-                   : nt == 2 ? coupling_mfma_lds<T, 2>(D, q)
-                             : coupling_mfma_lds<T, 4>(D, q);
-  return b <= limit;
+  return (nt == 1 ? coupling_mfma_lds<T, 1>(D, q) : coupling_mfma_lds<T, 2>(D, q)) <= limit;
 }
 
 template <typename T>
