@@ -98,7 +98,7 @@ struct DevBuf {
 
 // Integer arrays of one tree on the device (shared by both precisions).
 struct DevTreeInts {
-  DevBuf perm, begin, size, son0, son1, parent, leaves;
+  DevBuf perm, pinv, begin, size, son0, son1, parent, leaves;
 };
 
 template <typename T>
@@ -211,7 +211,10 @@ std::vector<T> convert(const std::vector<double> &v) {
 hipError_t upload_ints(DevTreeInts &d, const Tree &T) {
   hipError_t e;
   std::vector<int32_t> b(T.begin.begin(), T.begin.end()), s(T.size.begin(), T.size.end());
+  std::vector<uint32_t> pinv(T.perm.size());
+  for (size_t j = 0; j < T.perm.size(); ++j) pinv[T.perm[j]] = (uint32_t)j;
   if ((e = d.perm.upload(T.perm)) != hipSuccess) return e;
+  if ((e = d.pinv.upload(pinv)) != hipSuccess) return e;
   if ((e = d.begin.upload(b)) != hipSuccess) return e;
   if ((e = d.size.upload(s)) != hipSuccess) return e;
   if ((e = d.son0.upload(T.son0)) != hipSuccess) return e;
@@ -258,6 +261,7 @@ DevTreeView<T> make_view(const Tree &tr, const DevTreeInts &di, const DevTreeRea
   v.depth = tr.depth;
   v.npts = tr.npts;
   v.perm = di.perm.as<uint32_t>();
+  v.pinv = di.pinv.as<uint32_t>();
   v.pts = dr.pts.template as<T>();
   v.unit = dr.unit.template as<T>();
   v.bbmin = dr.bbmin.template as<T>();

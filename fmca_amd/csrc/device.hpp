@@ -17,6 +17,7 @@ struct DevTreeView {
   int dim = 0, ncl = 0, nleaves = 0, depth = 0;
   int64_t npts = 0;
   const uint32_t *perm = nullptr;   // cluster position -> natural index
+  const uint32_t *pinv = nullptr;   // natural index -> cluster position
   const T *pts = nullptr;           // dim x npts, cluster order
   const T *unit = nullptr;          // dim x npts: (x - bbmin) / bbsize of the point's leaf
   const T *bbmin = nullptr;         // ncl x dim

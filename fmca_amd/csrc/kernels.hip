@@ -27,19 +27,36 @@ namespace {
 using namespace dev;
 
 // ---------------------------------------------------------------------------
-// 1. rhs (column-major, natural order) -> xs (cluster order, point-major)
+// 1. rhs (column-major, natural order) -> xs (cluster order, point-major).
+// A workgroup moves kPermRows natural rows in column chunks of kPermCols
+// through LDS: the reads run down the columns (consecutive rows, coalesced)
+// and the writes along each row at its cluster position pinv[i]
+// (consecutive columns, contiguous).
+constexpr int kPermRows = 64, kPermCols = 64;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_permute_in(const double *__restrict__ rhs,
                                                        int64_t ld, int nrhs, int64_t n,
-                                                       const uint32_t *__restrict__ perm,
+                                                       const uint32_t *__restrict__ pinv,
                                                        T *__restrict__ xs) {
-  const int64_t total = n * nrhs;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = e / nrhs;
-    const int r = (int)(e - j * nrhs);
-    const int64_t src = perm ? (int64_t)perm[j] : j;
-    xs[e] = (T)rhs[src + r * ld];
+  __shared__ T S[kPermRows * (kPermCols + 1)];
+  const int64_t i0 = (int64_t)blockIdx.x * kPermRows;
+  const int rows = (int)min<int64_t>(kPermRows, n - i0);
+  for (int c0 = 0; c0 < nrhs; c0 += kPermCols) {
+    const int cols = min(kPermCols, nrhs - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kPermRows * cols; e += blockDim.x) {
+      const int ii = e % kPermRows, r = e / kPermRows;
+      if (ii < rows) S[ii * (kPermCols + 1) + r] = (T)rhs[i0 + ii + (int64_t)(c0 + r) * ld];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kPermRows * cols; e += blockDim.x) {
+      const int ii = e / cols, r = e % cols;
+      if (ii < rows) {
+        const int64_t pos = pinv ? (int64_t)pinv[i0 + ii] : i0 + ii;
+        xs[pos * nrhs + c0 + r] = S[ii * (kPermCols + 1) + r];
+      }
+    }
   }
 }
 
@@ -596,10 +613,9 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   mark(PH_PERMUTE);
   // 1. permute into cluster order
   {
-    const int64_t total = cols.npts * nrhs;
-    const int grid = (int)std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
-    hipLaunchKernelGGL((k_permute_in<T>), dim3(grid), dim3(kBlock), 0, stream, rhs, ld_rhs, nrhs,
-                       cols.npts, natural ? cols.perm : nullptr, buf.xs);
+    const int64_t grid = (cols.npts + kPermRows - 1) / kPermRows;
+    hipLaunchKernelGGL((k_permute_in<T>), dim3((unsigned)grid), dim3(kBlock), 0, stream, rhs,
+                       ld_rhs, nrhs, cols.npts, natural ? cols.pinv : nullptr, buf.xs);
   }
   // 2. leaf moments
   mark(PH_UPWARD_LEAF);

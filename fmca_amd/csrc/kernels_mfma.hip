@@ -61,7 +61,7 @@ constexpr int kChunkOf = 64;
 // without spilling; FP64 (twice the registers per value) and the
 // three-variant run-time switch keep the default.
 template <typename T, int NT, int KT, bool MULTI>
-constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2 && !(MULTI && KT < 0)) ? 4 : 1;
+constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2 && !(MULTI && KT < 0)) ? 3 : 1;
 
 // Packed tensor index of every moment: a0 | a1 << 8 | a2 << 16.
 __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0xff; }
@@ -181,24 +181,49 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
           }
         }
       }
-      // near field: A = K(rows, sources) generated per lane, B = source rhs
-      for (int nb = n0; nb < n1; ++nb) {
+      // near field: A = K(rows, sources) generated per lane, B = source rhs.
+      // The chunks of the near list are staged through registers one chunk
+      // ahead: the global loads of chunk c+1 overlap the MFMA work of chunk c.
+      constexpr int KX = kChunk * NC / 256;  // staged rhs values per thread
+      static_assert(kChunk * NC % 256 == 0 && D * kChunk <= 256, "staging layout");
+      T py = T(0), px[KX];
+      int nb = n0, j0 = 0, pnj = 0;
+      auto fetch = [&]() {
         const int s = pat.near_src[nb];
         const int sb = cols.begin[s], ss = cols.size[s];
-        for (int j0 = 0; j0 < ss; j0 += kChunk) {
-          const int nj = min(kChunk, ss - j0);
-          __syncthreads();
-          for (int e = threadIdx.x; e < D * kChunk; e += blockDim.x) {
-            const int d = e / kChunk, jj = e % kChunk;
-            Ys[e] = jj < nj ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj] : T(0);
-          }
-          for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
-            const int jj = e / NC, r = e % NC;
-            Xs[jj * NCP + r] = (jj < nj && rc + r < nrhs)
-                                   ? xs[(int64_t)(sb + j0 + jj) * nrhs + rc + r]
-                                   : T(0);
-          }
-          __syncthreads();
+        pnj = min(kChunk, ss - j0);
+        if (threadIdx.x < D * kChunk) {
+          const int d = threadIdx.x / kChunk, jj = threadIdx.x % kChunk;
+          py = jj < pnj ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < KX; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          const int jj = e / NC, r = e % NC;
+          px[k] = (jj < pnj && rc + r < nrhs) ? xs[(int64_t)(sb + j0 + jj) * nrhs + rc + r]
+                                              : T(0);
+        }
+        j0 += kChunk;
+        if (j0 >= ss) {
+          ++nb;
+          j0 = 0;
+        }
+      };
+      bool more = nb < n1;
+      if (more) fetch();
+      while (more) {
+        const int nj = pnj;
+        __syncthreads();
+        if (threadIdx.x < D * kChunk) Ys[threadIdx.x] = py;
+#pragma unroll
+        for (int k = 0; k < KX; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          Xs[(e / NC) * NCP + e % NC] = px[k];
+        }
+        __syncthreads();
+        more = nb < n1;
+        if (more) fetch();
+        {
           const int nsteps = (nj + 3) & ~3;
 #pragma unroll 2
           for (int ks = 0; ks < nsteps; ks += 4) {
@@ -330,56 +355,108 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
     for (int nt = 0; nt < NT; ++nt) tv[nt] = kp.tv[min((rc >> 4) + nt, kMaxVariantGroups - 1)];
     // column tiles of this pass that hold rhs columns (the rest are skipped)
     const int ntl = min(NT, (nrhs - rc + 15) >> 4);
-    for (int f = f0; f < f1; ++f) {
-      const int s = pat.far_src[f];
-      for (int be = 0; be < qpad; be += kChunk) {
-        const int nbeta = min(kChunk, qpad - be);
-        __syncthreads();
-        if (be == 0)
-          for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
-            const int d = e / P1, b = e % P1;
-            Y[d * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+    // K-steps of one staged chunk of source nodes [be, be + nbeta)
+    auto compute_chunk = [&](int be, int nbeta) {
+#pragma unroll 2
+      for (int ks = ks0; ks < nbeta; ks += kstep) {
+        const int bl = ks + lk;
+        const int pk = bidx[be + bl];
+        T y[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) y[d] = Y[d * kMaxP1 + unpack(pk, d)];
+        T b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NCP + nt * 16 + lr];
+#pragma unroll
+        for (int rt = 0; rt < kRT; ++rt) {
+          if (!tile_on[rt]) continue;
+          T r2 = 0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const T dd = x[rt][d] - y[d];
+            r2 = fmat(dd, dd, r2);
           }
-        for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
+          if constexpr (MULTI) {
+            T ev[3];
+            kernel_r2_all<KT, T>(r2, kp, ev);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+              if (nt < ntl) acc[rt][nt] = M::mma(pick_variant(ev, tv[nt]), b[nt], acc[rt][nt]);
+          } else {
+            const T e = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+              if (nt < ntl) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
+          }
+        }
+      }
+    };
+    if constexpr (sizeof(T) == 4) {
+      // FP32: the (far source, node chunk) pairs of the list are staged
+      // through registers one chunk ahead, so that the global loads of chunk
+      // c+1 overlap the MFMA work of chunk c.
+      constexpr int KM = kChunk * NC / 256;  // staged moments per thread
+      static_assert(kChunk * NC % 256 == 0 && kMaxDim * kMaxP1 <= 256, "staging layout");
+      T pyn = T(0), pm[KM];
+      int f = f0, nbe = 0, pbe = 0;  // next chunk to fetch: (f, nbe); fetched: pbe
+      auto fetch = [&]() {
+        const int s = pat.far_src[f];
+        pbe = nbe;
+        if (nbe == 0 && threadIdx.x < D * P1) {
+          const int d = threadIdx.x / P1, b = threadIdx.x % P1;
+          pyn = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+        }
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const int e = threadIdx.x + 256 * k;
           const int bl = e / NC, r = e % NC;
-          const int beta = be + bl;
-          Ms[bl * NCP + r] = (beta < q && rc + r < nrhs)
-                                 ? mom[((int64_t)s * q + beta) * nrhs + rc + r]
-                                 : T(0);
+          const int beta = nbe + bl;
+          pm[k] =
+              (beta < q && rc + r < nrhs) ? mom[((int64_t)s * q + beta) * nrhs + rc + r] : T(0);
+        }
+        nbe += kChunk;
+        if (nbe >= qpad) {
+          ++f;
+          nbe = 0;
+        }
+      };
+      bool more = f < f1;
+      if (more) fetch();
+      while (more) {
+        const int be = pbe;
+        __syncthreads();
+        if (be == 0 && threadIdx.x < D * P1)
+          Y[(threadIdx.x / P1) * kMaxP1 + threadIdx.x % P1] = pyn;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          Ms[(e / NC) * NCP + e % NC] = pm[k];
         }
         __syncthreads();
-#pragma unroll 2
-        for (int ks = ks0; ks < nbeta; ks += kstep) {
-          const int bl = ks + lk;
-          const int pk = bidx[be + bl];
-          T y[D];
-#pragma unroll
-          for (int d = 0; d < D; ++d) y[d] = Y[d * kMaxP1 + unpack(pk, d)];
-          T b[NT];
-#pragma unroll
-          for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NCP + nt * 16 + lr];
-#pragma unroll
-          for (int rt = 0; rt < kRT; ++rt) {
-            if (!tile_on[rt]) continue;
-            T r2 = 0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              const T dd = x[rt][d] - y[d];
-              r2 = fmat(dd, dd, r2);
+        more = f < f1;
+        if (more) fetch();
+        compute_chunk(be, min(kChunk, qpad - be));
+      }
+    } else {
+      // FP64 (one wave per SIMD): a lookahead measured slower; stage in place
+      for (int f = f0; f < f1; ++f) {
+        const int s = pat.far_src[f];
+        for (int be = 0; be < qpad; be += kChunk) {
+          __syncthreads();
+          if (be == 0)
+            for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
+              const int d = e / P1, b = e % P1;
+              Y[d * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
             }
-            if constexpr (MULTI) {
-              T ev[3];
-              kernel_r2_all<KT, T>(r2, kp, ev);
-#pragma unroll
-              for (int nt = 0; nt < NT; ++nt)
-                if (nt < ntl) acc[rt][nt] = M::mma(pick_variant(ev, tv[nt]), b[nt], acc[rt][nt]);
-            } else {
-              const T e = kernel_r2<KT, T>(r2, kp);
-#pragma unroll
-              for (int nt = 0; nt < NT; ++nt)
-                if (nt < ntl) acc[rt][nt] = M::mma(e, b[nt], acc[rt][nt]);
-            }
+          for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
+            const int bl = e / NC, r = e % NC;
+            const int beta = be + bl;
+            Ms[bl * NCP + r] = (beta < q && rc + r < nrhs)
+                                   ? mom[((int64_t)s * q + beta) * nrhs + rc + r]
+                                   : T(0);
           }
+          __syncthreads();
+          compute_chunk(be, min(kChunk, qpad - be));
         }
       }
     }
