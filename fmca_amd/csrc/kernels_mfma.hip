@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   int *bidx = reinterpret_cast<int *>(smem_raw);                     // [qpad]
   T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][kMaxP1]
   T *Ms = Y + D * kMaxP1;                                            // [kChunk][NCP]
-  T *Red = Ms + kChunk * NCP;  // [kRT * 16][NC] partial sums of the K-split (q <= 64)
+  T *Red = Ms + kChunk * NCP;  // [2 kRT * 16][NC] partial sums of the K-split (q <= 128)
   const int lane = threadIdx.x & 63;
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -318,8 +318,13 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   const int t = pat.far_task[4 * task], f0 = pat.far_task[4 * task + 1],
             f1 = pat.far_task[4 * task + 2];
   const bool split = (pat.far_task[4 * task + 3] & kTaskSplit) != 0;
-  const bool ksplit = q <= kRT * 16;
-  const int ks0 = ksplit ? 4 * wave : 0, kstep = ksplit ? 4 * kWaves : 4;
+  // K-split: with R row tiles, kspl groups of waves (4 for R <= kRT, 2 for
+  // R <= 2 kRT, else 1) share the K-steps, so that every wave holds up to kRT
+  // row tiles; wave = rg + wrows * sg
+  const int R = (q + 15) / 16;
+  const int kspl = R <= kRT ? 4 : (R <= 2 * kRT ? 2 : 1);
+  const int wrows = kWaves / kspl, rg = wave % wrows, sg = wave / wrows;
+  const int ks0 = 4 * sg, kstep = 4 * kspl;
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
     int rem = e < q ? e : 0, pk = 0;
 #pragma unroll
@@ -334,7 +339,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   bool tile_on[kRT];
 #pragma unroll
   for (int rt = 0; rt < kRT; ++rt) {
-    const int tile_row0 = (ksplit ? rt : wave + kWaves * rt) * 16;
+    const int tile_row0 = (rg + wrows * rt) * 16;
     tile_on[rt] = tile_row0 < q;
     int rem = tile_row0 + lr < q ? tile_row0 + lr : 0;
 #pragma unroll
@@ -460,10 +465,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
         }
       }
     }
-    if (ksplit) {
-      // sum the waves' partials in LDS in wave order, then store row-contiguous
-      for (int w = 0; w < kWaves; ++w) {
-        if (wave == w) {
+    if (kspl > 1) {
+      // sum the wave groups' partials in LDS in group order, then store
+      // row-contiguous
+      for (int g = 0; g < kspl; ++g) {
+        if (sg == g) {
 #pragma unroll
           for (int rt = 0; rt < kRT; ++rt) {
             if (!tile_on[rt]) continue;
@@ -472,9 +478,9 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
               if (nt >= ntl) continue;
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
-                const int alpha = rt * 16 + M::crow(lk, v);
-                T *p = Red + alpha * NC + nt * 16 + lr;
-                *p = (w == 0 ? T(0) : *p) + acc[rt][nt][v];
+                const int alpha = (rg + wrows * rt) * 16 + M::crow(lk, v);
+                T *dst = Red + alpha * NC + nt * 16 + lr;
+                *dst = (g == 0 ? T(0) : *dst) + acc[rt][nt][v];
               }
             }
           }
@@ -501,7 +507,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
           const int col = rc + nt * 16 + lr;
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int alpha = (wave + kWaves * rt) * 16 + M::crow(lk, v);
+            const int alpha = (rg + wrows * rt) * 16 + M::crow(lk, v);
             if (alpha < q && col < nrhs) {
               T *p = tlhs + ((int64_t)t * q + alpha) * nrhs + col;
               if (split)
@@ -530,7 +536,7 @@ template <typename T, int NT>
 size_t coupling_mfma_lds(int D, int q) {
   constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
-  const size_t red = q <= kRT * 16 ? (size_t)kRT * 16 * 16 * NT : 0;  // K-split partials
+  const size_t red = q <= 2 * kRT * 16 ? (size_t)2 * kRT * 16 * 16 * NT : 0;  // K-split partials
   return (size_t)((qpad + 3) & ~3) * sizeof(int) +
          ((size_t)D * kMaxP1 + (size_t)kChunk * (16 * NT + 16) + red) * sizeof(T);
 }
