@@ -358,6 +358,7 @@ KernelParams<T> kernel_params(const fmca_h2_actor *a, const VariantGroups &g) {
       break;
   }
   k.s = (T)sc;
+  k.sl2 = (T)(sc * 1.4426950408889634074);
   k.inv_l = (T)(1.0 / a->l);
   k.c2 = (T)(a->c * a->c);
   return k;

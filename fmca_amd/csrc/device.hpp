@@ -78,6 +78,7 @@ struct KernelParams {
   int kt = 0;
   T l = 1, c = 1, nu = 1;
   T s = 1;      // radial scale (see above)
+  T sl2 = 1;    // s * log2(e): FP32 exponentials are hardware exp2
   T inv_l = 1;  // 1 / l
   T c2 = 1;     // c * c
   int vmask = 1;

@@ -84,6 +84,16 @@ __device__ __forceinline__ double fmat(double a, double b, double c) {
 }
 __device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// exp(-u * s): in FP32 one hardware exp2 of u * (s log2 e), the product
+// formed on the host (kp.sl2); in FP64 the exp above.
+template <typename T>
+__device__ __forceinline__ T exp_neg_scaled(T u, const KernelParams<T> &kp) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_exp2f(-u * kp.sl2);
+  else
+    return Math<T>::exp_(-u * kp.s);
+}
+
 // Kernel value from the squared distance (CovarianceKernel.h:56-116) with the
 // scales of KernelParams.  KT < 0 selects the type at run time (uniform
 // branch) for the less common kernels.
@@ -93,14 +103,14 @@ __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
   const int kt = KT >= 0 ? KT : kp.kt;
   switch (kt) {
     case KT_EXPONENTIAL:
-      return M::exp_(-M::sqrt_(r2) * kp.s);
+      return exp_neg_scaled(M::sqrt_(r2), kp);
     case KT_MATERN32: {
-      const T a = M::sqrt_(r2) * kp.s;
-      return (T(1) + a) * M::exp_(-a);
+      const T r = M::sqrt_(r2), a = r * kp.s;
+      return (T(1) + a) * exp_neg_scaled(r, kp);
     }
     case KT_MATERN52: {
-      const T a = M::sqrt_(r2) * kp.s;
-      return (T(1) + a + a * a * T(1. / 3.)) * M::exp_(-a);
+      const T r = M::sqrt_(r2), a = r * kp.s;
+      return (T(1) + a + a * a * T(1. / 3.)) * exp_neg_scaled(r, kp);
     }
     case KT_MATERN72: {
       const T x = M::sqrt_(r2) * kp.inv_l;
@@ -115,7 +125,7 @@ __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
              M::exp_(-T(3) * x);
     }
     case KT_GAUSSIAN:
-      return M::exp_(r2 * kp.s);
+      return exp_neg_scaled(-r2, kp);
     case KT_INVMULTIQUADRIC:
       return T(1) / M::sqrt_(r2 * kp.inv_l * kp.inv_l + kp.c2);
     case KT_MULTIQUADRIC:
@@ -210,7 +220,7 @@ __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T
   using M = Math<T>;
   if constexpr (KT == KT_EXPONENTIAL) {
     // s = 1 / l: value e, dk/dl = r e / l^2, phi = -e / (l r)
-    const T r = M::sqrt_(r2), e = M::exp_(-r * kp.s);
+    const T r = M::sqrt_(r2), e = exp_neg_scaled(r, kp);
     ev[KV_VALUE] = e;
     ev[KV_DL] = r * (kp.s * kp.s) * e;
     ev[KV_PHI] = r > T(0) ? -(kp.s / r) * e : T(0);
@@ -218,11 +228,11 @@ __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T
     // the shared exponential; dk/dl = -(r^2 / l) phi for these f(r / l)
     T phi;
     if constexpr (KT == KT_GAUSSIAN) {
-      const T e = M::exp_(r2 * kp.s);
+      const T e = exp_neg_scaled(-r2, kp);
       ev[KV_VALUE] = e;
       phi = -(kp.inv_l * kp.inv_l) * e;
     } else {
-      const T x = M::sqrt_(r2) * kp.s, e = M::exp_(-x);
+      const T r = M::sqrt_(r2), x = r * kp.s, e = exp_neg_scaled(r, kp);
       if constexpr (KT == KT_MATERN32) {
         ev[KV_VALUE] = (T(1) + x) * e;
         phi = -kp.s * kp.s * e;
