@@ -179,8 +179,20 @@ def side_small_dense(args):
     for name, prec in (("fp64", fmca_amd.FP64), ("fp32", fmca_amd.FP32)):
         H.action_device(x, precision=prec)
         t, y = _sync_time(lambda: H.action_device(x, precision=prec), 10)
-        res[name] = {"ms": t * 1e3, "rel_err_vs_dense": float(torch.linalg.norm(y - yd)
-                                                              / torch.linalg.norm(yd))}
+        # the same action captured once in a graph and replayed (no launch overhead)
+        yg = torch.empty_like(x)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            H.action_device(x, out=yg, precision=prec)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            H.action_device(x, out=yg, precision=prec)
+        g.replay()
+        tg, _ = _sync_time(g.replay, 20)
+        res[name] = {"ms": t * 1e3, "graph_replay_ms": tg * 1e3,
+                     "rel_err_vs_dense": float(torch.linalg.norm(y - yd) / torch.linalg.norm(yd)),
+                     "graph_equals_eager": bool(torch.equal(yg, y))}
     A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 3, 0.8)
     nthreads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     t0 = time.perf_counter()

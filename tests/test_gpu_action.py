@@ -114,3 +114,36 @@ def test_clustered_points_with_split_targets(dim, deg, kernel, nrhs):
     y32 = H.action_device(xd, precision=fmca_amd.FP32).cpu().numpy()
     assert rel_err(y64, ref) <= FP64_TOL
     assert rel_err(y32, y64) <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_action_replays_in_a_hip_graph(precision):
+    """The device action enqueues only kernels on the caller's stream (no
+    memsets, no allocation once the workspaces are sized), so it can be
+    captured in a graph.  Captured once, replayed three times with eager work
+    between the replays (the input changed in place); every replay equals an
+    eager action on the same input."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print(f"[case] graph replay {precision}", flush=True)
+    prec = getattr(fmca_amd, precision)
+    n, nrhs = 6000, 16
+    P = W.uniform(n, 3, 51)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.4), P, P, 3, 0.8)
+    x = torch.randn(nrhs, n, dtype=torch.float64, device="cuda").t()
+    y = torch.empty(nrhs, n, dtype=torch.float64, device="cuda").t()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        H.action_device(x, out=y, precision=prec)  # sizes the workspaces
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        H.action_device(x, out=y, precision=prec)
+    for k in range(3):
+        x.mul_(-0.5).add_(0.25 * k)          # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        ref = H.action_device(x, precision=prec)
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref) or (torch.linalg.norm(y - ref) / torch.linalg.norm(ref)).item() <= 1e-15
