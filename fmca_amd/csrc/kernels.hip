@@ -151,68 +151,87 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
 
 // Mode-i product with a P1 x P1 matrix on a [q][nr] tensor held in LDS:
 // out[lin with a_i] = sum_b M[a_i][b] in[lin with b]   (transpose: M[b][a_i]).
-template <typename T>
+template <typename T, int NRC>
 __device__ __forceinline__ void mode_product(const T *__restrict__ in, T *__restrict__ out,
-                                             const T *__restrict__ M, int P1, int q, int nr,
-                                             int stride, bool transpose) {
-  for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
-    const int lin = e / nr, r = e - lin * nr;
-    const int ai = (lin / stride) % P1;
+                                             const T *__restrict__ M,
+                                             const int *__restrict__ aidx, int dimi, int P1,
+                                             int q, int stride, bool transpose) {
+  for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+    const int lin = e / NRC, r = e % NRC;
+    const int ai = (aidx[lin] >> (8 * dimi)) & 0xff;
     const int base = lin - ai * stride;
     T s = 0;
     for (int b = 0; b < P1; ++b) {
       const T m = transpose ? M[b * P1 + ai] : M[ai * P1 + b];
-      s += m * in[(base + b * stride) * nr + r];
+      s += m * in[(base + b * stride) * NRC + r];
     }
     out[e] = s;
+  }
+}
+
+// aidx[lin] = a0 | a1 << 8 | a2 << 16 for the moment index lin = a0 + P1 a1 + P1^2 a2
+template <int D>
+__device__ __forceinline__ void fill_moment_index(int *aidx, int q, int P1) {
+  for (int e = threadIdx.x; e < q; e += blockDim.x) {
+    int rem = e, pk = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      pk |= (rem % P1) << (8 * d);
+      rem /= P1;
+    }
+    aidx[e] = pk;
   }
 }
 
 // ---------------------------------------------------------------------------
 // 3. upward transfer for the clusters [cbeg, cend) of one level:
 //    mom[t] = sum_{s in sons(t)} (T_s0 (x) ... ) mom[s]
-template <typename T, int D>
+// NRC rhs columns per pass (compile time), the moments staged [q][NRC] in LDS
+// and the Kronecker factors applied as D mode products.
+template <typename T, int D, int NRC>
 __global__ __launch_bounds__(kBlock) void k_upward_transfer(DevTreeView<T> tr, InterpParams<T> ip,
-                                                            int cbeg, int nrhs, int nrc,
+                                                            int cbeg, int nrhs,
                                                             T *__restrict__ mom) {
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int c = cbeg + blockIdx.x;
   if (tr.son0[c] < 0) return;
-  T *A = reinterpret_cast<T *>(smem_raw);
-  T *B = A + q * nrc;
-  T *O = B + q * nrc;
-  T *Tm = O + q * nrc;  // D x P1 x P1
-  for (int rc = 0; rc < nrhs; rc += nrc) {
-    const int nr = min(nrc, nrhs - rc);
-    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) O[e] = 0;
+  int *aidx = reinterpret_cast<int *>(smem_raw);
+  T *A = reinterpret_cast<T *>(aidx + ((q + 3) & ~3));
+  T *B = A + q * NRC;
+  T *O = B + q * NRC;
+  T *Tm = O + q * NRC;  // D x P1 x P1
+  fill_moment_index<D>(aidx, q, P1);
+  for (int rc = 0; rc < nrhs; rc += NRC) {
+    const int nr = min(NRC, nrhs - rc);
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) O[e] = 0;
     for (int si = 0; si < 2; ++si) {
       const int s = si ? tr.son1[c] : tr.son0[c];
       __syncthreads();
       for (int e = threadIdx.x; e < D * P1 * P1; e += blockDim.x)
         Tm[e] = tr.transfer[(int64_t)s * D * P1 * P1 + e];
-      for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
-        const int lin = e / nr, r = e - lin * nr;
-        A[e] = mom[((int64_t)s * q + lin) * nrhs + rc + r];
+      for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+        const int lin = e / NRC, r = e % NRC;
+        A[e] = r < nr ? mom[((int64_t)s * q + lin) * nrhs + rc + r] : T(0);
       }
       __syncthreads();
       T *cur = A, *nxt = B;
       int stride = 1;
 #pragma unroll
       for (int i = 0; i < D; ++i) {
-        mode_product<T>(cur, nxt, Tm + i * P1 * P1, P1, q, nr, stride, false);
+        mode_product<T, NRC>(cur, nxt, Tm + i * P1 * P1, aidx, i, P1, q, stride, false);
         __syncthreads();
         T *tmp = cur;
         cur = nxt;
         nxt = tmp;
         stride *= P1;
       }
-      for (int e = threadIdx.x; e < q * nr; e += blockDim.x) O[e] += cur[e];
+      for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) O[e] += cur[e];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
-      const int lin = e / nr, r = e - lin * nr;
-      mom[((int64_t)c * q + lin) * nrhs + rc + r] = O[e];
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int lin = e / NRC, r = e % NRC;
+      if (r < nr) mom[((int64_t)c * q + lin) * nrhs + rc + r] = O[e];
     }
     __syncthreads();
   }
@@ -324,42 +343,43 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
 // ---------------------------------------------------------------------------
 // 5. downward transfer for the clusters [cbeg, cend) of one level >= 1:
 //    tlhs[c] += (T_c0 (x) ...)^T tlhs[parent(c)]
-template <typename T, int D>
+template <typename T, int D, int NRC>
 __global__ __launch_bounds__(kBlock) void k_downward_transfer(DevTreeView<T> tr,
                                                               InterpParams<T> ip, int cbeg,
-                                                              int nrhs, int nrc,
-                                                              T *__restrict__ tlhs) {
+                                                              int nrhs, T *__restrict__ tlhs) {
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int c = cbeg + blockIdx.x;
   const int p = tr.parent[c];
-  T *A = reinterpret_cast<T *>(smem_raw);
-  T *B = A + q * nrc;
-  T *Tm = B + q * nrc;
+  int *aidx = reinterpret_cast<int *>(smem_raw);
+  T *A = reinterpret_cast<T *>(aidx + ((q + 3) & ~3));
+  T *B = A + q * NRC;
+  T *Tm = B + q * NRC;
+  fill_moment_index<D>(aidx, q, P1);
   for (int e = threadIdx.x; e < D * P1 * P1; e += blockDim.x)
     Tm[e] = tr.transfer[(int64_t)c * D * P1 * P1 + e];
-  for (int rc = 0; rc < nrhs; rc += nrc) {
-    const int nr = min(nrc, nrhs - rc);
+  for (int rc = 0; rc < nrhs; rc += NRC) {
+    const int nr = min(NRC, nrhs - rc);
     __syncthreads();
-    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
-      const int lin = e / nr, r = e - lin * nr;
-      A[e] = tlhs[((int64_t)p * q + lin) * nrhs + rc + r];
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int lin = e / NRC, r = e % NRC;
+      A[e] = r < nr ? tlhs[((int64_t)p * q + lin) * nrhs + rc + r] : T(0);
     }
     __syncthreads();
     T *cur = A, *nxt = B;
     int stride = 1;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      mode_product<T>(cur, nxt, Tm + i * P1 * P1, P1, q, nr, stride, true);
+      mode_product<T, NRC>(cur, nxt, Tm + i * P1 * P1, aidx, i, P1, q, stride, true);
       __syncthreads();
       T *tmp = cur;
       cur = nxt;
       nxt = tmp;
       stride *= P1;
     }
-    for (int e = threadIdx.x; e < q * nr; e += blockDim.x) {
-      const int lin = e / nr, r = e - lin * nr;
-      tlhs[((int64_t)c * q + lin) * nrhs + rc + r] += cur[e];
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int lin = e / NRC, r = e % NRC;
+      if (r < nr) tlhs[((int64_t)c * q + lin) * nrhs + rc + r] += cur[e];
     }
   }
 }
@@ -624,23 +644,44 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
       const size_t lds = ((size_t)D * P1 * 64 + 64 * 1) * sizeof(T);
       hipLaunchKernelGGL((k_upward_leaf<T, D, 1>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
                          cols, ip, buf.xs, nrhs, buf.mom);
-    } else {
+    } else if (nrhs <= 8) {
       const size_t lds = ((size_t)D * P1 * 64 + 64 * 8) * sizeof(T);
       hipLaunchKernelGGL((k_upward_leaf<T, D, 8>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
+                         cols, ip, buf.xs, nrhs, buf.mom);
+    } else {
+      const size_t lds = ((size_t)D * P1 * 64 + 64 * 32) * sizeof(T);
+      hipLaunchKernelGGL((k_upward_leaf<T, D, 32>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
                          cols, ip, buf.xs, nrhs, buf.mom);
     }
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  // 3. upward transfers, deepest inner level first
+  // 3. upward transfers, deepest inner level first; columns per pass: the
+  // largest of 32, 8, 1 whose staging fits 64 KB of LDS
   mark(PH_UPWARD_TRANSFER);
-  const size_t budget = 64 * 1024 / sizeof(T) - (size_t)D * P1 * P1;
-  const int nrc_up = (int)std::max<size_t>(1, std::min<size_t>(nrhs, budget / (3 * (size_t)q)));
-  const int nrc_dn = (int)std::max<size_t>(1, std::min<size_t>(nrhs, budget / (2 * (size_t)q)));
+  auto transfer_lds = [&](int arrays, int nrc) {
+    return (size_t)((q + 3) & ~3) * sizeof(int) +
+           ((size_t)arrays * q * nrc + (size_t)D * P1 * P1) * sizeof(T);
+  };
+  auto transfer_nrc = [&](int arrays) {
+    if (nrhs > 8 && transfer_lds(arrays, 32) <= kLdsLimit) return 32;
+    if (nrhs > 1 && transfer_lds(arrays, 8) <= kLdsLimit) return 8;
+    return 1;
+  };
+  const int nrc_up = transfer_nrc(3), nrc_dn = transfer_nrc(2);
+  if (transfer_lds(3, nrc_up) > kLdsLimit || transfer_lds(2, nrc_dn) > kLdsLimit)
+    return hipErrorInvalidConfiguration;
   for (int L = lcols.depth - 1; L >= 0; --L) {
     const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
-    const size_t lds = ((size_t)3 * q * nrc_up + (size_t)D * P1 * P1) * sizeof(T);
-    hipLaunchKernelGGL((k_upward_transfer<T, D>), dim3(nc), dim3(kBlock), lds, stream, cols, ip,
-                       cb, nrhs, nrc_up, buf.mom);
+    const size_t lds = transfer_lds(3, nrc_up);
+    if (nrc_up == 32)
+      hipLaunchKernelGGL((k_upward_transfer<T, D, 32>), dim3(nc), dim3(kBlock), lds, stream, cols,
+                         ip, cb, nrhs, buf.mom);
+    else if (nrc_up == 8)
+      hipLaunchKernelGGL((k_upward_transfer<T, D, 8>), dim3(nc), dim3(kBlock), lds, stream, cols,
+                         ip, cb, nrhs, buf.mom);
+    else
+      hipLaunchKernelGGL((k_upward_transfer<T, D, 1>), dim3(nc), dim3(kBlock), lds, stream, cols,
+                         ip, cb, nrhs, buf.mom);
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 4. coupling (every target cluster is written, empty far lists give zeros)
@@ -655,9 +696,16 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   mark(PH_DOWNWARD_TRANSFER);
   for (int L = 1; L <= lrows.depth; ++L) {
     const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
-    const size_t lds = ((size_t)2 * q * nrc_dn + (size_t)D * P1 * P1) * sizeof(T);
-    hipLaunchKernelGGL((k_downward_transfer<T, D>), dim3(nc), dim3(kBlock), lds, stream, rows, ip,
-                       cb, nrhs, nrc_dn, buf.tlhs);
+    const size_t lds = transfer_lds(2, nrc_dn);
+    if (nrc_dn == 32)
+      hipLaunchKernelGGL((k_downward_transfer<T, D, 32>), dim3(nc), dim3(kBlock), lds, stream,
+                         rows, ip, cb, nrhs, buf.tlhs);
+    else if (nrc_dn == 8)
+      hipLaunchKernelGGL((k_downward_transfer<T, D, 8>), dim3(nc), dim3(kBlock), lds, stream,
+                         rows, ip, cb, nrhs, buf.tlhs);
+    else
+      hipLaunchKernelGGL((k_downward_transfer<T, D, 1>), dim3(nc), dim3(kBlock), lds, stream,
+                         rows, ip, cb, nrhs, buf.tlhs);
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 6. leaf output: far field through V^T plus the near field
