@@ -53,9 +53,11 @@ constexpr int kRows = kWaves * kRT * 16;
 // read rows k..k+3 of a staged operand, and a stride = 16 (mod 32) puts them
 // on disjoint banks (for 32 and for 64 banks)
 constexpr int kRowsP = kRows + 16;
-// source points / nodes staged per LDS chunk
+// source points / nodes staged per LDS chunk: 128 for FP32 32-column passes
+// (one chunk per far source at q <= 128, half the barriers; 1.3 % faster at
+// n = 1e6, 32 rhs), 64 otherwise
 template <typename T, int NT>
-constexpr int kChunkOf = 64;
+constexpr int kChunkOf = (sizeof(T) == 4 && NT == 2) ? 128 : 64;
 
 // Occupancy target (waves per SIMD) of the MFMA kernels: FP32 fits 4 waves
 // without spilling; FP64 (twice the registers per value) and the
@@ -184,17 +186,22 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
       // near field: A = K(rows, sources) generated per lane, B = source rhs.
       // The chunks of the near list are staged through registers one chunk
       // ahead: the global loads of chunk c+1 overlap the MFMA work of chunk c.
-      constexpr int KX = kChunk * NC / 256;  // staged rhs values per thread
-      static_assert(kChunk * NC % 256 == 0 && D * kChunk <= 256, "staging layout");
-      T py = T(0), px[KX];
+      constexpr int KX = kChunk * NC / 256;          // staged rhs values per thread
+      constexpr int KY = (D * kChunk + 255) / 256;  // staged coordinates per thread
+      static_assert(kChunk * NC % 256 == 0, "staging layout");
+      T py[KY], px[KX];
       int nb = n0, j0 = 0, pnj = 0;
       auto fetch = [&]() {
         const int s = pat.near_src[nb];
         const int sb = cols.begin[s], ss = cols.size[s];
         pnj = min(kChunk, ss - j0);
-        if (threadIdx.x < D * kChunk) {
-          const int d = threadIdx.x / kChunk, jj = threadIdx.x % kChunk;
-          py = jj < pnj ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj] : T(0);
+#pragma unroll
+        for (int k = 0; k < KY; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          const int d = e / kChunk, jj = e % kChunk;
+          py[k] = (e < D * kChunk && jj < pnj)
+                      ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj]
+                      : T(0);
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
@@ -214,7 +221,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
       while (more) {
         const int nj = pnj;
         __syncthreads();
-        if (threadIdx.x < D * kChunk) Ys[threadIdx.x] = py;
+#pragma unroll
+        for (int k = 0; k < KY; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          if (e < D * kChunk) Ys[e] = py[k];
+        }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
           const int e = threadIdx.x + 256 * k;
