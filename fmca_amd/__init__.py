@@ -85,22 +85,33 @@ def _points(P, name):
     return np.asfortranarray(P)
 
 
+INTERPOLATORS = {"tensor_product": _C.INTERP_TENSOR_PRODUCT,
+                 "total_degree": _C.INTERP_TOTAL_DEGREE}
+
+
 class H2kernelActor:
     """H2kernelActor(kernel, Peval, P, mpole_deg, eta) -> .action(rhs).
 
     Peval: (dim, m) targets, P: (dim, n) sources.  ``action`` takes an
     (n, nrhs) float64 array and returns the (m, nrhs) product in natural order.
+    ``interpolator`` is the Interpolator of NystromMoments<Interpolator>
+    (H2kernelActor.h:17-36): "tensor_product" (TensorProductInterpolator) or
+    "total_degree" (TotalDegreeInterpolator, as in FMCA's own actor test,
+    tests/test_H2kernelActor.cpp:20-24).
     """
 
     def __init__(self, kernel: CovarianceKernel, Peval, P, mpole_deg: int = 3,
-                 eta: float = 0.8):
+                 eta: float = 0.8, interpolator: str = "tensor_product"):
         Peval = _points(Peval, "Peval")
         P = _points(P, "P")
+        if interpolator not in INTERPOLATORS:
+            raise ValueError(f"interpolator must be one of {sorted(INTERPOLATORS)}")
         self.kernel = kernel
         self.deg = int(mpole_deg)
         self.eta = float(eta)
+        self.interpolator = interpolator
         self._a = _C.Actor(kernel.kernelType(), kernel.l, kernel.c, kernel.nu, Peval, P,
-                           self.deg, self.eta)
+                           self.deg, self.eta, INTERPOLATORS[interpolator])
         self.m = self._a.m
         self.n = self._a.n
         self.dim = self._a.dim
@@ -212,6 +223,14 @@ class H2kernelActor:
         r, c, lr = self._a.blocks()
         return r, c, lr.astype(bool)
 
+    def interpolation(self) -> dict:
+        """The interpolator's data: q, multi_indices (q, dim) in FMCA's order,
+        Xi (dim, q) nodes on [0,1]^dim and invV (q, q); see
+        fmca_h2_interpolation."""
+        d = self._a.interpolation()
+        d["interpolator"] = {v: k for k, v in INTERPOLATORS.items()}[d["interpolator"]]
+        return d
+
     def launch_order(self, which: str) -> np.ndarray:
         """(workgroups, 4): target (cluster id for "coupling", leaf index for
         "leaf_output"), list begin, list end, flags (1 split, 2 far field) of
@@ -223,11 +242,14 @@ class H2Matrix(H2kernelActor):
     """FMCA.H2Matrix(CovarianceKernel, Pr, Pc, p=3, eta=0.8) (py/FMCA.cpp:209-250).
 
     ``action(rhs)`` permutes into the column tree, applies H2MatrixBase::action
-    and permutes back, i.e. it returns K(Pr, Pc) @ rhs in natural order.
+    and permutes back, i.e. it returns K(Pr, Pc) @ rhs in natural order.  As in
+    FMCA's binding (py/FMCA.cpp:37-38) the moments are total-degree Legendre
+    moments (TotalDegreeInterpolator) unless ``interpolator`` says otherwise.
     """
 
-    def __init__(self, cov: CovarianceKernel, Pr, Pc, p: int = 3, eta: float = 0.8):
-        super().__init__(cov, Pr, Pc, p, eta)
+    def __init__(self, cov: CovarianceKernel, Pr, Pc, p: int = 3, eta: float = 0.8,
+                 interpolator: str = "total_degree"):
+        super().__init__(cov, Pr, Pc, p, eta, interpolator)
 
 
 def _as_device_matrix(t, rows):
@@ -253,4 +275,4 @@ def _check_out(out, rows, nrhs):
 
 
 __all__ = ["CovarianceKernel", "H2Matrix", "H2kernelActor", "FP64", "FP32", "ROWS", "COLS",
-           "KERNEL_TYPES"]
+           "KERNEL_TYPES", "INTERPOLATORS"]

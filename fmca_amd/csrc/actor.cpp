@@ -170,6 +170,8 @@ struct fmca_h2_actor {
   DevTreeReals<double> d_rows_64, d_cols_64;
   DevTreeReals<float> d_rows_32, d_cols_32;
   DevPattern d_pat, d_pat_t;
+  // total-degree interpolation data: multi-indices, unit nodes (dim-major), invV
+  DevBuf d_midx, d_unodes64, d_unodes32, d_invV64, d_invV32;
   Workspace<double> ws64;
   Workspace<float> ws32;
   DevBuf stage_in, stage_out;
@@ -295,18 +297,37 @@ fmca_status ensure_device(fmca_h2_actor *a) {
     HIP_TRY(upload_reals<float>(a->d_cols_32, a->cols(), a->pts_cols, a->trans_cols));
   }
   HIP_TRY(a->d_pat.upload(a->pat));
+  if (a->ip.scheme == IS_TOTAL_DEGREE) {
+    const int q = a->ip.q, d = a->dim;
+    std::vector<double> un((size_t)d * q);
+    for (int k = 0; k < q; ++k)
+      for (int i = 0; i < d; ++i) un[(size_t)i * q + k] = a->ip.nodes[(size_t)d * k + i];
+    HIP_TRY(a->d_midx.upload(a->ip.midx));
+    HIP_TRY(a->d_unodes64.upload(un));
+    HIP_TRY(a->d_unodes32.upload(std::vector<float>(un.begin(), un.end())));
+    HIP_TRY(a->d_invV64.upload(a->ip.invV));
+    HIP_TRY(a->d_invV32.upload(std::vector<float>(a->ip.invV.begin(), a->ip.invV.end())));
+  }
   a->uploaded = true;
   return FMCA_OK;
 }
 
 template <typename T>
-InterpParams<T> interp_params(const Interpolation &ip) {
+InterpParams<T> interp_params(const fmca_h2_actor *a) {
+  const Interpolation &ip = a->ip;
   InterpParams<T> p;
   p.P1 = ip.P1;
   p.q = ip.q;
   for (int k = 0; k < kMaxP1; ++k) {
-    p.xi[k] = k < ip.P1 ? (T)ip.xi[k] : T(0);
-    p.w[k] = k < ip.P1 ? (T)ip.w[k] : T(0);
+    p.xi[k] = k < (int)ip.xi.size() ? (T)ip.xi[k] : T(0);
+    p.w[k] = k < (int)ip.w.size() ? (T)ip.w[k] : T(0);
+  }
+  if (ip.scheme == IS_TOTAL_DEGREE) {
+    const bool f64 = sizeof(T) == sizeof(double);
+    p.td = 1;
+    p.midx = a->d_midx.as<int32_t>();
+    p.unodes = (f64 ? a->d_unodes64 : a->d_unodes32).as<T>();
+    p.invV = (f64 ? a->d_invV64 : a->d_invV32).as<T>();
   }
   return p;
 }
@@ -401,7 +422,7 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
     pe = &a->pending.back();
   }
   hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
-                                  interp_params<T>(a->ip), kernel_params<T>(a, groups), d_rhs, ld_rhs,
+                                  interp_params<T>(a), kernel_params<T>(a, groups), d_rhs, ld_rhs,
                                   (int)nrhs, d_out, ld_out, natural, buf, stream, pe);
   return from_hip(e, "launch_action");
 }
@@ -432,8 +453,9 @@ template <typename T>
 bool groups_in_one_action(const fmca_h2_actor *a, const VariantGroups &g, int64_t nrhs) {
   if (g.uniform()) return true;
   KernelParams<T> kp = kernel_params<T>(a, g);
-  return mfma_supported<T>(true, a->dim, a->ip.P1, a->ip.q, (int)nrhs, kp.kt, kp.vmask) &&
-         mfma_supported<T>(false, a->dim, a->ip.P1, a->ip.q, (int)nrhs, kp.kt, kp.vmask);
+  const int td = a->ip.scheme == IS_TOTAL_DEGREE;
+  return mfma_supported<T>(true, a->dim, a->ip.P1, a->ip.q, (int)nrhs, kp.kt, kp.vmask, td) &&
+         mfma_supported<T>(false, a->dim, a->ip.P1, a->ip.q, (int)nrhs, kp.kt, kp.vmask, td);
 }
 
 fmca_status host_action(fmca_h2_actor *a, const double *rhs, int64_t ld_rhs, int64_t nrhs,
@@ -493,6 +515,13 @@ const char *fmca_last_error(void) { return g_last_error.c_str(); }
 fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
                            const double *Peval, int64_t n, const double *P, int deg, double eta,
                            fmca_h2_actor **actor) {
+  return fmca_h2_create_interp(kernel, dim, m, Peval, n, P, deg, eta,
+                               FMCA_INTERP_TENSOR_PRODUCT, actor);
+}
+
+fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64_t m,
+                                  const double *Peval, int64_t n, const double *P, int deg,
+                                  double eta, int interpolator, fmca_h2_actor **actor) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
   *actor = nullptr;
   if (!kernel || !kernel->ktype) return fail(FMCA_ERR_INVALID_ARGUMENT, "kernel is null");
@@ -501,6 +530,8 @@ fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
   if (dim < 1 || dim > kMaxDim)
     return fail(FMCA_ERR_NOT_SUPPORTED, "dimension must be 1, 2 or 3");
   if (deg < 0 || deg + 1 > kMaxP1) return fail(FMCA_ERR_NOT_SUPPORTED, "degree must be 0..9");
+  if (interpolator != FMCA_INTERP_TENSOR_PRODUCT && interpolator != FMCA_INTERP_TOTAL_DEGREE)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown interpolator");
   if (m < 1 || n < 1 || !P || !Peval) return fail(FMCA_ERR_INVALID_ARGUMENT, "empty point set");
   if (n > (int64_t)INT32_MAX || m > (int64_t)INT32_MAX)
     return fail(FMCA_ERR_NOT_SUPPORTED, "more than 2^31-1 points");
@@ -523,17 +554,24 @@ fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
     a->l = kernel->l;
     a->c = kernel->c;
     a->nu = kernel->nu;
-    a->ip.init(dim, deg);
+    a->ip.init(dim, deg,
+               interpolator == FMCA_INTERP_TOTAL_DEGREE ? IS_TOTAL_DEGREE : IS_TENSOR);
     // H2ClusterTree::init (H2ClusterTree.h:65-74): minimum cluster size q
     a->shared = (P == Peval && m == n) ||
                 (m == n && std::memcmp(P, Peval, sizeof(double) * dim * n) == 0);
     build_tree(a->rows_tree, dim, m, Peval, a->ip.q);
     if (!a->shared) build_tree(a->cols_tree, dim, n, P, a->ip.q);
     compute_pattern(a->pat, a->rows(), a->cols(), eta, a->ip.q);
-    compute_transfers(a->trans_rows, a->rows(), a->ip);
+    auto transfers = [&](std::vector<double> &T, const Tree &tr) {
+      if (a->ip.scheme == IS_TOTAL_DEGREE)
+        compute_transfers_dense(T, tr, a->ip);
+      else
+        compute_transfers(T, tr, a->ip);
+    };
+    transfers(a->trans_rows, a->rows());
     a->pts_rows = cluster_order_points(a->rows(), Peval);
     if (!a->shared) {
-      compute_transfers(a->trans_cols, a->cols(), a->ip);
+      transfers(a->trans_cols, a->cols());
       a->pts_cols = cluster_order_points(a->cols(), P);
     }
   } catch (const std::bad_alloc &) {
@@ -709,6 +747,39 @@ fmca_status fmca_h2_info(const fmca_h2_actor *actor, double out[13]) {
   out[10] = actor->rows().depth;
   out[11] = actor->cols().depth;
   out[12] = actor->shared ? 1 : 0;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_interpolation(const fmca_h2_actor *actor, int *interpolator, int *q,
+                                  int32_t *multi_indices, double *nodes, double *invV) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  const Interpolation &ip = actor->ip;
+  const int d = ip.dim, nq = ip.q;
+  const bool td = ip.scheme == IS_TOTAL_DEGREE;
+  if (interpolator) *interpolator = td ? FMCA_INTERP_TOTAL_DEGREE : FMCA_INTERP_TENSOR_PRODUCT;
+  if (q) *q = nq;
+  for (int k = 0; k < nq; ++k) {
+    // FMCA position k -> packed multi-index
+    int pk = 0;
+    if (td) {
+      pk = ip.midx[k];
+    } else {
+      int lin = ip.fmca_order[k];
+      for (int i = 0; i < d; ++i) {
+        pk |= (lin % ip.P1) << (8 * i);
+        lin /= ip.P1;
+      }
+    }
+    for (int i = 0; i < d; ++i) {
+      const int ai = (pk >> (8 * i)) & 0xff;
+      if (multi_indices) multi_indices[(size_t)d * k + i] = ai;
+      if (nodes) nodes[(size_t)d * k + i] = td ? ip.nodes[(size_t)d * k + i] : ip.xi[ai];
+    }
+  }
+  if (invV)
+    for (int j = 0; j < nq; ++j)
+      for (int i = 0; i < nq; ++i)
+        invV[i + (size_t)nq * j] = td ? ip.invV[i + (size_t)nq * j] : (i == j ? 1. : 0.);
   return FMCA_OK;
 }
 

@@ -28,7 +28,8 @@ struct DevTreeView {
   const int32_t *son1 = nullptr;    // ncl
   const int32_t *parent = nullptr;  // ncl
   const int32_t *leaves = nullptr;  // nleaves
-  const T *transfer = nullptr;      // ncl x dim x P1 x P1, [a][b] = l_a(x(b))
+  const T *transfer = nullptr;      // tensor: ncl x dim x P1 x P1, [a][b] = l_a(x(b));
+                                    // total degree: ncl x q x q, [a][b] = E_c(a, b)
 };
 
 // Interaction lists of one (target tree, source tree) pair.
@@ -52,11 +53,20 @@ struct DevPatternView {
   const int32_t *near_split = nullptr;  // target leaf indices with several tasks
 };
 
+// Interpolation data of the kernels.  Tensor scheme: the per-axis Chebyshev
+// nodes xi and weights w.  Total-degree scheme (td = 1): the packed
+// multi-index of every basis function (midx, a_0 | a_1 << 8 | a_2 << 16), the
+// unit interpolation nodes unodes[d * q + k] and invV (q x q, column-major),
+// all in device memory.
 template <typename T>
 struct InterpParams {
   int P1 = 0, q = 0;
+  int td = 0;
   T xi[kMaxP1];
   T w[kMaxP1];
+  const int32_t *midx = nullptr;
+  const T *unodes = nullptr;
+  const T *invV = nullptr;
 };
 
 // Kernel parameters with the radial scale folded in on the host, so that no
@@ -130,13 +140,14 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          hipStream_t stream, const PhaseEvents *events = nullptr);
 
 // Matrix-core (MFMA) forms of the coupling (output = false) and the leaf
-// output (output = true), used from kMfmaMinRhs right-hand sides on.
+// output (output = true), used from kMfmaMinRhs right-hand sides on when
+// their LDS fits (td = 1: total-degree scheme).
 constexpr int kMfmaMinRhs = 8;
 
 // kt, vmask: the kernel parameters of the launch (several variants take
 // 64-column passes, also in FP64).
 template <typename T>
-bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask);
+bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask, int td);
 
 template <typename T>
 hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,

@@ -283,5 +283,44 @@ __device__ __forceinline__ void lagrange_1d(T u, const InterpParams<T> &ip, T *o
   }
 }
 
+// L2-normalised shifted Legendre polynomials sqrt(2a+1) P_a(2u-1), a < P1,
+// by the three-term recurrence (evalPolynomials.h:19-40), written with a
+// stride.
+template <typename T>
+__device__ __forceinline__ void legendre_1d(T u, int P1, T *out, int stride) {
+  const T x = 2 * u - 1;
+  T prev = 0, cur = 1;
+  out[0] = T(1);
+  for (int a = 1; a < P1; ++a) {
+    const T next = T(2 * a - 1) / T(a) * x * cur - T(a - 1) / T(a) * prev;
+    prev = cur;
+    cur = next;
+    out[a * stride] = next * sqrt(T(2 * a + 1));
+  }
+}
+
+// 1-D factors of the leaf basis at unit coordinate u: Lagrange factors
+// (tensor scheme) or Legendre polynomials (total-degree scheme).
+template <typename T>
+__device__ __forceinline__ void basis_1d(T u, const InterpParams<T> &ip, T *out, int stride) {
+  if (ip.td)
+    legendre_1d<T>(u, ip.P1, out, stride);
+  else
+    lagrange_1d<T>(u, ip, out, stride);
+}
+
+// Packed multi-index a_0 | a_1 << 8 | a_2 << 16 of basis function alpha.
+template <typename T, int D>
+__device__ __forceinline__ int basis_index(int alpha, const InterpParams<T> &ip) {
+  if (ip.td) return ip.midx[alpha];
+  int pk = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    pk |= (alpha % ip.P1) << (8 * d);
+    alpha /= ip.P1;
+  }
+  return pk;
+}
+
 }  // namespace dev
 }  // namespace fmca_amd

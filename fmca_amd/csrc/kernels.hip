@@ -88,8 +88,9 @@ __global__ __launch_bounds__(kBlock) void k_zero_leaf_rows(DevTreeView<T> rows,
 
 // ---------------------------------------------------------------------------
 // 2. leaf moments: mom[t][alpha][r] = sum_p V_t[alpha][p] xs[p][r]
-// One workgroup per leaf; the Lagrange factors of 64 points at a time are
-// staged in LDS and the tensor-product basis value is formed on the fly.
+// One workgroup per leaf; the 1-D factors (Lagrange or Legendre) of 64 points
+// at a time are staged in LDS and the basis value, a product of D factors, is
+// formed on the fly.
 template <typename T, int D, int NR>
 __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, InterpParams<T> ip,
                                                         const T *__restrict__ xs, int nrhs,
@@ -106,12 +107,9 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
       const int alpha = abase + threadIdx.x;
       int a[D];
       {
-        int rem = alpha;
+        const int pk = basis_index<T, D>(alpha < q ? alpha : 0, ip);
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-          a[i] = rem % P1;
-          rem /= P1;
-        }
+        for (int i = 0; i < D; ++i) a[i] = (pk >> (8 * i)) & 0xff;
       }
       T acc[NR];
 #pragma unroll
@@ -122,8 +120,8 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
         for (int e = threadIdx.x; e < D * 64; e += kBlock) {
           const int i = e / 64, jj = e % 64;
           if (jj < np)
-            lagrange_1d<T>(tr.unit[(int64_t)i * tr.npts + b0 + p0 + jj], ip,
-                           F + (i * P1) * 64 + jj, 64);
+            basis_1d<T>(tr.unit[(int64_t)i * tr.npts + b0 + p0 + jj], ip, F + (i * P1) * 64 + jj,
+                        64);
         }
         for (int e = threadIdx.x; e < 64 * NR; e += kBlock) {
           const int jj = e / NR, r = e % NR;
@@ -249,8 +247,10 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
                                                      T *__restrict__ tlhs) {
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
-  T *Y = reinterpret_cast<T *>(smem_raw);  // [D][P1] source nodes
-  T *Mv = Y + D * kMaxP1;                  // [q][NR] source moments
+  // source nodes: [D][kMaxP1] per-axis values (tensor) or [D][q] (total degree)
+  const int ys = ip.td ? q : kMaxP1;
+  T *Y = reinterpret_cast<T *>(smem_raw);
+  T *Mv = Y + D * ys;  // [q][NR] source moments
   const int task = pat.far_order[blockIdx.x];
   if (task < 0) return;
   const int t = pat.far_task[4 * task], f0 = pat.far_task[4 * task + 1],
@@ -262,12 +262,12 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
       const int alpha = abase + threadIdx.x;
       T x[D];
       {
-        int rem = alpha < q ? alpha : 0;
+        const int al = alpha < q ? alpha : 0;
+        const int pk = basis_index<T, D>(al, ip);
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-          const int ai = rem % P1;
-          rem /= P1;
-          x[i] = ip.xi[ai] * rows.bbsize[t * D + i] + rows.bbmin[t * D + i];
+          const T u = ip.td ? ip.unodes[i * q + al] : ip.xi[(pk >> (8 * i)) & 0xff];
+          x[i] = u * rows.bbsize[t * D + i] + rows.bbmin[t * D + i];
         }
       }
       T acc[NR];
@@ -276,16 +276,36 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
       for (int f = f0; f < f1; ++f) {
         const int s = pat.far_src[f];
         __syncthreads();
-        for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
-          const int i = e / P1, b = e % P1;
-          Y[i * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + i] + cols.bbmin[s * D + i];
+        if (ip.td) {
+          for (int e = threadIdx.x; e < D * q; e += blockDim.x) {
+            const int i = e / q;
+            Y[e] = ip.unodes[e] * cols.bbsize[s * D + i] + cols.bbmin[s * D + i];
+          }
+        } else {
+          for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
+            const int i = e / P1, b = e % P1;
+            Y[i * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + i] + cols.bbmin[s * D + i];
+          }
         }
         for (int e = threadIdx.x; e < q * NR; e += blockDim.x) {
           const int beta = e / NR, r = e % NR;
           Mv[e] = r < nr ? mom[((int64_t)s * q + beta) * nrhs + rc + r] : T(0);
         }
         __syncthreads();
-        if (alpha < q) {
+        if (alpha < q && ip.td) {
+          // total degree: nodes without tensor structure
+          for (int beta = 0; beta < q; ++beta) {
+            T r2 = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+              const T di = x[i] - Y[i * q + beta];
+              r2 = fmat(di, di, r2);
+            }
+            const T kv = kernel_r2<KT, T>(r2, kp);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
+          }
+        } else if (alpha < q) {
           // tensor loop over the source nodes: the squared distance is
           // accumulated dimension by dimension, one fma per entry innermost
           int beta = 0;
@@ -385,6 +405,107 @@ __global__ __launch_bounds__(kBlock) void k_downward_transfer(DevTreeView<T> tr,
 }
 
 // ---------------------------------------------------------------------------
+// Total-degree scheme (TotalDegreeInterpolator).  The transfers are dense
+// q x q matrices per son, E_c[a * q + b] (compute_cluster_bases_impl.h:36-51),
+// and the coupling matrices invV K(xi_t, xi_s) invV^T
+// (unsymmetricNystromEvaluator.h:56) are applied as
+//   tlhs[t] = invV (sum_s K(xi_t, xi_s) (invV^T mom[s])),
+// i.e. the source moments are changed to the nodal basis once per cluster
+// (k_cluster_basis_change, transposed) before the coupling and the sums back
+// to the polynomial basis once per cluster after it.
+
+// 3'. upward: mom[t] = E_s0 mom[s0] + E_s1 mom[s1] for the inner clusters of
+// one level; the sons' moments staged [2][q][NRC] in LDS, E read from L2.
+template <typename T, int NRC>
+__global__ __launch_bounds__(kBlock) void k_upward_transfer_dense(DevTreeView<T> tr, int q,
+                                                                  int cbeg, int nrhs,
+                                                                  T *__restrict__ mom) {
+  extern __shared__ unsigned char smem_raw[];
+  T *S = reinterpret_cast<T *>(smem_raw);  // [2][q][NRC]
+  const int c = cbeg + blockIdx.x;
+  const int s0 = tr.son0[c], s1 = tr.son1[c];
+  if (s0 < 0) return;
+  const T *E0 = tr.transfer + (int64_t)s0 * q * q;
+  const T *E1 = tr.transfer + (int64_t)s1 * q * q;
+  for (int rc = 0; rc < nrhs; rc += NRC) {
+    const int nr = min(NRC, nrhs - rc);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * q * NRC; e += blockDim.x) {
+      const int si = e / (q * NRC), b = (e / NRC) % q, r = e % NRC;
+      S[e] = r < nr ? mom[((int64_t)(si ? s1 : s0) * q + b) * nrhs + rc + r] : T(0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int a = e / NRC, r = e % NRC;
+      if (r >= nr) continue;
+      T acc = 0;
+      for (int b = 0; b < q; ++b) acc += E0[a * q + b] * S[b * NRC + r];
+      for (int b = 0; b < q; ++b) acc += E1[a * q + b] * S[(q + b) * NRC + r];
+      mom[((int64_t)c * q + a) * nrhs + rc + r] = acc;
+    }
+  }
+}
+
+// 5'. downward: tlhs[c] += E_c^T tlhs[parent(c)] for the clusters of one
+// level >= 1, the father's coefficients staged [q][NRC] in LDS.
+template <typename T, int NRC>
+__global__ __launch_bounds__(kBlock) void k_downward_transfer_dense(DevTreeView<T> tr, int q,
+                                                                    int cbeg, int nrhs,
+                                                                    T *__restrict__ tlhs) {
+  extern __shared__ unsigned char smem_raw[];
+  T *S = reinterpret_cast<T *>(smem_raw);  // [q][NRC]
+  const int c = cbeg + blockIdx.x;
+  const int p = tr.parent[c];
+  const T *E = tr.transfer + (int64_t)c * q * q;
+  for (int rc = 0; rc < nrhs; rc += NRC) {
+    const int nr = min(NRC, nrhs - rc);
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int a = e / NRC, r = e % NRC;
+      S[e] = r < nr ? tlhs[((int64_t)p * q + a) * nrhs + rc + r] : T(0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int b = e / NRC, r = e % NRC;
+      if (r >= nr) continue;
+      T acc = 0;
+      for (int a = 0; a < q; ++a) acc += E[a * q + b] * S[a * NRC + r];
+      tlhs[((int64_t)c * q + b) * nrhs + rc + r] += acc;
+    }
+  }
+}
+
+// Basis change of every cluster's q x nrhs block, in place:
+//   transpose: x[c] <- invV^T x[c]  (polynomial moments -> nodal moments)
+//   else:      x[c] <- invV x[c]    (nodal sums -> polynomial coefficients)
+// invV is q x q column-major.  One workgroup per cluster.
+template <typename T, int NRC>
+__global__ __launch_bounds__(kBlock) void k_cluster_basis_change(int q, const T *__restrict__ invV,
+                                                                 int transpose, int nrhs,
+                                                                 T *__restrict__ x) {
+  extern __shared__ unsigned char smem_raw[];
+  T *S = reinterpret_cast<T *>(smem_raw);  // [q][NRC]
+  const int c = blockIdx.x;
+  const int64_t ia = transpose ? q : 1, ib = transpose ? 1 : q;  // M(a, b) = invV[a ia + b ib]
+  for (int rc = 0; rc < nrhs; rc += NRC) {
+    const int nr = min(NRC, nrhs - rc);
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int b = e / NRC, r = e % NRC;
+      S[e] = r < nr ? x[((int64_t)c * q + b) * nrhs + rc + r] : T(0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
+      const int a = e / NRC, r = e % NRC;
+      if (r >= nr) continue;
+      T acc = 0;
+      for (int b = 0; b < q; ++b) acc += invV[a * ia + b * ib] * S[b * NRC + r];
+      x[((int64_t)c * q + a) * nrhs + rc + r] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 6. leaf output: y_t = V_t^T tlhs[t] + sum_{s in near(t)} K(P_t, P_s) xs_s,
 // written to out (natural order through perm, or cluster order).
 // One workgroup per target leaf, thread <-> target point.  LDS holds the
@@ -440,8 +561,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
       if (act && do_far) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
-          lagrange_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
-                         F + (d * P1) * kBlock + threadIdx.x, kBlock);
+          basis_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
+                      F + (d * P1) * kBlock + threadIdx.x, kBlock);
       }
       int a[D];
 #pragma unroll
@@ -456,6 +577,11 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
         __syncthreads();
         if (act) {
           for (int al = 0; al < na; ++al) {
+            if (ip.td) {
+              const int pk = ip.midx[at + al];
+#pragma unroll
+              for (int d = 0; d < D; ++d) a[d] = (pk >> (8 * d)) & 0xff;
+            }
             T v = F[(0 * P1 + a[0]) * kBlock + threadIdx.x];
 #pragma unroll
             for (int d = 1; d < D; ++d) v *= F[(d * P1 + a[d]) * kBlock + threadIdx.x];
@@ -523,8 +649,8 @@ size_t output_lds(int D, int P1) {
 }
 
 template <typename T, int NR>
-size_t coupling_lds(int D, int q) {
-  return ((size_t)D * kMaxP1 + (size_t)q * NR) * sizeof(T);
+size_t coupling_lds(int D, int q, bool td) {
+  return ((size_t)D * (td ? q : kMaxP1) + (size_t)q * NR) * sizeof(T);
 }
 
 template <typename T, int D, int KT, int NR>
@@ -534,7 +660,7 @@ hipError_t launch_far_near(const DevTreeView<T> &rows, const DevTreeView<T> &col
                            double *out, int64_t ldo, bool natural, hipStream_t stream) {
   const int q = ip.q;
   const int threads = q <= 64 ? 64 : (q <= 128 ? 128 : kBlock);
-  const size_t lds = coupling_lds<T, NR>(D, q);
+  const size_t lds = coupling_lds<T, NR>(D, q, ip.td != 0);
   hipLaunchKernelGGL((k_coupling<T, D, KT, NR>), dim3(pat.nfar_order), dim3(threads), lds, stream, rows,
                      cols, pat, ip, kp, nrhs, buf.mom, buf.tlhs);
   return hipGetLastError();
@@ -581,14 +707,14 @@ constexpr size_t kLdsLimit = 64 * 1024;
 
 // Largest rhs tile (32, 8 or 1) whose LDS footprint fits one workgroup.
 template <typename T>
-int pick_nr(bool output, int D, int P1, int q, int nrhs) {
+int pick_nr(bool output, int D, int P1, int q, int nrhs, bool td) {
   auto fits = [&](int nr) {
     const size_t b = output ? (nr == 32   ? output_lds<T, 32>(D, P1)
                                : nr == 8 ? output_lds<T, 8>(D, P1)
                                          : output_lds<T, 1>(D, P1))
-                            : (nr == 32   ? coupling_lds<T, 32>(D, q)
-                               : nr == 8 ? coupling_lds<T, 8>(D, q)
-                                         : coupling_lds<T, 1>(D, q));
+                            : (nr == 32   ? coupling_lds<T, 32>(D, q, td)
+                               : nr == 8 ? coupling_lds<T, 8>(D, q, td)
+                                         : coupling_lds<T, 1>(D, q, td));
     return b <= kLdsLimit;
   };
   if (nrhs > 8 && fits(32)) return 32;
@@ -601,13 +727,14 @@ hipError_t dispatch_nr(bool output, const DevTreeView<T> &rows, const DevTreeVie
                        const DevPatternView &pat, const InterpParams<T> &ip,
                        const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
                        int64_t ldo, bool natural, hipStream_t stream) {
-  if (mfma_supported<T>(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask))
+  if (mfma_supported<T>(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask, ip.td))
     return launch_mfma<T>(output, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.mom, buf.tlhs, out,
                           ldo, natural, stream);
   // the VALU forms evaluate one variant per launch (kp.kt)
   if ((kp.vmask & (kp.vmask - 1)) != 0) return hipErrorInvalidValue;
-  const int nr = pick_nr<T>(output, D, ip.P1, ip.q, nrhs);
-  if (output ? output_lds<T, 1>(D, ip.P1) > kLdsLimit : coupling_lds<T, 1>(D, ip.q) > kLdsLimit)
+  const bool td = ip.td != 0;
+  const int nr = pick_nr<T>(output, D, ip.P1, ip.q, nrhs, td);
+  if (output ? output_lds<T, 1>(D, ip.P1) > kLdsLimit : coupling_lds<T, 1>(D, ip.q, td) > kLdsLimit)
     return hipErrorInvalidConfiguration;
   if (nr == 1)
     return dispatch_kt<T, D, 1>(output, rows, cols, pat, ip, kp, nrhs, buf, out, ldo, natural,
@@ -667,10 +794,45 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
     if (nrhs > 1 && transfer_lds(arrays, 8) <= kLdsLimit) return 8;
     return 1;
   };
+  // total degree: dense transfers and basis changes, [arrays][q][nrc] in LDS
+  auto dense_nrc = [&](int arrays) {
+    if (nrhs > 8 && (size_t)arrays * q * 32 * sizeof(T) <= kLdsLimit) return 32;
+    if (nrhs > 1 && (size_t)arrays * q * 8 * sizeof(T) <= kLdsLimit) return 8;
+    return 1;
+  };
+  auto dense_lds = [&](int arrays, int nrc) { return (size_t)arrays * q * nrc * sizeof(T); };
+  if (ip.td) {
+    const int nu = dense_nrc(2);
+    if (dense_lds(2, nu) > kLdsLimit) return hipErrorInvalidConfiguration;
+    for (int L = lcols.depth - 1; L >= 0; --L) {
+      const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
+      if (nu == 32)
+        hipLaunchKernelGGL((k_upward_transfer_dense<T, 32>), dim3(nc), dim3(kBlock),
+                           dense_lds(2, 32), stream, cols, q, cb, nrhs, buf.mom);
+      else if (nu == 8)
+        hipLaunchKernelGGL((k_upward_transfer_dense<T, 8>), dim3(nc), dim3(kBlock),
+                           dense_lds(2, 8), stream, cols, q, cb, nrhs, buf.mom);
+      else
+        hipLaunchKernelGGL((k_upward_transfer_dense<T, 1>), dim3(nc), dim3(kBlock),
+                           dense_lds(2, 1), stream, cols, q, cb, nrhs, buf.mom);
+    }
+  }
+  auto basis_change = [&](int ncl, int transpose, T *x) {
+    const int nb = dense_nrc(1);
+    if (nb == 32)
+      hipLaunchKernelGGL((k_cluster_basis_change<T, 32>), dim3(ncl), dim3(kBlock),
+                         dense_lds(1, 32), stream, q, ip.invV, transpose, nrhs, x);
+    else if (nb == 8)
+      hipLaunchKernelGGL((k_cluster_basis_change<T, 8>), dim3(ncl), dim3(kBlock),
+                         dense_lds(1, 8), stream, q, ip.invV, transpose, nrhs, x);
+    else
+      hipLaunchKernelGGL((k_cluster_basis_change<T, 1>), dim3(ncl), dim3(kBlock),
+                         dense_lds(1, 1), stream, q, ip.invV, transpose, nrhs, x);
+  };
   const int nrc_up = transfer_nrc(3), nrc_dn = transfer_nrc(2);
-  if (transfer_lds(3, nrc_up) > kLdsLimit || transfer_lds(2, nrc_dn) > kLdsLimit)
+  if (!ip.td && (transfer_lds(3, nrc_up) > kLdsLimit || transfer_lds(2, nrc_dn) > kLdsLimit))
     return hipErrorInvalidConfiguration;
-  for (int L = lcols.depth - 1; L >= 0; --L) {
+  for (int L = lcols.depth - 1; !ip.td && L >= 0; --L) {
     const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
     const size_t lds = transfer_lds(3, nrc_up);
     if (nrc_up == 32)
@@ -686,15 +848,32 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 4. coupling (every target cluster is written, empty far lists give zeros)
   mark(PH_COUPLING);
+  if (ip.td) basis_change(cols.ncl, 1, buf.mom);
   if (pat.nfar_split > 0)
     hipLaunchKernelGGL((k_zero_clusters<T>), dim3(pat.nfar_split), dim3(kBlock), 0, stream,
                        pat.far_split, (int64_t)q * nrhs, buf.tlhs);
   if ((err = dispatch_nr<T, D>(false, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
                                stream)) != hipSuccess)
     return err;
+  if (ip.td) basis_change(rows.ncl, 0, buf.tlhs);
   // 5. downward transfers, top level first
   mark(PH_DOWNWARD_TRANSFER);
-  for (int L = 1; L <= lrows.depth; ++L) {
+  if (ip.td) {
+    const int nd = dense_nrc(1);
+    for (int L = 1; L <= lrows.depth; ++L) {
+      const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
+      if (nd == 32)
+        hipLaunchKernelGGL((k_downward_transfer_dense<T, 32>), dim3(nc), dim3(kBlock),
+                           dense_lds(1, 32), stream, rows, q, cb, nrhs, buf.tlhs);
+      else if (nd == 8)
+        hipLaunchKernelGGL((k_downward_transfer_dense<T, 8>), dim3(nc), dim3(kBlock),
+                           dense_lds(1, 8), stream, rows, q, cb, nrhs, buf.tlhs);
+      else
+        hipLaunchKernelGGL((k_downward_transfer_dense<T, 1>), dim3(nc), dim3(kBlock),
+                           dense_lds(1, 1), stream, rows, q, cb, nrhs, buf.tlhs);
+    }
+  }
+  for (int L = 1; !ip.td && L <= lrows.depth; ++L) {
     const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
     const size_t lds = transfer_lds(2, nrc_dn);
     if (nrc_dn == 32)

@@ -104,25 +104,18 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
   const bool split = (flags & kTaskSplit) != 0, do_far = (flags & kTaskFar) != 0;
   const int c = rows.leaves[li];
   const int b0 = rows.begin[c], sz = rows.size[c];
-  for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
-    int rem = e < q ? e : 0, pk = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      pk |= (rem % P1) << (8 * d);
-      rem /= P1;
-    }
-    aidx[e] = pk;
-  }
+  for (int e = threadIdx.x; e < qpad; e += blockDim.x)
+    aidx[e] = basis_index<T, D>(e < q ? e : 0, ip);
   for (int pbase = 0; pbase < sz; pbase += kRows) {
-    // Lagrange factors of the rows of this pass (thread <-> row)
+    // 1-D basis factors of the rows of this pass (thread <-> row)
     __syncthreads();
     {
       const int i = pbase + threadIdx.x;
       if (i < sz) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
-          lagrange_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
-                         F + (d * P1) * kRowsP + threadIdx.x, kRowsP);
+          basis_1d<T>(rows.unit[(int64_t)d * rows.npts + b0 + i], ip,
+                      F + (d * P1) * kRowsP + threadIdx.x, kRowsP);
       } else {
         for (int d = 0; d < D; ++d)
           for (int a = 0; a < P1; ++a) F[(d * P1 + a) * kRowsP + threadIdx.x] = T(0);
@@ -316,9 +309,12 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
+  // source node coordinates: [D][kMaxP1] per-axis values (tensor) or
+  // [D][qpad] per node (total degree)
+  const int ys = ip.td ? qpad : kMaxP1;
   int *bidx = reinterpret_cast<int *>(smem_raw);                     // [qpad]
-  T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][kMaxP1]
-  T *Ms = Y + D * kMaxP1;                                            // [kChunk][NCP]
+  T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][ys]
+  T *Ms = Y + D * ys;                                                // [kChunk][NCP]
   T *Red = Ms + kChunk * NCP;  // [2 kRT * 16][NC] partial sums of the K-split (q <= 128)
   const int lane = threadIdx.x & 63;
   // wave index as a scalar: the row-tile guards below are then uniform branches
@@ -336,14 +332,16 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   const int kspl = R <= kRT ? 4 : (R <= 2 * kRT ? 2 : 1);
   const int wrows = kWaves / kspl, rg = wave % wrows, sg = wave / wrows;
   const int ks0 = 4 * sg, kstep = 4 * kspl;
+  // bidx[beta]: the LDS offsets in Y of source node beta's coordinates,
+  // 10 bits per axis
   for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
-    int rem = e < q ? e : 0, pk = 0;
+    const int be = e < q ? e : 0;
+    const int pk = basis_index<T, D>(be, ip);
+    int off = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      pk |= (rem % P1) << (8 * d);
-      rem /= P1;
-    }
-    bidx[e] = pk;
+    for (int d = 0; d < D; ++d)
+      off |= (d * ys + (ip.td ? be : unpack(pk, d))) << (10 * d);
+    bidx[e] = off;
   }
   // this lane's target node per row tile
   T x[kRT][D];
@@ -352,14 +350,21 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   for (int rt = 0; rt < kRT; ++rt) {
     const int tile_row0 = (rg + wrows * rt) * 16;
     tile_on[rt] = tile_row0 < q;
-    int rem = tile_row0 + lr < q ? tile_row0 + lr : 0;
+    const int row = tile_row0 + lr < q ? tile_row0 + lr : 0;
+    const int pk = basis_index<T, D>(row, ip);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int ad = rem % P1;
-      rem /= P1;
-      x[rt][d] = ip.xi[ad] * rows.bbsize[t * D + d] + rows.bbmin[t * D + d];
+      const T u = ip.td ? ip.unodes[d * q + row] : ip.xi[unpack(pk, d)];
+      x[rt][d] = u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d];
     }
   }
+  // total degree: source s's node coordinates into Y
+  auto stage_td_nodes = [&](int s) {
+    for (int e = threadIdx.x; e < D * q; e += blockDim.x) {
+      const int d = e / q, k = e - d * q;
+      Y[d * ys + k] = ip.unodes[e] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+    }
+  };
   for (int rc = 0; rc < nrhs; rc += NC) {
     acc_t acc[kRT][NT];
 #pragma unroll
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
         const int pk = bidx[be + bl];
         T y[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) y[d] = Y[d * kMaxP1 + unpack(pk, d)];
+        for (int d = 0; d < D; ++d) y[d] = Y[(pk >> (10 * d)) & 1023];
         T b[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NCP + nt * 16 + lr];
@@ -415,10 +420,12 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
       static_assert(kChunk * NC % 256 == 0 && kMaxDim * kMaxP1 <= 256, "staging layout");
       T pyn = T(0), pm[KM];
       int f = f0, nbe = 0, pbe = 0;  // next chunk to fetch: (f, nbe); fetched: pbe
+      int psrc = 0;                  // source of the fetched chunk
       auto fetch = [&]() {
         const int s = pat.far_src[f];
         pbe = nbe;
-        if (nbe == 0 && threadIdx.x < D * P1) {
+        psrc = s;
+        if (nbe == 0 && !ip.td && threadIdx.x < D * P1) {
           const int d = threadIdx.x / P1, b = threadIdx.x % P1;
           pyn = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
         }
@@ -441,8 +448,12 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
       while (more) {
         const int be = pbe;
         __syncthreads();
-        if (be == 0 && threadIdx.x < D * P1)
-          Y[(threadIdx.x / P1) * kMaxP1 + threadIdx.x % P1] = pyn;
+        if (be == 0) {
+          if (ip.td)
+            stage_td_nodes(psrc);
+          else if (threadIdx.x < D * P1)
+            Y[(threadIdx.x / P1) * kMaxP1 + threadIdx.x % P1] = pyn;
+        }
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
           const int e = threadIdx.x + 256 * k;
@@ -459,7 +470,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
         const int s = pat.far_src[f];
         for (int be = 0; be < qpad; be += kChunk) {
           __syncthreads();
-          if (be == 0)
+          if (be == 0 && ip.td) stage_td_nodes(s);
+          if (be == 0 && !ip.td)
             for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
               const int d = e / P1, b = e % P1;
               Y[d * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
@@ -544,12 +556,13 @@ size_t leaf_output_mfma_lds(int D, int P1, int q) {
 }
 
 template <typename T, int NT>
-size_t coupling_mfma_lds(int D, int q) {
+size_t coupling_mfma_lds(int D, int q, bool td) {
   constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
+  const size_t ys = td ? (size_t)qpad : (size_t)kMaxP1;
   const size_t red = q <= 2 * kRT * 16 ? (size_t)2 * kRT * 16 * 16 * NT : 0;  // K-split partials
   return (size_t)((qpad + 3) & ~3) * sizeof(int) +
-         ((size_t)D * kMaxP1 + (size_t)kChunk * (16 * NT + 16) + red) * sizeof(T);
+         ((size_t)D * ys + (size_t)kChunk * (16 * NT + 16) + red) * sizeof(T);
 }
 
 // Specialised entry code for the common kernels, a run-time switch otherwise.
@@ -572,7 +585,7 @@ hipError_t launch_mfma_nt(bool output, const DevTreeView<T> &rows, const DevTree
                           const KernelParams<T> &kp, int nrhs, const T *xs, const T *mom,
                           T *tlhs, double *out, int64_t ldo, bool natural, hipStream_t stream) {
   const size_t lds_out = leaf_output_mfma_lds<T, NT>(D, ip.P1, ip.q);
-  const size_t lds_cpl = coupling_mfma_lds<T, NT>(D, ip.q);
+  const size_t lds_cpl = coupling_mfma_lds<T, NT>(D, ip.q, ip.td != 0);
   const int base = kp.kt % KT_COUNT;
   // MULTI kernels take the base kernel; single-variant kernels take kp.kt
   // (a derivative variant there goes to the run-time switch, K = -1)
@@ -643,15 +656,16 @@ hipError_t launch_mfma_d(bool output, const DevTreeView<T> &rows, const DevTreeV
 }  // namespace
 
 template <typename T>
-bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask) {
-  if (nrhs < kMfmaMinRhs) return false;
+bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask, int td) {
+  if (nrhs < kMfmaMinRhs) return false;  // the VALU forms are faster there
   const size_t limit = 64 * 1024;
   const int nt = mfma_nt<T>(nrhs, mfma_multi(kt, vmask));
   if (output)
     return (nt == 1 ? leaf_output_mfma_lds<T, 1>(D, P1, q)
                     : leaf_output_mfma_lds<T, 2>(D, P1, q)) <= limit;
   if (q > kWaves * kRT * 16) return false;
-  return (nt == 1 ? coupling_mfma_lds<T, 1>(D, q) : coupling_mfma_lds<T, 2>(D, q)) <= limit;
+  return (nt == 1 ? coupling_mfma_lds<T, 1>(D, q, td != 0) : coupling_mfma_lds<T, 2>(D, q, td != 0)) <=
+         limit;
 }
 
 template <typename T>
@@ -673,8 +687,8 @@ hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeVie
   return hipErrorInvalidValue;
 }
 
-template bool mfma_supported<float>(bool, int, int, int, int, int, int);
-template bool mfma_supported<double>(bool, int, int, int, int, int, int);
+template bool mfma_supported<float>(bool, int, int, int, int, int, int, int);
+template bool mfma_supported<double>(bool, int, int, int, int, int, int, int);
 template hipError_t launch_mfma<float>(bool, const DevTreeView<float> &,
                                        const DevTreeView<float> &, const DevPatternView &,
                                        const InterpParams<float> &, const KernelParams<float> &,

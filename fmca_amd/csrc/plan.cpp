@@ -83,10 +83,201 @@ const char *kernel_type_name(int kt) {
 }
 
 // ---------------------------------------------------------------------------
-void Interpolation::init(int d, int p) {
+namespace {
+
+// Point k (k >= 0) of the Halton sequence in the first `dim` prime bases,
+// radical inverses by Horner from the leading digit in long double, as
+// HaltonSet::next forms them (util/HaltonSet.h:37-66).
+void halton_point(int64_t k, int dim, double *out) {
+  static const int kPrimes[kMaxDim] = {2, 3, 5};
+  for (int i = 0; i < dim; ++i) {
+    const int b = kPrimes[i];
+    int dig[64], nd = 0;
+    for (int64_t r = k; r > 0; r /= b) dig[nd++] = (int)(r % b);
+    const long double binv = 1.0L / (long double)b;
+    long double v = nd ? dig[nd - 1] : 0;
+    for (int j = nd - 2; j >= 0; --j) v = binv * v + dig[j];
+    out[i] = (double)(v * binv);
+  }
+}
+
+// Greedy column selection of Householder QR with column pivoting on the
+// rows x cols column-major matrix A (overwritten): step k takes the remaining
+// column of largest downdated norm (lowest index on ties), reflects it onto
+// e_k and downdates the other norms, recomputing a norm once the downdate has
+// cancelled more than half of its digits (the rule of LAPACK's xGEQP3, which
+// Eigen's ColPivHouseholderQR follows).  Returns the original column index of
+// each of the first min(rows, cols) pivots.
+std::vector<int> greedy_pivots(std::vector<double> &A, int rows, int cols) {
+  const int steps = std::min(rows, cols);
+  std::vector<int> idx(cols);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::vector<double> ref(cols), cur(cols);  // last recomputed / downdated norms
+  auto c = [&](int j) { return &A[(size_t)rows * j]; };
+  auto tail_norm = [&](int j, int from) {
+    double s = 0;
+    for (int i = from; i < rows; ++i) s += c(j)[i] * c(j)[i];
+    return std::sqrt(s);
+  };
+  for (int j = 0; j < cols; ++j) ref[j] = cur[j] = tail_norm(j, 0);
+  const double tol = std::sqrt(kEps);
+  for (int k = 0; k < steps; ++k) {
+    const int piv = (int)(std::max_element(cur.begin() + k, cur.end()) - cur.begin());
+    if (piv != k) {
+      for (int i = 0; i < rows; ++i) std::swap(c(k)[i], c(piv)[i]);
+      std::swap(ref[k], ref[piv]);
+      std::swap(cur[k], cur[piv]);
+      std::swap(idx[k], idx[piv]);
+    }
+    double *v = c(k);
+    double s2 = 0;
+    for (int i = k + 1; i < rows; ++i) s2 += v[i] * v[i];
+    double tau = 0;
+    const double alpha = v[k];
+    double beta = alpha;
+    if (s2 > std::numeric_limits<double>::min()) {
+      beta = std::sqrt(alpha * alpha + s2);
+      if (alpha >= 0) beta = -beta;
+      for (int i = k + 1; i < rows; ++i) v[i] /= (alpha - beta);
+      tau = (beta - alpha) / beta;
+    } else {
+      for (int i = k + 1; i < rows; ++i) v[i] = 0;
+    }
+    v[k] = beta;
+    for (int j = k + 1; j < cols; ++j) {
+      double *u = c(j);
+      if (tau != 0) {
+        double dot = u[k];
+        for (int i = k + 1; i < rows; ++i) dot += v[i] * u[i];
+        u[k] -= tau * dot;
+        for (int i = k + 1; i < rows; ++i) u[i] -= tau * v[i] * dot;
+      }
+      if (cur[j] == 0) continue;
+      double t = std::abs(u[k]) / cur[j];
+      t = std::max(0., (1 + t) * (1 - t));
+      const double rr = cur[j] / ref[j];
+      if (t * (rr * rr) <= tol)
+        ref[j] = cur[j] = tail_norm(j, k + 1);
+      else
+        cur[j] *= std::sqrt(t);
+    }
+  }
+  idx.resize(steps);
+  return idx;
+}
+
+// Inverse of the n x n column-major matrix M by LU with partial pivoting.
+std::vector<double> lu_inverse(std::vector<double> M, int n) {
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  auto at = [&](int i, int j) -> double & { return M[i + (size_t)n * j]; };
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::abs(at(i, k)) > std::abs(at(p, k))) p = i;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(at(k, j), at(p, j));
+      std::swap(perm[k], perm[p]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      at(i, k) /= at(k, k);
+      for (int j = k + 1; j < n; ++j) at(i, j) -= at(i, k) * at(k, j);
+    }
+  }
+  std::vector<double> X((size_t)n * n, 0.);
+  for (int col = 0; col < n; ++col) {
+    double *x = &X[(size_t)n * col];
+    for (int i = 0; i < n; ++i) x[i] = perm[i] == col ? 1. : 0.;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < i; ++j) x[i] -= at(i, j) * x[j];
+    for (int i = n - 1; i >= 0; --i) {
+      for (int j = i + 1; j < n; ++j) x[i] -= at(i, j) * x[j];
+      x[i] /= at(i, i);
+    }
+  }
+  return X;
+}
+
+}  // namespace
+
+void Interpolation::legendre_1d(double u, double *out) const {
+  double prev = 0, cur = 1;
+  out[0] = 1;
+  for (int a = 1; a <= deg; ++a) {
+    const double next =
+        double(2 * a - 1) / double(a) * (2 * u - 1) * cur - double(a - 1) / double(a) * prev;
+    prev = cur;
+    cur = next;
+    out[a] = next * std::sqrt(double(2 * a + 1));
+  }
+}
+
+void Interpolation::basis(const double *u, double *out) const {
+  double f[kMaxDim][kMaxP1];
+  for (int i = 0; i < dim; ++i) legendre_1d(u[i], f[i]);
+  for (int k = 0; k < q; ++k) {
+    double v = 1;
+    for (int i = 0; i < dim; ++i) v *= f[i][(midx[k] >> (8 * i)) & 0xff];
+    out[k] = v;
+  }
+}
+
+void Interpolation::init(int d, int p, int sch) {
   dim = d;
   deg = p;
   P1 = p + 1;
+  scheme = sch;
+  midx.clear();
+  nodes.clear();
+  invV.clear();
+  if (sch == IS_TOTAL_DEGREE) {
+    // {alpha : |alpha|_1 <= p} in graded-lexicographic order
+    std::vector<std::vector<int>> mi;
+    std::vector<int> a(d, 0);
+    for (;;) {
+      int sum = 0;
+      for (int v : a) sum += v;
+      if (sum <= p) mi.push_back(a);
+      int i = 0;
+      while (i < d && a[i] == p) a[i++] = 0;
+      if (i == d) break;
+      ++a[i];
+    }
+    std::sort(mi.begin(), mi.end(), [](const std::vector<int> &x, const std::vector<int> &y) {
+      const int sx = std::accumulate(x.begin(), x.end(), 0);
+      const int sy = std::accumulate(y.begin(), y.end(), 0);
+      return sx != sy ? sx < sy : x < y;
+    });
+    q = (int)mi.size();
+    midx.resize(q);
+    for (int k = 0; k < q; ++k) {
+      int pk = 0;
+      for (int i = 0; i < d; ++i) pk |= mi[k][i] << (8 * i);
+      midx[k] = pk;
+    }
+    fmca_order.resize(q);
+    std::iota(fmca_order.begin(), fmca_order.end(), 0);
+    // DeMarchi points: Halton points 100 .. 100 + 10 q - 1, the first q
+    // pivots of the column-pivoted QR of their basis values
+    const int ncand = 10 * q;
+    std::vector<double> cand((size_t)d * ncand), B((size_t)q * ncand);
+    for (int j = 0; j < ncand; ++j) {
+      halton_point(100 + j, d, &cand[(size_t)d * j]);
+      basis(&cand[(size_t)d * j], &B[(size_t)q * j]);
+    }
+    const std::vector<int> piv = greedy_pivots(B, q, ncand);
+    nodes.resize((size_t)d * q);
+    for (int k = 0; k < q; ++k)
+      for (int i = 0; i < d; ++i) nodes[(size_t)d * k + i] = cand[(size_t)d * piv[k] + i];
+    std::vector<double> V((size_t)q * q);
+    std::vector<double> row(q);
+    for (int k = 0; k < q; ++k) {
+      basis(&nodes[(size_t)d * k], row.data());
+      for (int a2 = 0; a2 < q; ++a2) V[k + (size_t)q * a2] = row[a2];
+    }
+    invV = lu_inverse(V, q);
+    return;
+  }
   q = 1;
   for (int i = 0; i < d; ++i) q *= P1;
   const double weight = 2. * kPi / (2. * double(p) + 2.);
@@ -542,6 +733,33 @@ void compute_transfers(std::vector<double> &T, const Tree &tree, const Interpola
         for (int a = 0; a < P1; ++a) Ti[a * P1 + b] = vals[a];
       }
     }
+  });
+}
+
+void compute_transfers_dense(std::vector<double> &E, const Tree &tree, const Interpolation &ip) {
+  const int d = tree.dim, q = ip.q;
+  E.assign((size_t)tree.ncl * q * q, 0.);
+  parallel_for(tree.ncl - 1, 16, [&](int64_t ci) {
+    const int c = 1 + (int)ci;
+    if (tree.size[c] == 0) return;  // FMCA leaves E = 0 for empty sons
+    const int p = tree.parent[c];
+    std::vector<double> Pv((size_t)q * q);  // Pv[j * q + a] = P^father_a(x(j))
+    double x[kMaxDim];
+    for (int j = 0; j < q; ++j) {
+      for (int i = 0; i < d; ++i) {
+        const double sp = tree.bbsize[(size_t)p * d + i];
+        x[i] = ip.nodes[(size_t)d * j + i] * tree.bbsize[(size_t)c * d + i] / sp +
+               (tree.bbmin[(size_t)c * d + i] - tree.bbmin[(size_t)p * d + i]) / sp;
+      }
+      ip.basis(x, &Pv[(size_t)q * j]);
+    }
+    double *Ec = E.data() + (size_t)c * q * q;
+    for (int a = 0; a < q; ++a)
+      for (int b = 0; b < q; ++b) {
+        double acc = 0;
+        for (int j = 0; j < q; ++j) acc += Pv[(size_t)q * j + a] * ip.invV[b + (size_t)q * j];
+        Ec[(size_t)a * q + b] = acc;
+      }
   });
 }
 

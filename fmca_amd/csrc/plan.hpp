@@ -38,18 +38,41 @@ constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 constexpr int kTaskSplit = 1, kTaskFar = 2;  // task flags
 constexpr int kMaxP1 = 10;  // polynomial degree <= 9
 
-// Tensor-product Chebyshev interpolation on [0,1]^d
-// (FMCA/src/Interpolator/TensorProductInterpolator.h:39-81).  Moments are held
-// in tensor order lin = a_0 + P1*a_1 + P1^2*a_2; fmca_order maps FMCA's
-// graded-lexicographic position k (MultiIndexSet.h:87-111) to lin.
+// Interpolation schemes of NystromMoments<Interpolator>.
+enum InterpScheme : int {
+  IS_TENSOR = 0,        // TensorProductInterpolator
+  IS_TOTAL_DEGREE = 1,  // TotalDegreeInterpolator
+};
+
+// Interpolation on [0,1]^d.
+//
+// Tensor scheme (FMCA/src/Interpolator/TensorProductInterpolator.h:39-81):
+// Chebyshev nodes xi and barycentric weights w per axis, invV = I.  Moments
+// are held in tensor order lin = a_0 + P1*a_1 + P1^2*a_2; fmca_order maps
+// FMCA's graded-lexicographic position k (MultiIndexSet.h:87-111) to lin.
+//
+// Total-degree scheme (FMCA/src/Interpolator/TotalDegreeInterpolator.h:27-48):
+// the q = binom(deg+d, d) products of L2-normalised shifted Legendre
+// polynomials (evalPolynomials.h:19-66) with |alpha|_1 <= deg, in FMCA's
+// graded-lexicographic order (fmca_order is the identity); interpolation
+// nodes are DeMarchi points (DeMarchiPoints.h:22-50) and invV = V^-1 with
+// V(i, a) = P_a(node i).
 struct Interpolation {
   int dim = 0, deg = 0, P1 = 0, q = 0;
+  int scheme = IS_TENSOR;
   std::vector<double> xi, w;
   std::vector<int32_t> fmca_order;
-  void init(int dim, int deg);
+  std::vector<int32_t> midx;   // total degree: a_0 | a_1 << 8 | a_2 << 16 per basis function
+  std::vector<double> nodes;   // total degree: dim x q, node k in column k
+  std::vector<double> invV;    // total degree: q x q, column-major
+  void init(int dim, int deg, int scheme = IS_TENSOR);
   // l_a(u), a = 0..deg, with FMCA's rule that a factor within eps of its node
   // is 1 (TensorProductInterpolator.h:71-78).
   void lagrange_1d(double u, double *out) const;
+  // sqrt(2a+1) P_a(2u-1), a = 0..deg (LegendrePolynomials::eval).
+  void legendre_1d(double u, double *out) const;
+  // all q basis values at a point of [0,1]^d (total degree)
+  void basis(const double *u, double *out) const;
 };
 
 // A cluster tree in FMCA's level order.  Clusters of one level are contiguous.
@@ -114,6 +137,12 @@ std::vector<int32_t> xcd_launch_order(const std::vector<double> &work,
 // T[((c*dim + i)*P1 + a)*P1 + b] = l_a(x_i(b)) with x_i(b) the son's node b in
 // the father's reference frame.
 void compute_transfers(std::vector<double> &T, const Tree &tree, const Interpolation &ip);
+
+// Total-degree transfers (compute_cluster_bases_impl.h:36-51): for every
+// non-root cluster c the dense q x q matrix
+//   E_c[a * q + b] = sum_j P^father_a(x(j)) invV[b][j],
+// x(j) = node j of c in the father's reference frame; zero for empty sons.
+void compute_transfers_dense(std::vector<double> &E, const Tree &tree, const Interpolation &ip);
 
 struct Statistics {
   double rows, cols, low_rank_blocks, dense_blocks, nz_per_row, storage_gb;

@@ -44,7 +44,7 @@ struct Actor {
 
 std::unique_ptr<Actor> create(const std::string &ktype, double l, double c, double nu,
                               const py::array &Peval_in, const py::array &P_in, int deg,
-                              double eta) {
+                              double eta, int interpolator) {
   farr Peval = as_f64(Peval_in, "Peval"), P = as_f64(P_in, "P");
   if (Peval.ndim() != 2 || P.ndim() != 2 || Peval.shape(0) != P.shape(0))
     throw py::value_error("points must be (dim, n) arrays of equal dim");
@@ -55,7 +55,8 @@ std::unique_ptr<Actor> create(const std::string &ktype, double l, double c, doub
   fmca_kernel_desc k{ktype.c_str(), l, c, nu};
   {
     py::gil_scoped_release nogil;
-    check(fmca_h2_create(&k, a->dim, a->m, Peval.data(), a->n, P.data(), deg, eta, &a->h));
+    check(fmca_h2_create_interp(&k, a->dim, a->m, Peval.data(), a->n, P.data(), deg, eta,
+                                interpolator, &a->h));
   }
   return a;
 }
@@ -137,9 +138,12 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
   m.attr("KERNEL_VALUE") = (int)FMCA_KERNEL_VALUE;
   m.attr("KERNEL_DL") = (int)FMCA_KERNEL_DL;
   m.attr("KERNEL_PHI") = (int)FMCA_KERNEL_PHI;
+  m.attr("INTERP_TENSOR_PRODUCT") = (int)FMCA_INTERP_TENSOR_PRODUCT;
+  m.attr("INTERP_TOTAL_DEGREE") = (int)FMCA_INTERP_TOTAL_DEGREE;
   py::class_<Actor>(m, "Actor")
       .def(py::init(&create), py::arg("ktype"), py::arg("l"), py::arg("c"), py::arg("nu"),
-           py::arg("Peval"), py::arg("P"), py::arg("deg"), py::arg("eta"))
+           py::arg("Peval"), py::arg("P"), py::arg("deg"), py::arg("eta"),
+           py::arg("interpolator") = (int)FMCA_INTERP_TENSOR_PRODUCT)
       .def_readonly("dim", &Actor::dim)
       .def_readonly("m", &Actor::m)
       .def_readonly("n", &Actor::n)
@@ -264,6 +268,23 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
           },
           py::arg("reset") = false)
       .def("info", &info)
+      .def("interpolation",
+           [](const Actor &a) {
+             int kind = 0, q = 0;
+             check(fmca_h2_interpolation(a.h, &kind, &q, nullptr, nullptr, nullptr));
+             py::array_t<int32_t> mi({(py::ssize_t)q, (py::ssize_t)a.dim});
+             farr Xi({(py::ssize_t)a.dim, (py::ssize_t)q});
+             farr invV({(py::ssize_t)q, (py::ssize_t)q});
+             check(fmca_h2_interpolation(a.h, nullptr, nullptr, mi.mutable_data(),
+                                         Xi.mutable_data(), invV.mutable_data()));
+             py::dict d;
+             d["interpolator"] = kind;
+             d["q"] = q;
+             d["multi_indices"] = mi;
+             d["Xi"] = Xi;
+             d["invV"] = invV;
+             return d;
+           })
       .def("clusters", &clusters)
       .def("blocks", &blocks)
       .def("launch_order", &launch_order, py::arg("which"));

@@ -61,6 +61,33 @@ fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
                            const double *Peval, int64_t n, const double *P, int deg,
                            double eta, fmca_h2_actor **actor);
 
+/* Interpolators of NystromMoments<Interpolator>. */
+enum {
+  FMCA_INTERP_TENSOR_PRODUCT = 0, /* TensorProductInterpolator: q = (deg+1)^dim */
+  FMCA_INTERP_TOTAL_DEGREE = 1    /* TotalDegreeInterpolator: q = binom(deg+dim, dim) */
+};
+
+/* Replaces H2kernelActor<CovarianceKernel, NystromMoments<Interpolator>, ...>
+ * (FMCA/src/H2Matrix/H2kernelActor.h:17-36) with the interpolator chosen at
+ * run time.  FMCA_INTERP_TOTAL_DEGREE is TotalDegreeInterpolator
+ * (FMCA/src/Interpolator/TotalDegreeInterpolator.h:27-48), the interpolator
+ * of FMCA's own actor test (tests/test_H2kernelActor.cpp:20-24) and Python
+ * binding (py/FMCA.cpp:37-38): Legendre moments, DeMarchi nodes, dense invV;
+ * the tree's minimum cluster size is its q.  fmca_h2_create is this call
+ * with FMCA_INTERP_TENSOR_PRODUCT. */
+fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64_t m,
+                                  const double *Peval, int64_t n, const double *P, int deg,
+                                  double eta, int interpolator, fmca_h2_actor **actor);
+
+/* Interpolation data, as the interpolator's idcs(), Xi() and invV() return
+ * them (TotalDegreeInterpolator.h:50-55, TensorProductInterpolator.h:83-85):
+ * the interpolator id, q, the q multi-indices in FMCA's order (q x dim,
+ * row-major), the nodes Xi (dim x q, column-major) and invV (q x q,
+ * column-major; the identity for the tensor product).  Null outputs are
+ * skipped. */
+fmca_status fmca_h2_interpolation(const fmca_h2_actor *actor, int *interpolator, int *q,
+                                  int32_t *multi_indices, double *nodes, double *invV);
+
 /* Destroys the actor and frees its device memory. */
 fmca_status fmca_h2_destroy(fmca_h2_actor *actor);
 

@@ -31,6 +31,7 @@ Pinning status of each oracle function (what checks it in tests/test_oracle.py):
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -67,12 +68,14 @@ def lib():
         L.orc_parse_kernel.restype = ctypes.c_int
         L.orc_kernel_eval.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, vp, lp, vp]
-        L.orc_interp.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_interp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp,
+                                 vp, vp]
         L.orc_interp.restype = ctypes.c_int
-        L.orc_eval_polynomials.argtypes = [ctypes.c_int, ctypes.c_int, vp, lp, vp]
+        L.orc_eval_polynomials.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, lp,
+                                           vp]
         L.orc_create.argtypes = [ctypes.c_int, lp, vp, lp, vp, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                                 ctypes.c_double]
+                                 ctypes.c_double, ctypes.c_int]
         L.orc_create.restype = vp
         L.orc_destroy.argtypes = [vp]
         L.orc_setup_seconds.argtypes = [vp]
@@ -131,6 +134,16 @@ def kernel_eval(name: str, r, l=1.0, c=1.0, nu=1.0) -> np.ndarray:
     return out
 
 
+SCHEMES = {"tensor": 0, "total_degree": 1}
+
+
+def num_basis(dim: int, deg: int, scheme: str = "tensor") -> int:
+    """q: (deg+1)^dim (tensor) or binomial(deg+dim, dim) (total degree)."""
+    if scheme == "tensor":
+        return (deg + 1) ** dim
+    return math.comb(deg + dim, dim)
+
+
 def interp(dim: int, deg: int):
     """Chebyshev nodes, weights, multi-indices (q x dim) and Xi (dim x q)."""
     q = (deg + 1) ** dim
@@ -138,16 +151,29 @@ def interp(dim: int, deg: int):
     w = np.empty(deg + 1)
     mi = np.empty((q, dim), dtype=np.int32)
     Xi = np.empty((q, dim))  # row k = column k of FMCA's Xi
-    lib().orc_interp(dim, deg, _ptr(xi), _ptr(w), _ptr(mi), _ptr(Xi))
+    lib().orc_interp(dim, deg, 0, _ptr(xi), _ptr(w), _ptr(mi), _ptr(Xi), None, None)
     return xi, w, mi, Xi.T.copy()
 
 
-def eval_polynomials(dim: int, deg: int, pts: np.ndarray) -> np.ndarray:
-    """pts: (dim, npts) -> (q, npts)."""
+def interp_total_degree(dim: int, deg: int):
+    """Total-degree data: multi-indices (q x dim), DeMarchi points Xi (dim x q),
+    V (q x q, V[i, a] = P_a(Xi[:, i])) and invV."""
+    q = num_basis(dim, deg, "total_degree")
+    mi = np.empty((q, dim), dtype=np.int32)
+    Xi = np.empty((q, dim))
+    V = np.empty((q, q), order="F")
+    invV = np.empty((q, q), order="F")
+    lib().orc_interp(dim, deg, 1, None, None, _ptr(mi), _ptr(Xi), _ptr(V), _ptr(invV))
+    return mi, Xi.T.copy(), V, invV
+
+
+def eval_polynomials(dim: int, deg: int, pts: np.ndarray, scheme: str = "tensor") -> np.ndarray:
+    """pts: (dim, npts) -> (q, npts): Lagrange products (tensor) or Legendre
+    products (total degree)."""
     pts = np.asfortranarray(pts, dtype=np.float64)
-    q = (deg + 1) ** dim
+    q = num_basis(dim, deg, scheme)
     out = np.empty((q, pts.shape[1]), order="F")
-    lib().orc_eval_polynomials(dim, deg, _ptr(pts), pts.shape[1], _ptr(out))
+    lib().orc_eval_polynomials(dim, deg, SCHEMES[scheme], _ptr(pts), pts.shape[1], _ptr(out))
     return out
 
 
@@ -172,10 +198,12 @@ def dense_matrix(P, Peval, kernel, l, c=1.0, nu=1.0):
 
 
 class OracleH2:
-    """FMCA's H2kernelActor(kernel, Peval, P, deg, eta) restated on the CPU."""
+    """FMCA's H2kernelActor(kernel, Peval, P, deg, eta) restated on the CPU.
+    scheme: "tensor" (NystromMoments<TensorProductInterpolator>) or
+    "total_degree" (NystromMoments<TotalDegreeInterpolator>)."""
 
     def __init__(self, P, Peval, kernel: str, l: float, deg: int, eta: float,
-                 c: float = 1.0, nu: float = 1.0):
+                 c: float = 1.0, nu: float = 1.0, scheme: str = "tensor"):
         self.P = np.asfortranarray(P, dtype=np.float64)
         self.Peval = np.asfortranarray(Peval, dtype=np.float64)
         self.dim = self.P.shape[0]
@@ -183,7 +211,7 @@ class OracleH2:
         self.m = self.Peval.shape[1]
         L = lib()
         self._h = L.orc_create(self.dim, self.n, _ptr(self.P), self.m, _ptr(self.Peval),
-                               kernel_id(kernel), l, c, nu, deg, eta)
+                               kernel_id(kernel), l, c, nu, deg, eta, SCHEMES[scheme])
         if not self._h:
             raise ValueError("oracle: invalid arguments")
         self.q = L.orc_q(self._h)

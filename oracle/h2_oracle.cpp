@@ -11,7 +11,11 @@
 //   splitter        FMCA/src/Clustering/ClusterTreeSplitter.h:198-219
 //   level order     FMCA/src/util/IDDFSForwardIterator.h:37-65
 //   multi-indices   FMCA/src/util/MultiIndexSet.h:87-111,147-187
-//   interpolation   FMCA/src/Interpolator/TensorProductInterpolator.h:39-81
+//   interpolation   FMCA/src/Interpolator/TensorProductInterpolator.h:39-81,
+//                   FMCA/src/Interpolator/TotalDegreeInterpolator.h:27-48,
+//                   FMCA/src/Interpolator/DeMarchiPoints.h:22-50,
+//                   FMCA/src/Interpolator/evalPolynomials.h:19-66,
+//                   FMCA/src/util/HaltonSet.h:23-131
 //   leaf moments    FMCA/src/Moments/NystromMoments.h:24-34
 //   transfers       FMCA/src/H2Matrix/compute_cluster_bases_impl.h:28-57
 //   nestedness KAT  FMCA/src/H2Matrix/compute_cluster_bases_impl.h:59-80
@@ -26,9 +30,13 @@
 //   CG              FMCA/src/util/PCG.h:15-44
 //
 // Deliberate differences from FMCA, none of which changes a result:
-//   * coupling blocks skip FMCA's products with invV = I
+//   * tensor-scheme coupling blocks skip FMCA's products with invV = I
 //     (unsymmetricNystromEvaluator.h:56); multiplying by an identity matrix is
 //     exact in IEEE arithmetic, so S is unchanged;
+//   * the DeMarchi points come from this file's own column-pivoted Householder
+//     QR and invV from Gauss-Jordan elimination instead of Eigen's
+//     ColPivHouseholderQR and inverse(): the pivot order is the same unless
+//     two candidate norms tie to rounding, and invV agrees to rounding;
 //   * kernel entries are evaluated through a switch instead of two
 //     std::function calls with heap temporaries (CovarianceKernel.h:127-143).
 // Both make this CPU action faster than FMCA's, i.e. a conservative baseline.
@@ -127,17 +135,200 @@ struct Kernel {
 };
 
 // ---------------------------------------------------------------------------
-// Tensor-product Chebyshev interpolation (TensorProductInterpolator.h:39-81)
-// with the graded-lexicographic multi-index order of MultiIndexSet.h:87-100.
+// Pieces of the total-degree interpolator (TotalDegreeInterpolator.h:27-48).
+
+// HaltonSet<S> (util/HaltonSet.h:23-131): the first M primes as bases, b-adic
+// digits incremented with carry, radical inverse by Horner in long double.
+// The constructor advances `skip` times (init_HaltonVector); `v` is the
+// current point.
+struct Halton {
+  static constexpr int kDigits = 40;  // HaltonSet = HaltonSetBase<S, 40>
+  std::vector<uint32_t> primes;
+  std::vector<std::vector<uint32_t>> digits;
+  std::vector<int> top;  // highest digit position reached per base
+  std::vector<double> v;
+
+  Halton(int M, int skip) {
+    primes.resize(M);
+    int found = 0;
+    if (M > 0) primes[found++] = 2;
+    for (uint32_t cand = 3; found < M; cand += 2) {
+      bool prime = true;
+      const uint32_t lim = (uint32_t)std::sqrt((double)cand);
+      for (int i = 0; i < found && primes[i] <= lim; ++i)
+        if (cand % primes[i] == 0) {
+          prime = false;
+          break;
+        }
+      if (prime) primes[found++] = cand;
+    }
+    digits.assign(M, std::vector<uint32_t>(kDigits, 0));
+    top.assign(M, 0);
+    v.assign(M, 0.);
+    for (int s = 0; s < skip; ++s) next();
+  }
+
+  void next() {
+    for (size_t i = 0; i < primes.size(); ++i) {
+      int k = 0;
+      for (; k < kDigits; ++k) {
+        if (digits[i][k] + 1 < primes[i]) {
+          ++digits[i][k];
+          break;
+        }
+        digits[i][k] = 0;
+      }
+      top[i] = std::max(top[i], k);
+      const long double binv = (long double)1 / (long double)primes[i];
+      long double r = digits[i][top[i]];
+      for (int j = top[i] - 1; j >= 0; --j) r = binv * r + digits[i][j];
+      r *= binv;
+      v[i] = (double)r;
+    }
+  }
+};
+
+// LegendrePolynomials::eval (evalPolynomials.h:19-40): shifted Legendre
+// polynomials on [0,1] by the three-term recurrence, each scaled by
+// sqrt(2k+1) after the recurrence has used it.
+static void legendre_values(double x, int deg, double *out) {
+  double p0 = 0, p1 = 1;
+  out[0] = p1;
+  for (int k = 1; k <= deg; ++k) {
+    const double pk = double(2 * k - 1) / double(k) * (2 * x - 1) * p1 -
+                      double(k - 1) / double(k) * p0;
+    p0 = p1;
+    p1 = pk;
+    out[k] = pk * std::sqrt(double(2 * k + 1));
+  }
+}
+
+// Column order of Householder QR with column pivoting, as Eigen's
+// ColPivHouseholderQR computes it (DeMarchiPoints.h:38-40 uses
+// colsPermutation): at step k the remaining column of largest (downdated)
+// norm, the first one on ties; norms downdated after each reflection and
+// recomputed when the downdate loses more than half the digits (the xGEQP3
+// safeguard).  A is rows x cols, column-major, and is overwritten.
+static std::vector<int> pivoted_qr_order(std::vector<double> &A, int rows, int cols) {
+  auto col = [&](int j) { return A.data() + (size_t)rows * j; };
+  std::vector<double> nrm_direct(cols), nrm(cols);
+  std::vector<int> order(cols);
+  std::iota(order.begin(), order.end(), 0);
+  for (int j = 0; j < cols; ++j) {
+    double s = 0;
+    for (int i = 0; i < rows; ++i) s += col(j)[i] * col(j)[i];
+    nrm_direct[j] = nrm[j] = std::sqrt(s);
+  }
+  const double downdate_tol = std::sqrt(kEps);
+  const int steps = std::min(rows, cols);
+  for (int k = 0; k < steps; ++k) {
+    int best = k;
+    for (int j = k + 1; j < cols; ++j)
+      if (nrm[j] > nrm[best]) best = j;
+    if (best != k) {
+      std::swap_ranges(col(k), col(k) + rows, col(best));
+      std::swap(nrm[k], nrm[best]);
+      std::swap(nrm_direct[k], nrm_direct[best]);
+      std::swap(order[k], order[best]);
+    }
+    // reflector I - tau v v^T with v = [1; ess] mapping A(k:, k) to beta e_1
+    double *a = col(k);
+    double tail = 0;
+    for (int i = k + 1; i < rows; ++i) tail += a[i] * a[i];
+    double tau = 0, beta = a[k];
+    if (tail > std::numeric_limits<double>::min()) {
+      beta = std::sqrt(a[k] * a[k] + tail);
+      if (a[k] >= 0) beta = -beta;
+      const double den = a[k] - beta;
+      for (int i = k + 1; i < rows; ++i) a[i] /= den;
+      tau = (beta - a[k]) / beta;
+    } else {
+      for (int i = k + 1; i < rows; ++i) a[i] = 0;
+    }
+    a[k] = beta;
+    for (int j = k + 1; j < cols; ++j) {
+      double *b = col(j);
+      if (tau != 0) {
+        double wdot = b[k];
+        for (int i = k + 1; i < rows; ++i) wdot += a[i] * b[i];
+        b[k] -= tau * wdot;
+        for (int i = k + 1; i < rows; ++i) b[i] -= tau * a[i] * wdot;
+      }
+      if (nrm[j] != 0) {
+        double t = std::abs(b[k]) / nrm[j];
+        t = std::max(0., (1 + t) * (1 - t));
+        const double ratio = nrm[j] / nrm_direct[j];
+        if (t * (ratio * ratio) <= downdate_tol) {
+          double s = 0;
+          for (int i = k + 1; i < rows; ++i) s += b[i] * b[i];
+          nrm_direct[j] = nrm[j] = std::sqrt(s);
+        } else {
+          nrm[j] *= std::sqrt(t);
+        }
+      }
+    }
+  }
+  return order;
+}
+
+// Inverse of an n x n column-major matrix by Gauss-Jordan elimination with
+// partial pivoting (Eigen's inverse() of a dynamic matrix is an LU solve).
+static std::vector<double> invert(std::vector<double> M, int n) {
+  std::vector<double> R((size_t)n * n, 0.);
+  for (int i = 0; i < n; ++i) R[i + (size_t)n * i] = 1;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < n; ++i)
+      if (std::abs(M[i + (size_t)n * c]) > std::abs(M[piv + (size_t)n * c])) piv = i;
+    if (piv != c)
+      for (int j = 0; j < n; ++j) {
+        std::swap(M[c + (size_t)n * j], M[piv + (size_t)n * j]);
+        std::swap(R[c + (size_t)n * j], R[piv + (size_t)n * j]);
+      }
+    const double d = M[c + (size_t)n * c];
+    for (int j = 0; j < n; ++j) {
+      M[c + (size_t)n * j] /= d;
+      R[c + (size_t)n * j] /= d;
+    }
+    for (int i = 0; i < n; ++i) {
+      if (i == c) continue;
+      const double f = M[i + (size_t)n * c];
+      if (f == 0) continue;
+      for (int j = 0; j < n; ++j) {
+        M[i + (size_t)n * j] -= f * M[c + (size_t)n * j];
+        R[i + (size_t)n * j] -= f * R[c + (size_t)n * j];
+      }
+    }
+  }
+  return R;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation on [0,1]^d with the graded-lexicographic multi-index order of
+// MultiIndexSet.h:87-100:
+//   kTensor       tensor-product Chebyshev nodes, barycentric Lagrange values
+//                 (TensorProductInterpolator.h:39-81), invV = I;
+//   kTotalDegree  total-degree Legendre products at DeMarchi points
+//                 (TotalDegreeInterpolator.h:27-48, DeMarchiPoints.h:22-50,
+//                 evalPolynomials.h:42-66), dense invV = V^-1.
+enum Scheme : int { kTensor = 0, kTotalDegree = 1 };
+
 struct Interp {
   int dim = 0, deg = 0, q = 0;
+  int scheme = kTensor;
   std::vector<double> xi, w;           // deg+1 nodes / barycentric weights
   std::vector<std::vector<int>> mi;    // q multi-indices, FMCA order
   std::vector<double> Xi;              // dim x q, column-major
+  std::vector<double> V, invV;         // q x q, column-major (total degree only)
 
-  void init(int d, int p) {
+  void init(int d, int p, int sch = kTensor) {
     dim = d;
     deg = p;
+    scheme = sch;
+    if (sch == kTotalDegree) {
+      init_total_degree();
+      return;
+    }
     const double weight = 2. * kPi / (2. * double(p) + 2.);
     xi.resize(p + 1);
     w.resize(p + 1);
@@ -170,8 +361,64 @@ struct Interp {
       for (int i = 0; i < d; ++i) Xi[i + (size_t)d * k] = xi[mi[k][i]];
   }
 
-  // Barycentric Lagrange values of all q polynomials at a point of [0,1]^d.
+  // MultiIndexSet<TotalDegree> (MultiIndexSet.h:39-50,147-187), DeMarchi
+  // points from 10 q Halton points after skipping 100 (DeMarchiPoints.h:22-40,
+  // TotalDegreeInterpolator.h:41-46) and V(i, :) = P(Xi(:, i))^T, invV = V^-1.
+  void init_total_degree() {
+    mi.clear();
+    std::vector<int> a(dim, 0);
+    while (true) {
+      int sum = 0;
+      for (int v : a) sum += v;
+      if (sum <= deg) mi.push_back(a);
+      int i = 0;
+      while (i < dim && a[i] == deg) a[i++] = 0;
+      if (i == dim) break;
+      ++a[i];
+    }
+    std::sort(mi.begin(), mi.end(), [](const std::vector<int> &x, const std::vector<int> &y) {
+      int nx = 0, ny = 0;
+      for (int v : x) nx += v;
+      for (int v : y) ny += v;
+      if (nx != ny) return nx < ny;
+      return std::lexicographical_compare(x.begin(), x.end(), y.begin(), y.end());
+    });
+    q = (int)mi.size();
+    const int ncand = 10 * q;
+    std::vector<double> cand((size_t)dim * ncand), VT((size_t)q * ncand);
+    Halton hs(dim, 100);
+    for (int j = 0; j < ncand; ++j) {
+      for (int i = 0; i < dim; ++i) cand[i + (size_t)dim * j] = hs.v[i];
+      hs.next();
+    }
+    for (int j = 0; j < ncand; ++j) eval(cand.data() + (size_t)dim * j, VT.data() + (size_t)q * j);
+    const std::vector<int> order = pivoted_qr_order(VT, q, ncand);
+    Xi.assign((size_t)dim * q, 0.);
+    for (int k = 0; k < q; ++k)
+      for (int i = 0; i < dim; ++i) Xi[i + (size_t)dim * k] = cand[i + (size_t)dim * order[k]];
+    V.assign((size_t)q * q, 0.);
+    std::vector<double> row(q);
+    for (int k = 0; k < q; ++k) {
+      eval(Xi.data() + (size_t)dim * k, row.data());
+      for (int a2 = 0; a2 < q; ++a2) V[k + (size_t)q * a2] = row[a2];
+    }
+    invV = invert(V, q);
+  }
+
+  // Values of all q basis polynomials at a point of [0,1]^d: barycentric
+  // Lagrange products (tensor) or Legendre products (total degree,
+  // evalPolynomials.h:53-64).
   void eval(const double *pt, double *out) const {
+    if (scheme == kTotalDegree) {
+      std::vector<double> pv((size_t)dim * (deg + 1));
+      for (int i = 0; i < dim; ++i) legendre_values(pt[i], deg, pv.data() + (size_t)(deg + 1) * i);
+      for (int k = 0; k < q; ++k) {
+        double v = 1.;
+        for (int i = 0; i < dim; ++i) v *= pv[(size_t)(deg + 1) * i + mi[k][i]];
+        out[k] = v;
+      }
+      return;
+    }
     std::vector<double> weight(dim);
     for (int i = 0; i < dim; ++i) {
       double s = 0;
@@ -369,6 +616,16 @@ static void compute_bases(Tree &T, int t, const Interp &I) {
                     (T.bb(c, i, 0) - T.bb(t, i, 0)) / T.bb(t, i, 2);
           I.eval(pt.data(), E.data() + (size_t)q * j);
         }
+      if (I.scheme == kTotalDegree) {
+        // E = E * invV^T (compute_cluster_bases_impl.h:50)
+        std::vector<double> P = E;
+        for (int b2 = 0; b2 < q; ++b2)
+          for (int a2 = 0; a2 < q; ++a2) {
+            double acc = 0;
+            for (int j = 0; j < q; ++j) acc += P[a2 + (size_t)q * j] * I.invV[b2 + (size_t)q * j];
+            E[a2 + (size_t)q * b2] = acc;
+          }
+      }
     }
   } else {
     nd.V.assign((size_t)q * nd.size, 0.);
@@ -534,7 +791,8 @@ static void backward_rec(const Actor &A, int t, double *lhs, Index nrhs,
   }
 }
 
-// interpolate_kernel (unsymmetricNystromEvaluator.h:42-58), invV = I
+// interpolate_kernel (unsymmetricNystromEvaluator.h:42-58); the products with
+// invV are skipped for the tensor scheme, where invV = I
 static void coupling_matrix(const Actor &A, int t, int s, std::vector<double> &S) {
   const int d = A.dim, q = A.q;
   std::vector<double> X((size_t)d * q), Y((size_t)d * q);
@@ -547,6 +805,22 @@ static void coupling_matrix(const Actor &A, int t, int s, std::vector<double> &S
   for (int j = 0; j < q; ++j)
     for (int i = 0; i < q; ++i)
       S[i + (size_t)q * j] = A.K(X.data() + (size_t)d * i, Y.data() + (size_t)d * j, d);
+  if (A.I.scheme == kTotalDegree) {
+    // S = (invV * S) * invV^T (unsymmetricNystromEvaluator.h:56)
+    const std::vector<double> &iv = A.I.invV;
+    std::vector<double> L((size_t)q * q, 0.);
+    for (int j = 0; j < q; ++j)
+      for (int k = 0; k < q; ++k) {
+        const double sv = S[k + (size_t)q * j];
+        for (int i = 0; i < q; ++i) L[i + (size_t)q * j] += iv[i + (size_t)q * k] * sv;
+      }
+    for (int j = 0; j < q; ++j)
+      for (int i = 0; i < q; ++i) {
+        double acc = 0;
+        for (int k = 0; k < q; ++k) acc += L[i + (size_t)q * k] * iv[j + (size_t)q * k];
+        S[i + (size_t)q * j] = acc;
+      }
+  }
 }
 
 // One leaf block of the hot loop (H2kernelActor.h:59-70); srhs/lhs in cluster order.
@@ -621,12 +895,17 @@ void orc_kernel_eval(int type, double l, double c, double nu, const double *r, l
   for (long i = 0; i < n; ++i) out[i] = K(r[i]);
 }
 
-// Chebyshev data: xi, w (deg+1 each), multi-indices (q x dim, FMCA order), Xi (dim x q).
-int orc_interp(int dim, int deg, double *xi, double *w, int *mi, double *Xi) {
+// Interpolation data of a scheme (0 tensor, 1 total degree): xi, w (deg+1
+// each, tensor only), multi-indices (q x dim, FMCA order), Xi (dim x q), and
+// for total degree V, invV (q x q, column-major).  Returns q.
+int orc_interp(int dim, int deg, int scheme, double *xi, double *w, int *mi, double *Xi,
+               double *V, double *invV) {
   Interp I;
-  I.init(dim, deg);
+  I.init(dim, deg, scheme);
   if (xi) std::copy(I.xi.begin(), I.xi.end(), xi);
   if (w) std::copy(I.w.begin(), I.w.end(), w);
+  if (V) std::copy(I.V.begin(), I.V.end(), V);
+  if (invV) std::copy(I.invV.begin(), I.invV.end(), invV);
   if (mi)
     for (int k = 0; k < I.q; ++k)
       for (int i = 0; i < dim; ++i) mi[i + dim * k] = I.mi[k][i];
@@ -635,15 +914,17 @@ int orc_interp(int dim, int deg, double *xi, double *w, int *mi, double *Xi) {
 }
 
 // evalPolynomials at npts points (dim x npts) -> q x npts.
-void orc_eval_polynomials(int dim, int deg, const double *pts, long npts, double *out) {
+void orc_eval_polynomials(int dim, int deg, int scheme, const double *pts, long npts,
+                          double *out) {
   Interp I;
-  I.init(dim, deg);
+  I.init(dim, deg, scheme);
   for (long j = 0; j < npts; ++j) I.eval(pts + (size_t)dim * j, out + (size_t)I.q * j);
 }
 
 void *orc_create(int dim, long n, const double *P, long m, const double *Peval, int ktype,
-                 double l, double c, double nu, int deg, double eta) {
+                 double l, double c, double nu, int deg, double eta, int scheme) {
   if (dim < 1 || n < 1 || m < 1 || deg < 0 || ktype < 0) return nullptr;
+  if (scheme != kTensor && scheme != kTotalDegree) return nullptr;
   auto t0 = std::chrono::steady_clock::now();
   Actor *A = new Actor();
   A->dim = dim;
@@ -654,7 +935,7 @@ void *orc_create(int dim, long n, const double *P, long m, const double *Peval, 
   A->Peval.assign(Peval, Peval + (size_t)dim * m);
   A->K = Kernel{ktype, l, c, nu};
   A->eta = eta;
-  A->I.init(dim, deg);
+  A->I.init(dim, deg, scheme);
   A->q = A->I.q;
   // H2ClusterTree::init (H2ClusterTree.h:65-74): min cluster size = max(0, q)
   const Index mincsize = (Index)A->q;

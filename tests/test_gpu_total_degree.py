@@ -1,0 +1,160 @@
+"""GPU parity of the total-degree form (TotalDegreeInterpolator) with the oracle.
+
+FMCA's own actor test (tests/test_H2kernelActor.cpp:20-24) and its Python
+binding (py/FMCA.cpp:37) build NystromMoments on TotalDegreeInterpolator.
+The GPU applies the coupling invV K invV^T as basis changes of the moments
+(invV^T before, invV after) around the nodal coupling kernels, and the
+transfers as dense q x q matrices; the oracle follows FMCA's order (coupling
+matrices contracted with invV per block).
+
+Bar: FP64 GPU action within 1e-12 relative of the oracle on the same tree;
+FP32 within 1e-5 of the FP64 GPU action.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err, uniform_points
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+# (label, n, m or None (= same points), dim, deg, kernel, l, eta, nrhs)
+CASES = [
+    ("td-2d-exp-1rhs", 10000, None, 2, 4, "EXPONENTIAL", 1.0, 0.8, 1),
+    ("td-3d-matern32-32rhs", 6000, None, 3, 4, "MATERN32", 0.5, 0.8, 32),
+    ("td-3d-deg6-gauss-8rhs", 5000, None, 3, 6, "GAUSSIAN", 0.3, 0.8, 8),
+    ("td-rect-3d-deg3-5rhs", 3000, 4000, 3, 3, "MATERN52", 0.4, 0.8, 5),
+    ("td-1d-deg5-3rhs", 4000, None, 1, 5, "EXPONENTIAL", 0.3, 0.8, 3),
+    ("td-3d-deg9-q220-2rhs", 4000, None, 3, 9, "EXPONENTIAL", 0.5, 0.8, 2),
+    ("td-2d-65rhs", 5000, None, 2, 5, "MATERN32", 0.3, 0.8, 65),
+    ("td-invmq-eta04-16rhs", 3000, 2000, 2, 4, "INVMULTIQUADRIC", 0.7, 0.4, 16),
+    ("td-3d-deg7-32rhs-valu-fallback", 4000, None, 3, 7, "MATERN32", 0.4, 0.8, 32),
+]
+
+
+def _setup(case):
+    import fmca_amd
+    label, n, m, d, deg, ker, l, eta, nrhs = case
+    print(f"[case] {label}", flush=True)
+    P = uniform_points(n, d, seed=21)
+    Peval = P if m is None else uniform_points(m, d, seed=22)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel(ker, l), Peval, P, deg, eta,
+                               interpolator="total_degree")
+    A = oracle.OracleH2(P, Peval, ker, l, deg, eta, scheme="total_degree")
+    x = np.asfortranarray(np.random.default_rng(23).standard_normal((n, nrhs)))
+    return H, A, x
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_total_degree_fp64_action_matches_oracle(case):
+    H, A, x = _setup(case)
+    assert H.interpolation()["q"] == A.q
+    err = rel_err(H.action(x), A.action(x))
+    print(f"  fp64 vs oracle rel err {err:.3e}", flush=True)
+    assert err <= FP64_TOL
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_total_degree_cluster_order_matches_oracle(case):
+    H, A, x = _setup(case)
+    assert rel_err(H.action_cluster_order(x), A.action_cluster_order(x)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_total_degree_fp32_close_to_fp64(case):
+    import fmca_amd
+    H, _, x = _setup(case)
+    err = rel_err(H.action(x, precision=fmca_amd.FP32), H.action(x))
+    print(f"  fp32 vs fp64 rel err {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_total_degree_fmca_actor_test_configuration():
+    """tests/test_H2kernelActor.cpp: 10000 points in [-1,1]^2, EXPONENTIAL
+    l = 1, degree 4, unit-vector columns, eta = 0.8, 0.4, 0.2, 0.1: the GPU
+    matches the oracle and the error against the exact column falls with eta."""
+    import fmca_amd
+    from scipy.spatial.distance import cdist
+    rng = np.random.default_rng(5)
+    n = 10000
+    P = rng.uniform(-1, 1, (2, n))
+    idx = rng.integers(0, n, 10)
+    X = np.zeros((n, 10), order="F")
+    X[idx, np.arange(10)] = 1
+    C = np.exp(-cdist(P.T, P[:, idx].T))
+    errs = []
+    for eta in (0.8, 0.4, 0.2, 0.1):
+        print(f"[case] actor test eta {eta}", flush=True)
+        H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("EXPONENTIAL", 1.0), P, P, 4, eta,
+                                   interpolator="total_degree")
+        y = H.action(X)
+        yo = oracle.OracleH2(P, P, "EXPONENTIAL", 1.0, 4, eta, scheme="total_degree").action(X)
+        assert rel_err(y, yo) <= FP64_TOL
+        errs.append(rel_err(y, C))
+    print(f"  errors vs exact columns {errs}", flush=True)
+    assert errs[0] < 1e-4 and all(b < a for a, b in zip(errs, errs[1:]))
+
+
+def test_h2matrix_binding_defaults_to_total_degree():
+    """fmca_amd.H2Matrix mirrors FMCA's binding, whose Moments use
+    TotalDegreeInterpolator (py/FMCA.cpp:37-38)."""
+    import fmca_amd
+    P = uniform_points(4000, 3, seed=31)
+    x = np.random.default_rng(32).standard_normal(4000)
+    H = fmca_amd.H2Matrix(fmca_amd.CovarianceKernel("GAUSSIAN", 0.4), P, P, 3, 0.8)
+    assert H.interpolation()["interpolator"] == "total_degree"
+    ref = oracle.OracleH2(P, P, "GAUSSIAN", 0.4, 3, 0.8, scheme="total_degree").action(x)
+    assert rel_err(H.action(x).ravel(), ref.ravel()) <= FP64_TOL
+
+
+def test_total_degree_transposed_action_matches_oracle():
+    import torch
+    import fmca_amd
+    P = uniform_points(4000, 3, seed=5)
+    Pe = uniform_points(2500, 3, seed=6)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN52", 0.4), Pe, P, 4, 0.8,
+                               interpolator="total_degree")
+    g = np.random.default_rng(7).standard_normal((2500, 5))
+    got = H.action_transpose_device(torch.from_numpy(g).cuda()).cpu().numpy()
+    ref = oracle.OracleH2(Pe, P, "MATERN52", 0.4, 4, 0.8, scheme="total_degree").action(g)
+    assert rel_err(got, ref) <= FP64_TOL
+
+
+def test_total_degree_gradients_vs_dense_autograd():
+    """d/dalpha, d/dl and d/dpoints of 0.5 |K alpha - y|^2 through the
+    total-degree H2 matvec against dense FP64 autograd."""
+    import torch
+    from fmca_amd.autograd import H2Spec, h2_matvec
+    from fmca_amd import workloads as W
+    print("[case] total-degree autograd", flush=True)
+    n, l = 4096, 0.3
+    P = torch.from_numpy(W.uniform(n, 2, 0)).cuda()
+    alpha = torch.from_numpy(W.normal_rhs(n, 4, 1)).cuda()
+    y = torch.from_numpy(W.normal_rhs(n, 4, 2)).cuda()
+    spec = H2Spec("MATERN32", 6, 0.8, interpolator="total_degree")
+    Pg, ag = P.clone().requires_grad_(True), alpha.clone().requires_grad_(True)
+    lt = torch.tensor(l, dtype=torch.float64, requires_grad=True)
+    (0.5 * ((h2_matvec(ag, lt, Pg, spec) - y) ** 2).sum()).backward()
+    Pd, ad = P.clone().requires_grad_(True), alpha.clone().requires_grad_(True)
+    ld = torch.tensor(l, dtype=torch.float64, device="cuda", requires_grad=True)
+    diff = Pd[:, :, None] - Pd[:, None, :]
+    r2 = (diff ** 2).sum(0)
+    zero = r2 == 0  # sqrt has no derivative at 0: the kernel's is 0 there
+    r = torch.where(zero, torch.zeros_like(r2),
+                    torch.sqrt(torch.where(zero, torch.ones_like(r2), r2)))
+    a = np.sqrt(3.0) * r / ld
+    K = (1 + a) * torch.exp(-a)
+    (0.5 * ((K @ ad - y) ** 2).sum()).backward()
+    with torch.no_grad():
+        efwd = rel_err(spec.actor(P, None, l).action_device(alpha).cpu().numpy(),
+                       (K @ alpha).cpu().numpy())
+    ea = rel_err(ag.grad.cpu().numpy(), ad.grad.cpu().numpy())
+    el = abs(lt.grad.item() - ld.grad.item()) / abs(ld.grad.item())
+    eP = rel_err(Pg.grad.cpu().numpy(), Pd.grad.cpu().numpy())
+    print(f"  forward H2 err {efwd:.2e}; rel err alpha {ea:.2e} l {el:.2e} points {eP:.2e}",
+          flush=True)
+    assert efwd < 1e-5
+    bound = 20 * efwd
+    assert ea < bound and el < bound and eP < bound

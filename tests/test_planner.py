@@ -65,10 +65,12 @@ def test_statistics_semantics():
     """H2MatrixBase::statistics (H2MatrixBase.h:70-105)."""
     P = uniform_points(8000, 3, seed=3)
     H = fmca_amd.H2Matrix(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 3, 0.8)
+    q = H.interpolation()["q"]
+    assert q == 20  # FMCA's binding: TotalDegreeInterpolator, binom(3 + 3, 3)
     st = H.statistics().ravel()
     r, c, lr = H.blocks()
     cl = H.clusters(0)
-    mem = lr.sum() * 64 ** 2 + sum(cl["size"][t] * cl["size"][s] for t, s, l in zip(r, c, lr) if not l)
+    mem = lr.sum() * q ** 2 + sum(cl["size"][t] * cl["size"][s] for t, s, l in zip(r, c, lr) if not l)
     assert st[0] == 8000 and st[1] == 8000
     assert st[2] == lr.sum() and st[3] == (~lr).sum()
     assert st[4] == round(mem / 8000) and st[5] == pytest.approx(mem * 8 / 1e9)
