@@ -47,7 +47,8 @@ def parse():
     p.add_argument("--no-solve", action="store_true")
     p.add_argument("--ridge", type=float, default=100.0)
     p.add_argument("--workload", default="matvec",
-                   choices=["matvec", "small-dense", "clustered-cg", "autograd", "large"],
+                   choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
+                            "total-degree"],
                    help="matvec: the headline line (default); the others print a side line "
                         "for the other configurations")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -239,6 +240,52 @@ def side_clustered_cg(args):
     return res
 
 
+def side_total_degree(args):
+    """n = 1e6 uniform 3-D, MATERN32 l = 0.5, deg 4, eta 0.8 with
+    TotalDegreeInterpolator (q = 35, leaves of 35-69 points), 32 rhs N(0,1):
+    plan time, FP32 / FP64 action times per phase, 1-rhs times, and the H2
+    error on 200 sampled rows against exact K x, beside the tensor-product
+    form (q = 125) of the same degree."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    n, nrhs = 1_000_000, 32
+    P = W.uniform(n, 3, 0)
+    x = torch.from_numpy(W.normal_rhs(n, nrhs, 1)).cuda()
+    out = torch.empty((nrhs, n), dtype=torch.float64, device="cuda").t()
+    rows = np.sort(np.random.default_rng(2).choice(n, 200, replace=False))
+    Pt = torch.from_numpy(np.ascontiguousarray(P.T)).cuda()
+    r = torch.cdist(Pt[torch.from_numpy(rows).cuda()], Pt) * (np.sqrt(3.0) / 0.5)
+    exact = ((1 + r) * torch.exp(-r)) @ x
+    del r
+    res = {"workload": "total-degree: n=1e6 uniform 3D, MATERN32 l=0.5, deg 4, eta 0.8, "
+                       "TotalDegreeInterpolator (q=35), 32 rhs"}
+    for interp in ("total_degree", "tensor_product"):
+        t0 = time.perf_counter()
+        H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 4, 0.8,
+                                   interpolator=interp)
+        entry = {"plan_s": time.perf_counter() - t0, "info": H.info()}
+        ys = {}
+        for name, prec in (("fp32", fmca_amd.FP32), ("fp64", fmca_amd.FP64)):
+            t, ph = timed_steps(H, x, out, prec, 3, 1, True)
+            ys[name] = out.clone()
+            entry[name] = {"ms": t * 1e3, "mpts": n * nrhs / t / 1e6,
+                           "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph.items()}}
+        entry["fp32_vs_fp64_rel_err"] = float(torch.linalg.norm(ys["fp32"] - ys["fp64"])
+                                              / torch.linalg.norm(ys["fp64"]))
+        sel = ys["fp64"][torch.from_numpy(rows).cuda()]
+        entry["h2_rel_err_200_rows"] = float(torch.linalg.norm(sel - exact)
+                                             / torch.linalg.norm(exact))
+        x1 = x[:, :1].contiguous()
+        for name, prec in (("fp32", fmca_amd.FP32), ("fp64", fmca_amd.FP64)):
+            H.action_device(x1, precision=prec)
+            t, _ = _sync_time(lambda: H.action_device(x1, precision=prec), 3)
+            entry[name]["rhs1_ms"] = t * 1e3
+        res[interp] = entry
+        del ys, H
+    return res
+
+
 def side_autograd(args):
     """n = 2e5 uniform 2-D, MATERN32 l = 0.3, deg 5, eta 0.8, 8 rhs N(0,1):
     L = 0.5 ||K(l) alpha - y||^2, forward and backward (d alpha, d l, d points)
@@ -330,7 +377,8 @@ def main():
     import fmca_amd
 
     side = {"small-dense": side_small_dense, "clustered-cg": side_clustered_cg,
-            "autograd": side_autograd, "large": side_large}.get(args.workload)
+            "autograd": side_autograd, "large": side_large,
+            "total-degree": side_total_degree}.get(args.workload)
     if side is not None:
         print(json.dumps(side(args)), flush=True)
         return

@@ -252,6 +252,38 @@ class H2Matrix(H2kernelActor):
         super().__init__(cov, Pr, Pc, p, eta, interpolator)
 
 
+class MultipoleFunctionEvaluator:
+    """FMCA::MultipoleFunctionEvaluator(kernel, P, Peval, eta=0, mpole_deg=3)
+    (FMCA/src/KernelInterpolation/MultipoleFunctionEvaluator.h:17-67):
+    out-of-sample evaluation K(Peval, P) C through the H2 action with
+    TotalDegreeInterpolator moments.  ``evaluate(P, Peval, C)`` returns the
+    (m, ncols) values in natural order; point sets other than the ones of
+    ``init`` are planned anew."""
+
+    def __init__(self, kernel: CovarianceKernel = None, P=None, Peval=None, eta: float = 0.0,
+                 mpole_deg: int = 3):
+        self._h = None
+        if kernel is not None:
+            self.init(kernel, P, Peval, eta, mpole_deg)
+
+    def init(self, kernel: CovarianceKernel, P, Peval, eta: float = 0.0, mpole_deg: int = 3):
+        self.kernel = kernel
+        self.eta = float(eta) if eta >= 0 else 0.0  # MultipoleFunctionEvaluator.h:37
+        self.mpole_deg = int(mpole_deg)
+        self._P = _points(P, "P").copy(order="F")
+        self._Peval = _points(Peval, "Peval").copy(order="F")
+        self._h = H2kernelActor(kernel, self._Peval, self._P, self.mpole_deg, self.eta,
+                                interpolator="total_degree")
+
+    def evaluate(self, P, Peval, C, precision: int = FP64) -> np.ndarray:
+        if self._h is None:
+            raise RuntimeError("MultipoleFunctionEvaluator: init() first")
+        P, Peval = _points(P, "P"), _points(Peval, "Peval")
+        if not (np.array_equal(P, self._P) and np.array_equal(Peval, self._Peval)):
+            self.init(self.kernel, P, Peval, self.eta, self.mpole_deg)
+        return self._h.action(C, precision)
+
+
 def _as_device_matrix(t, rows):
     import torch
     if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
@@ -275,4 +307,4 @@ def _check_out(out, rows, nrhs):
 
 
 __all__ = ["CovarianceKernel", "H2Matrix", "H2kernelActor", "FP64", "FP32", "ROWS", "COLS",
-           "KERNEL_TYPES", "INTERPOLATORS"]
+           "KERNEL_TYPES", "INTERPOLATORS", "MultipoleFunctionEvaluator"]

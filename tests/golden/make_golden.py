@@ -6,7 +6,8 @@ for the H2 error, from its dense K*x.  FMCA itself cannot be built here (its
 Eigen3 dependency is absent), so these pin this build's planner and action
 against later changes; they are not FMCA outputs.
 
-  python tests/golden/make_golden.py     (rewrites tests/golden/*.npz)
+  python tests/golden/make_golden.py [name ...]   (rewrites tests/golden/<name>.npz,
+                                                   all cases without names)
 """
 from __future__ import annotations
 
@@ -20,23 +21,33 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import oracle  # noqa: E402
 
-# name, dim, n, m (None: square), kernel, l, deg, eta, nrhs, seed
+# name, dim, n, m (None: square), kernel, l, deg, eta, nrhs, seed, interpolator
 CASES = [
-    ("exp3d_n4000_deg2", 3, 4000, None, "EXPONENTIAL", 0.5, 2, 0.8, 2, 101),
-    ("matern32_2d_n2000_deg4", 2, 2000, None, "MATERN32", 0.3, 4, 0.8, 3, 102),
-    ("gauss3d_rect_n3000_m2500_deg2", 3, 3000, 2500, "GAUSSIAN", 0.2, 2, 0.8, 2, 103),
+    ("exp3d_n4000_deg2", 3, 4000, None, "EXPONENTIAL", 0.5, 2, 0.8, 2, 101, "tensor_product"),
+    ("matern32_2d_n2000_deg4", 2, 2000, None, "MATERN32", 0.3, 4, 0.8, 3, 102, "tensor_product"),
+    ("gauss3d_rect_n3000_m2500_deg2", 3, 3000, 2500, "GAUSSIAN", 0.2, 2, 0.8, 2, 103,
+     "tensor_product"),
+    ("td_matern32_3d_n4000_deg4", 3, 4000, None, "MATERN32", 0.4, 4, 0.8, 3, 104, "total_degree"),
+    ("td_exp2d_rect_n3000_m2000_deg4", 2, 3000, 2000, "EXPONENTIAL", 1.0, 4, 0.8, 2, 105,
+     "total_degree"),
 ]
+SCHEME = {"tensor_product": "tensor", "total_degree": "total_degree"}
 
 
 def make(case):
-    name, dim, n, m, kernel, l, deg, eta, nrhs, seed = case
+    name, dim, n, m, kernel, l, deg, eta, nrhs, seed, interpolator = case
     rng = np.random.default_rng(seed)
     P = np.asfortranarray(rng.random((dim, n)))
     Pe = P if m is None else np.asfortranarray(rng.random((dim, m)))
     x = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
-    A = oracle.OracleH2(P, Pe, kernel, l, deg, eta)
+    A = oracle.OracleH2(P, Pe, kernel, l, deg, eta, scheme=SCHEME[interpolator])
     row, col, lr = A.blocks()
+    extra = {}
+    if interpolator == "total_degree":
+        _, Xi, _, invV = oracle.interp_total_degree(dim, deg)
+        extra = dict(Xi=Xi, invV=invV)
     return dict(
+        **extra, interpolator=interpolator,
         dim=dim, kernel=kernel, l=l, deg=deg, eta=eta, P=P, Peval=Pe, rhs=x,
         rindices=A.indices(0), cindices=A.indices(1),
         block_row=row, block_col=col, block_lr=lr,
@@ -47,7 +58,10 @@ def make(case):
 
 def main():
     oracle.build()
+    names = sys.argv[1:]
     for case in CASES:
+        if names and case[0] not in names:
+            continue
         d = make(case)
         path = os.path.join(HERE, case[0] + ".npz")
         np.savez_compressed(path, **d)

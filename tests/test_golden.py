@@ -17,18 +17,31 @@ IDS = [os.path.basename(f)[:-4] for f in FIXTURES]
 
 def load(path):
     with np.load(path, allow_pickle=False) as z:
-        return {k: z[k] for k in z.files}
+        g = {k: z[k] for k in z.files}
+    g["interpolator"] = str(g.get("interpolator", "tensor_product"))
+    return g
+
+
+SCHEME = {"tensor_product": "tensor", "total_degree": "total_degree"}
+
+
+def actor(g):
+    import fmca_amd
+    K = fmca_amd.CovarianceKernel(str(g["kernel"]), float(g["l"]))
+    return fmca_amd.H2kernelActor(K, g["Peval"], g["P"], int(g["deg"]), float(g["eta"]),
+                                  interpolator=g["interpolator"])
 
 
 def test_fixtures_present():
-    assert len(FIXTURES) >= 3
+    assert len(FIXTURES) >= 5
+    assert {load(f)["interpolator"] for f in FIXTURES} == {"tensor_product", "total_degree"}
 
 
 @pytest.mark.parametrize("path", FIXTURES, ids=IDS)
 def test_oracle_reproduces_fixture(path):
     g = load(path)
     A = oracle.OracleH2(g["P"], g["Peval"], str(g["kernel"]), float(g["l"]), int(g["deg"]),
-                        float(g["eta"]))
+                        float(g["eta"]), scheme=SCHEME[g["interpolator"]])
     assert np.array_equal(A.indices(0), g["rindices"])
     assert np.array_equal(A.indices(1), g["cindices"])
     row, col, lr = A.blocks()
@@ -42,10 +55,13 @@ def test_oracle_reproduces_fixture(path):
 
 @pytest.mark.parametrize("path", FIXTURES, ids=IDS)
 def test_planner_reproduces_fixture(path):
-    import fmca_amd
     g = load(path)
-    K = fmca_amd.CovarianceKernel(str(g["kernel"]), float(g["l"]))
-    H = fmca_amd.H2kernelActor(K, g["Peval"], g["P"], int(g["deg"]), float(g["eta"]))
+    H = actor(g)
+    if g["interpolator"] == "total_degree":
+        # DeMarchi nodes bit for bit, invV to rounding
+        ip = H.interpolation()
+        assert np.array_equal(ip["Xi"], g["Xi"])
+        assert np.abs(ip["invV"] - g["invV"]).max() <= 1e-13
     assert np.array_equal(H.rindices(), g["rindices"])
     assert np.array_equal(H.cindices(), g["cindices"])
     row, col, lr = H.blocks()
@@ -60,8 +76,7 @@ def test_gpu_action_matches_fixture(path):
     import fmca_amd
     g = load(path)
     print(f"[case] golden {os.path.basename(path)}", flush=True)
-    K = fmca_amd.CovarianceKernel(str(g["kernel"]), float(g["l"]))
-    H = fmca_amd.H2kernelActor(K, g["Peval"], g["P"], int(g["deg"]), float(g["eta"]))
+    H = actor(g)
     x = torch.from_numpy(g["rhs"]).cuda()
     y64 = H.action_device(x, precision=fmca_amd.FP64).cpu().numpy()
     y32 = H.action_device(x, precision=fmca_amd.FP32).cpu().numpy()

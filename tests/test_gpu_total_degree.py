@@ -158,3 +158,26 @@ def test_total_degree_gradients_vs_dense_autograd():
     assert efwd < 1e-5
     bound = 20 * efwd
     assert ea < bound and el < bound and eP < bound
+
+
+def test_multipole_function_evaluator():
+    """MultipoleFunctionEvaluator (MultipoleFunctionEvaluator.h:17-67):
+    evaluate(P, Peval, C) = the total-degree H2 action K(Peval, P) C.  At the
+    default eta = 0 no block is admissible, so the result is the dense
+    product; at eta = 0.8 it is the oracle's H2 action."""
+    import fmca_amd
+    print("[case] multipole function evaluator", flush=True)
+    P = uniform_points(3000, 2, seed=41)
+    Pe = uniform_points(1500, 2, seed=42)
+    C = np.random.default_rng(43).standard_normal((3000, 3))
+    K = fmca_amd.CovarianceKernel("MATERN52", 0.3)
+    ev = fmca_amd.MultipoleFunctionEvaluator(K, P, Pe)
+    dense = oracle.dense_action(P, Pe, "MATERN52", 0.3, C)
+    assert rel_err(ev.evaluate(P, Pe, C), dense) <= 1e-13
+    ev = fmca_amd.MultipoleFunctionEvaluator(K, P, Pe, 0.8, 4)
+    ref = oracle.OracleH2(P, Pe, "MATERN52", 0.3, 4, 0.8, scheme="total_degree").action(C)
+    assert rel_err(ev.evaluate(P, Pe, C), ref) <= FP64_TOL
+    # other evaluation points: planned anew
+    Pe2 = uniform_points(1000, 2, seed=44)
+    ref2 = oracle.OracleH2(P, Pe2, "MATERN52", 0.3, 4, 0.8, scheme="total_degree").action(C)
+    assert rel_err(ev.evaluate(P, Pe2, C), ref2) <= FP64_TOL

@@ -118,6 +118,46 @@ def test_c_abi_create_plan_without_device():
                               ptr, 3, ctypes.c_double(0.8), ctypes.byref(h)) == 3
 
 
+def test_c_abi_total_degree_plan_without_device():
+    """fmca_h2_create_interp with FMCA_INTERP_TOTAL_DEGREE: q = binom(deg+d, d),
+    the tree's minimum cluster size is that q, and fmca_h2_interpolation
+    returns the oracle's DeMarchi nodes and invV."""
+    lib = ctypes.CDLL(LIB)
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("ktype", ctypes.c_char_p), ("l", ctypes.c_double), ("c", ctypes.c_double),
+                    ("nu", ctypes.c_double)]
+    P = uniform_points(3000, 3, seed=4)
+    h = ctypes.c_void_p()
+    desc = Desc(b"EXPONENTIAL", 0.5, 1.0, 1.0)
+    ptr = P.ctypes.data_as(ctypes.c_void_p)
+    assert lib.fmca_h2_create_interp(ctypes.byref(desc), 3, ctypes.c_int64(3000), ptr,
+                                     ctypes.c_int64(3000), ptr, 4, ctypes.c_double(0.8), 1,
+                                     ctypes.byref(h)) == 0
+    kind, q = ctypes.c_int(), ctypes.c_int()
+    assert lib.fmca_h2_interpolation(h, ctypes.byref(kind), ctypes.byref(q), None, None,
+                                     None) == 0
+    assert kind.value == 1 and q.value == 35
+    mi = np.empty((35, 3), dtype=np.int32)
+    Xi = np.empty((3, 35), order="F")
+    invV = np.empty((35, 35), order="F")
+    assert lib.fmca_h2_interpolation(h, None, None, mi.ctypes.data_as(ctypes.c_void_p),
+                                     Xi.ctypes.data_as(ctypes.c_void_p),
+                                     invV.ctypes.data_as(ctypes.c_void_p)) == 0
+    omi, oXi, _, oinvV = oracle.interp_total_degree(3, 4)
+    assert np.array_equal(mi, omi) and np.array_equal(Xi, oXi)
+    assert np.abs(invV - oinvV).max() < 1e-13
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 4, 0.8, scheme="total_degree")
+    perm = np.empty(3000, dtype=np.uint32)
+    assert lib.fmca_h2_indices(h, 0, perm.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.array_equal(perm, A.indices(0))
+    assert lib.fmca_h2_destroy(h) == 0
+    # unknown interpolator id
+    assert lib.fmca_h2_create_interp(ctypes.byref(desc), 3, ctypes.c_int64(3000), ptr,
+                                     ctypes.c_int64(3000), ptr, 4, ctypes.c_double(0.8), 7,
+                                     ctypes.byref(h)) == 1
+
+
 def test_python_errors():
     P = uniform_points(500, 2, seed=1)
     K = fmca_amd.CovarianceKernel("GAUSSIAN", 0.3)
