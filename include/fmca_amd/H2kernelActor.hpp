@@ -3,10 +3,12 @@
 // Drop-in for FMCA::H2kernelActor<Covariance, Moments, H2ClusterTree,
 // MatrixEvaluator> (FMCA/src/H2Matrix/H2kernelActor.h:17-94): the same
 // constructor (kernel, Peval, P, mpole_deg, eta) and `M action(const M &rhs)`.
-// The template arguments after the kernel type are accepted and ignored (the
-// moments are tensor-product Chebyshev, the tree a ClusterTree with
-// CardinalityBisection, the evaluator unsymmetric Nystrom -- the hot-path
-// instantiation of FMCA).
+// The second template argument is the Moments type: FMCA's
+// NystromMoments<TotalDegreeInterpolator> (recognised by its interpolator's
+// idcs(), TotalDegreeInterpolator.h:56) or fmca_amd::TotalDegreeMoments selects
+// the total-degree form, anything else the tensor-product Chebyshev moments.
+// The arguments after it are accepted and ignored (the tree is a ClusterTree
+// with CardinalityBisection, the evaluator unsymmetric Nystrom).
 //
 // Matrix types: anything column-major with rows(), cols(), data() and a
 // (rows, cols) constructor -- Eigen::MatrixXd qualifies.  Kernel types:
@@ -21,6 +23,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../fmca_amd.h"
@@ -63,7 +67,26 @@ class CovarianceKernel {
 
 enum class Precision : int { FP64 = FMCA_PRECISION_FP64, FP32 = FMCA_PRECISION_FP32 };
 
-template <typename Covariance = CovarianceKernel, typename... FmcaPolicies>
+// Moments tags for callers without FMCA's headers.
+struct TensorProductMoments {};
+struct TotalDegreeMoments {
+  struct Interpolator {
+    int idcs() const { return 0; }
+  };
+  Interpolator interp() const { return {}; }
+};
+
+// FMCA_INTERP_* of a Moments type: total degree when moments.interp().idcs()
+// exists (TotalDegreeInterpolator), tensor product otherwise.
+template <typename Moments, typename = void>
+struct moments_interpolator : std::integral_constant<int, FMCA_INTERP_TENSOR_PRODUCT> {};
+template <typename Moments>
+struct moments_interpolator<Moments,
+                            std::void_t<decltype(std::declval<const Moments &>().interp().idcs())>>
+    : std::integral_constant<int, FMCA_INTERP_TOTAL_DEGREE> {};
+
+template <typename Covariance = CovarianceKernel, typename Moments = TensorProductMoments,
+          typename... FmcaPolicies>
 class H2kernelActor {
  public:
   template <typename PointMatrix>
@@ -75,9 +98,9 @@ class H2kernelActor {
                           (double)kernel.nu()};
     if (Peval.rows() != P.rows()) throw Error(FMCA_ERR_INVALID_ARGUMENT, "point dimensions");
     fmca_h2_actor *h = nullptr;
-    check(fmca_h2_create(&desc, (int)P.rows(), m_, Peval.data(), n_, P.data(), (int)mpole_deg,
-                         eta, &h),
-          "fmca_h2_create");
+    check(fmca_h2_create_interp(&desc, (int)P.rows(), m_, Peval.data(), n_, P.data(),
+                                (int)mpole_deg, eta, moments_interpolator<Moments>::value, &h),
+          "fmca_h2_create_interp");
     h_.reset(h, [](fmca_h2_actor *p) { fmca_h2_destroy(p); });
   }
 

@@ -2,7 +2,9 @@
 // FMCA::H2kernelActor: construct from (kernel, Peval, P, deg, eta), call
 // action(rhs) on a column-major matrix.  Reads points and rhs from a binary
 // file, writes the result; "plan" mode only builds the actor.
-//   facade_check plan|action <in.bin> <out.bin>
+//   facade_check plan|action|plan_td|action_td <in.bin> <out.bin>
+// (the _td modes instantiate the actor with a Moments type shaped like
+// FMCA's NystromMoments<TotalDegreeInterpolator>)
 // in.bin: int64 dim, n, nrhs, deg; double eta, l; char[16] kernel;
 //         double P[dim*n] (column-major); double rhs[n*nrhs]
 #include <cstdio>
@@ -23,6 +25,34 @@ struct Mat {
   long r_, c_;
   std::vector<double> v_;
 };
+
+// stand-ins shaped like FMCA::NystromMoments<TotalDegreeInterpolator> and
+// NystromMoments<TensorProductInterpolator>: only interp().idcs() is looked at
+struct TDInterp {
+  struct Set {};
+  Set idcs() const { return {}; }
+};
+struct TPInterp {};
+template <typename I>
+struct NystromMomentsLike {
+  const I &interp() const { return i_; }
+  I i_;
+};
+
+template <typename Actor>
+int run(Actor &act, const char *mode, const Mat &rhs, const char *path) {
+  std::ofstream out(path, std::ios::binary);
+  if (std::strncmp(mode, "plan", 4) == 0) {
+    auto idx = act.indices(FMCA_COLS);
+    auto st = act.statistics();
+    out.write((const char *)idx.data(), sizeof(uint32_t) * idx.size());
+    out.write((const char *)st.data(), sizeof(double) * st.size());
+    return 0;
+  }
+  Mat y = act.action(rhs);
+  out.write((const char *)y.data(), sizeof(double) * y.rows() * y.cols());
+  return 0;
+}
 
 // a kernel type of the caller's own, with the FMCA::CovarianceKernel accessors
 struct MyKernel {
@@ -50,19 +80,18 @@ int main(int argc, char **argv) {
   if (!in) return 3;
   try {
     const MyKernel kernel{std::string(kname), par[1]};
-    // FMCA's four template arguments are accepted (and ignored) after the kernel type
-    fmca_amd::H2kernelActor<MyKernel, struct Moments, struct H2ClusterTree, struct Evaluator>
-        act(kernel, P, P, (unsigned)hdr[3], par[0]);
-    std::ofstream out(argv[3], std::ios::binary);
-    if (std::strcmp(argv[1], "plan") == 0) {
-      auto idx = act.indices(FMCA_COLS);
-      auto st = act.statistics();
-      out.write((const char *)idx.data(), sizeof(uint32_t) * idx.size());
-      out.write((const char *)st.data(), sizeof(double) * st.size());
-      return 0;
+    const bool td = std::strstr(argv[1], "_td") != nullptr;
+    // FMCA's four template arguments: kernel, Moments, tree, evaluator
+    if (td) {
+      fmca_amd::H2kernelActor<MyKernel, NystromMomentsLike<TDInterp>, struct H2ClusterTree,
+                              struct Evaluator>
+          act(kernel, P, P, (unsigned)hdr[3], par[0]);
+      return run(act, argv[1], rhs, argv[3]);
     }
-    Mat y = act.action(rhs);
-    out.write((const char *)y.data(), sizeof(double) * y.rows() * y.cols());
+    fmca_amd::H2kernelActor<MyKernel, NystromMomentsLike<TPInterp>, struct H2ClusterTree,
+                            struct Evaluator>
+        act(kernel, P, P, (unsigned)hdr[3], par[0]);
+    return run(act, argv[1], rhs, argv[3]);
   } catch (const fmca_amd::Error &e) {
     std::cerr << e.what() << std::endl;
     return 10 + (int)e.status();

@@ -14,7 +14,9 @@ BIN = os.path.join(HERE, "_build", "facade_check")
 
 
 def _binary():
-    if not os.path.exists(BIN):
+    src = [os.path.join(HERE, "facade_check.cpp"),
+           os.path.join(ROOT, "include", "fmca_amd", "H2kernelActor.hpp")]
+    if not os.path.exists(BIN) or os.path.getmtime(BIN) < max(map(os.path.getmtime, src)):
         subprocess.run(["bash", os.path.join(HERE, "build.sh")], check=True)
     return BIN
 
@@ -45,6 +47,20 @@ def test_facade_plan_matches_oracle(tmp_path):
     assert st[2] == counts["far_blocks"] and st[3] == counts["near_blocks"]
 
 
+def test_facade_total_degree_moments_select_total_degree(tmp_path):
+    """A Moments argument shaped like NystromMoments<TotalDegreeInterpolator>
+    gives the total-degree plan (leaves of q = 20 .. 39 points at deg 3)."""
+    P = uniform_points(5000, 3, seed=21)
+    _write(tmp_path / "in.bin", P, np.zeros((5000, 1)), 3, 0.8, "matern32", 0.5)
+    r = subprocess.run([_binary(), "plan_td", str(tmp_path / "in.bin"),
+                        str(tmp_path / "out.bin")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    idx = np.frombuffer((tmp_path / "out.bin").read_bytes()[:4 * 5000], dtype=np.uint32)
+    A = oracle.OracleH2(P, P, "MATERN32", 0.5, 3, 0.8, scheme="total_degree")
+    assert np.array_equal(idx, A.indices(1))
+    assert not np.array_equal(idx, oracle.OracleH2(P, P, "MATERN32", 0.5, 3, 0.8).indices(1))
+
+
 def test_facade_reports_errors(tmp_path):
     P = uniform_points(100, 3, seed=1)
     _write(tmp_path / "in.bin", P, np.zeros((100, 1)), 3, 0.8, "NOSUCHKERNEL", 0.5)
@@ -63,4 +79,17 @@ def test_facade_action_matches_oracle(tmp_path):
     assert r.returncode == 0, r.stderr
     y = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(5, 6000).T
     ref = oracle.OracleH2(P, P, "EXPONENTIAL", 0.4, 4, 0.8).action(rhs)
+    assert rel_err(y, ref) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_facade_total_degree_action_matches_oracle(tmp_path):
+    P = uniform_points(5000, 2, seed=24)
+    rhs = np.asfortranarray(np.random.default_rng(25).standard_normal((5000, 3)))
+    _write(tmp_path / "in.bin", P, rhs, 4, 0.8, "exponential", 1.0)
+    r = subprocess.run([_binary(), "action_td", str(tmp_path / "in.bin"),
+                        str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    y = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(3, 5000).T
+    ref = oracle.OracleH2(P, P, "EXPONENTIAL", 1.0, 4, 0.8, scheme="total_degree").action(rhs)
     assert rel_err(y, ref) <= 1e-12
