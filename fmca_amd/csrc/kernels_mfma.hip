@@ -309,12 +309,9 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
-  // source node coordinates: [D][kMaxP1] per-axis values (tensor) or
-  // [D][qpad] per node (total degree)
-  const int ys = ip.td ? qpad : kMaxP1;
-  int *bidx = reinterpret_cast<int *>(smem_raw);                     // [qpad]
-  T *Y = reinterpret_cast<T *>(bidx + ((qpad + 3) & ~3));            // [D][ys]
-  T *Ms = Y + D * ys;                                                // [kChunk][NCP]
+  T *U = reinterpret_cast<T *>(smem_raw);  // [D][qpad] unit coordinates of the nodes
+  T *Yc = U + D * qpad;                    // [D][kChunk] coordinates of the chunk's rows
+  T *Ms = Yc + D * kChunk;                 // [kChunk][NCP] moments of the chunk's rows
   T *Red = Ms + kChunk * NCP;  // [2 kRT * 16][NC] partial sums of the K-split (q <= 128)
   const int lane = threadIdx.x & 63;
   // wave index as a scalar: the row-tile guards below are then uniform branches
@@ -332,16 +329,19 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   const int kspl = R <= kRT ? 4 : (R <= 2 * kRT ? 2 : 1);
   const int wrows = kWaves / kspl, rg = wave % wrows, sg = wave / wrows;
   const int ks0 = 4 * sg, kstep = 4 * kspl;
-  // bidx[beta]: the LDS offsets in Y of source node beta's coordinates,
-  // 10 bits per axis
-  for (int e = threadIdx.x; e < qpad; e += blockDim.x) {
-    const int be = e < q ? e : 0;
-    const int pk = basis_index<T, D>(be, ip);
-    int off = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-      off |= (d * ys + (ip.td ? be : unpack(pk, d))) << (10 * d);
-    bidx[e] = off;
+  // The K dimension runs over the far sources' nodes.  A chunk of kChunk
+  // rows holds G = kChunk / qpad whole sources when that is 2 or more (small
+  // q: fewer stages and barriers per source), else kChunk nodes of one
+  // source.  Row bl of a chunk is node bl % qpad of the chunk's source
+  // bl / qpad (G >= 2) or node be + bl (G = 1); rows past a source's q nodes
+  // or past the list carry zero moments and the coordinates of a real node.
+  const int G = max(1, kChunk / qpad);
+  const float rqpad = 1.0f / (float)qpad;  // bl / qpad = floor((bl + 0.5) / qpad), bl < 256
+  // bl / qpad = (bl * qmagic) >> 16 for bl < 128 and qpad <= 64 (the G >= 2 case)
+  const int qmagic = (65536 + qpad - 1) / qpad;
+  for (int e = threadIdx.x; e < D * qpad; e += blockDim.x) {
+    const int d = e / qpad, b = e - d * qpad, bb = b < q ? b : 0;
+    U[e] = ip.td ? ip.unodes[d * q + bb] : ip.xi[unpack(basis_index<T, D>(bb, ip), d)];
   }
   // this lane's target node per row tile
   T x[kRT][D];
@@ -358,12 +358,13 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
       x[rt][d] = u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d];
     }
   }
-  // total degree: source s's node coordinates into Y
-  auto stage_td_nodes = [&](int s) {
-    for (int e = threadIdx.x; e < D * q; e += blockDim.x) {
-      const int d = e / q, k = e - d * q;
-      Y[d * ys + k] = ip.unodes[e] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
-    }
+  __syncthreads();  // U
+  // source slot and node of chunk row bl; node -1: a padding row
+  auto chunk_row = [&](int bl, int be, int nsrc, int &slot, int &node) {
+    slot = G > 1 ? (int)(((float)bl + 0.5f) * rqpad) : 0;
+    node = be + bl - slot * qpad;
+    if (slot >= nsrc || node >= q) node = -1;
+    if (slot >= nsrc) slot = 0;
   };
   for (int rc = 0; rc < nrhs; rc += NC) {
     acc_t acc[kRT][NT];
@@ -376,15 +377,14 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
     for (int nt = 0; nt < NT; ++nt) tv[nt] = kp.tv[min((rc >> 4) + nt, kMaxVariantGroups - 1)];
     // column tiles of this pass that hold rhs columns (the rest are skipped)
     const int ntl = min(NT, (nrhs - rc + 15) >> 4);
-    // K-steps of one staged chunk of source nodes [be, be + nbeta)
-    auto compute_chunk = [&](int be, int nbeta) {
+    // K-steps of the staged chunk, rows [0, nrows)
+    auto compute_chunk = [&](int nrows) {
 #pragma unroll 2
-      for (int ks = ks0; ks < nbeta; ks += kstep) {
+      for (int ks = ks0; ks < nrows; ks += kstep) {
         const int bl = ks + lk;
-        const int pk = bidx[be + bl];
         T y[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) y[d] = Y[(pk >> (10 * d)) & 1023];
+        for (int d = 0; d < D; ++d) y[d] = Yc[d * kChunk + bl];
         T b[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) b[nt] = Ms[bl * NCP + nt * 16 + lr];
@@ -412,80 +412,109 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
         }
       }
     };
-    if constexpr (sizeof(T) == 4) {
-      // FP32: the (far source, node chunk) pairs of the list are staged
-      // through registers one chunk ahead, so that the global loads of chunk
-      // c+1 overlap the MFMA work of chunk c.
-      constexpr int KM = kChunk * NC / 256;  // staged moments per thread
-      static_assert(kChunk * NC % 256 == 0 && kMaxDim * kMaxP1 <= 256, "staging layout");
-      T pyn = T(0), pm[KM];
-      int f = f0, nbe = 0, pbe = 0;  // next chunk to fetch: (f, nbe); fetched: pbe
-      int psrc = 0;                  // source of the fetched chunk
-      auto fetch = [&]() {
-        const int s = pat.far_src[f];
-        pbe = nbe;
-        psrc = s;
-        if (nbe == 0 && !ip.td && threadIdx.x < D * P1) {
-          const int d = threadIdx.x / P1, b = threadIdx.x % P1;
-          pyn = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+    // Chunks are fetched into registers (coordinates py, moments pm) and then
+    // written to LDS between two barriers.  FP32 fetches the next chunk right
+    // after the barriers, so its global loads overlap the MFMA work of the
+    // current one; FP64 (one wave per SIMD) fetches just before staging, as a
+    // lookahead measured slower there.
+    constexpr int KM = kChunk * NC / 256;          // staged moments per thread
+    constexpr int KY = (D * kChunk + 255) / 256;  // staged coordinates per thread
+    static_assert(kChunk * NC % 256 == 0, "staging layout");
+    T py[KY], pm[KM];
+    int f = f0, nbe = 0;  // next chunk: sources from f, nodes from nbe (G = 1)
+    int pn = 0;           // rows of the fetched chunk
+    auto fetch = [&]() {
+      const int nsrc = G > 1 ? min(G, f1 - f) : 1;
+      const int be = nbe;
+      pn = G > 1 ? nsrc * qpad : min(kChunk, qpad - be);
+      if (G > 1) {
+        // several sources: per-row source for the coordinates; the rows of
+        // one wave-wide moment load (64 / NC consecutive rows, starting at a
+        // multiple of 64 / NC) never straddle a source since qpad is a
+        // multiple of 4, so that load's source is wave-uniform
+#pragma unroll
+        for (int k = 0; k < KY; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          const int d = e / kChunk, bl = e % kChunk;
+          if (e < D * kChunk) {
+            int slot, node;
+            chunk_row(bl, 0, nsrc, slot, node);
+            const int s = pat.far_src[f + slot];
+            py[k] = U[d * qpad + (node < 0 ? 0 : node)] * cols.bbsize[s * D + d] +
+                    cols.bbmin[s * D + d];
+          }
         }
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
           const int e = threadIdx.x + 256 * k;
           const int bl = e / NC, r = e % NC;
-          const int beta = nbe + bl;
-          pm[k] =
-              (beta < q && rc + r < nrhs) ? mom[((int64_t)s * q + beta) * nrhs + rc + r] : T(0);
+          const int slot = (__builtin_amdgcn_readfirstlane(bl) * qmagic) >> 16;
+          const int node = bl - slot * qpad;
+          const T *ms = mom + (int64_t)pat.far_src[f + min(slot, nsrc - 1)] * q * nrhs;
+          pm[k] = (slot < nsrc && node < q && rc + r < nrhs) ? ms[node * nrhs + rc + r] : T(0);
         }
+      } else {
+        // one source, nodes [be, be + kChunk)
+        const int s = pat.far_src[f];
+        const T *ms = mom + (int64_t)s * q * nrhs;
+#pragma unroll
+        for (int k = 0; k < KY; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          const int d = e / kChunk, bl = e % kChunk;
+          if (e < D * kChunk) {
+            const int node = be + bl < q ? be + bl : 0;
+            py[k] = U[d * qpad + node] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          const int bl = e / NC, r = e % NC;
+          const int node = be + bl;
+          pm[k] = (node < q && rc + r < nrhs) ? ms[node * nrhs + rc + r] : T(0);
+        }
+      }
+      if (G > 1) {
+        f += nsrc;
+      } else {
         nbe += kChunk;
         if (nbe >= qpad) {
           ++f;
           nbe = 0;
         }
-      };
+      }
+    };
+    auto stage = [&]() {
+#pragma unroll
+      for (int k = 0; k < KY; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        if (e < D * kChunk) Yc[e] = py[k];
+      }
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        Ms[(e / NC) * NCP + e % NC] = pm[k];
+      }
+    };
+    if constexpr (sizeof(T) == 4) {
       bool more = f < f1;
       if (more) fetch();
       while (more) {
-        const int be = pbe;
+        const int nrows = pn;
         __syncthreads();
-        if (be == 0) {
-          if (ip.td)
-            stage_td_nodes(psrc);
-          else if (threadIdx.x < D * P1)
-            Y[(threadIdx.x / P1) * kMaxP1 + threadIdx.x % P1] = pyn;
-        }
-#pragma unroll
-        for (int k = 0; k < KM; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          Ms[(e / NC) * NCP + e % NC] = pm[k];
-        }
+        stage();
         __syncthreads();
         more = f < f1;
         if (more) fetch();
-        compute_chunk(be, min(kChunk, qpad - be));
+        compute_chunk(nrows);
       }
     } else {
-      // FP64 (one wave per SIMD): a lookahead measured slower; stage in place
-      for (int f = f0; f < f1; ++f) {
-        const int s = pat.far_src[f];
-        for (int be = 0; be < qpad; be += kChunk) {
-          __syncthreads();
-          if (be == 0 && ip.td) stage_td_nodes(s);
-          if (be == 0 && !ip.td)
-            for (int e = threadIdx.x; e < D * P1; e += blockDim.x) {
-              const int d = e / P1, b = e % P1;
-              Y[d * kMaxP1 + b] = ip.xi[b] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
-            }
-          for (int e = threadIdx.x; e < kChunk * NC; e += blockDim.x) {
-            const int bl = e / NC, r = e % NC;
-            const int beta = be + bl;
-            Ms[bl * NCP + r] = (beta < q && rc + r < nrhs)
-                                   ? mom[((int64_t)s * q + beta) * nrhs + rc + r]
-                                   : T(0);
-          }
-          __syncthreads();
-          compute_chunk(be, min(kChunk, qpad - be));
-        }
+      while (f < f1) {
+        fetch();
+        __syncthreads();
+        stage();
+        __syncthreads();
+        compute_chunk(pn);
       }
     }
     if (kspl > 1) {
@@ -557,12 +586,12 @@ size_t leaf_output_mfma_lds(int D, int P1, int q) {
 
 template <typename T, int NT>
 size_t coupling_mfma_lds(int D, int q, bool td) {
+  (void)td;  // both schemes stage per-row coordinates
   constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
-  const size_t ys = td ? (size_t)qpad : (size_t)kMaxP1;
   const size_t red = q <= 2 * kRT * 16 ? (size_t)2 * kRT * 16 * 16 * NT : 0;  // K-split partials
-  return (size_t)((qpad + 3) & ~3) * sizeof(int) +
-         ((size_t)D * ys + (size_t)kChunk * (16 * NT + 16) + red) * sizeof(T);
+  return ((size_t)D * qpad + (size_t)D * kChunk + (size_t)kChunk * (16 * NT + 16) + red) *
+         sizeof(T);
 }
 
 // Specialised entry code for the common kernels, a run-time switch otherwise.
