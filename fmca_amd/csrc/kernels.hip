@@ -415,18 +415,28 @@ __global__ __launch_bounds__(kBlock) void k_downward_transfer(DevTreeView<T> tr,
 // to the polynomial basis once per cluster after it.
 
 // 3'. upward: mom[t] = E_s0 mom[s0] + E_s1 mom[s1] for the inner clusters of
-// one level; the sons' moments staged [2][q][NRC] in LDS, E read from L2.
+// one level; the sons' moments staged [2][q][NRC] in LDS, the two E matrices
+// too when stage_e (else read through L2).
 template <typename T, int NRC>
 __global__ __launch_bounds__(kBlock) void k_upward_transfer_dense(DevTreeView<T> tr, int q,
-                                                                  int cbeg, int nrhs,
+                                                                  int cbeg, int nrhs, int stage_e,
                                                                   T *__restrict__ mom) {
   extern __shared__ unsigned char smem_raw[];
   T *S = reinterpret_cast<T *>(smem_raw);  // [2][q][NRC]
+  T *Es = S + 2 * q * NRC;                 // [2][q][q] when stage_e
   const int c = cbeg + blockIdx.x;
   const int s0 = tr.son0[c], s1 = tr.son1[c];
   if (s0 < 0) return;
   const T *E0 = tr.transfer + (int64_t)s0 * q * q;
   const T *E1 = tr.transfer + (int64_t)s1 * q * q;
+  if (stage_e) {
+    for (int e = threadIdx.x; e < q * q; e += blockDim.x) {
+      Es[e] = E0[e];
+      Es[q * q + e] = E1[e];
+    }
+    E0 = Es;
+    E1 = Es + q * q;
+  }
   for (int rc = 0; rc < nrhs; rc += NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
@@ -447,16 +457,22 @@ __global__ __launch_bounds__(kBlock) void k_upward_transfer_dense(DevTreeView<T>
 }
 
 // 5'. downward: tlhs[c] += E_c^T tlhs[parent(c)] for the clusters of one
-// level >= 1, the father's coefficients staged [q][NRC] in LDS.
+// level >= 1, the father's coefficients staged [q][NRC] in LDS, E_c too when
+// stage_e.
 template <typename T, int NRC>
 __global__ __launch_bounds__(kBlock) void k_downward_transfer_dense(DevTreeView<T> tr, int q,
-                                                                    int cbeg, int nrhs,
+                                                                    int cbeg, int nrhs, int stage_e,
                                                                     T *__restrict__ tlhs) {
   extern __shared__ unsigned char smem_raw[];
   T *S = reinterpret_cast<T *>(smem_raw);  // [q][NRC]
+  T *Es = S + q * NRC;                     // [q][q] when stage_e
   const int c = cbeg + blockIdx.x;
   const int p = tr.parent[c];
   const T *E = tr.transfer + (int64_t)c * q * q;
+  if (stage_e) {
+    for (int e = threadIdx.x; e < q * q; e += blockDim.x) Es[e] = E[e];
+    E = Es;
+  }
   for (int rc = 0; rc < nrhs; rc += NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
@@ -482,11 +498,16 @@ __global__ __launch_bounds__(kBlock) void k_downward_transfer_dense(DevTreeView<
 template <typename T, int NRC>
 __global__ __launch_bounds__(kBlock) void k_cluster_basis_change(int q, const T *__restrict__ invV,
                                                                  int transpose, int nrhs,
-                                                                 T *__restrict__ x) {
+                                                                 int stage_e, T *__restrict__ x) {
   extern __shared__ unsigned char smem_raw[];
   T *S = reinterpret_cast<T *>(smem_raw);  // [q][NRC]
+  T *Vs = S + q * NRC;                     // [q][q] copy of invV when stage_e
   const int c = blockIdx.x;
   const int64_t ia = transpose ? q : 1, ib = transpose ? 1 : q;  // M(a, b) = invV[a ia + b ib]
+  if (stage_e) {
+    for (int e = threadIdx.x; e < q * q; e += blockDim.x) Vs[e] = invV[e];
+    invV = Vs;
+  }
   for (int rc = 0; rc < nrhs; rc += NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
@@ -801,33 +822,43 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
     return 1;
   };
   auto dense_lds = [&](int arrays, int nrc) { return (size_t)arrays * q * nrc * sizeof(T); };
+  // the q x q matrices (E, invV) go to LDS too where that fits
+  auto mats_fit = [&](int arrays, int nrc) {
+    return dense_lds(arrays, nrc) + (size_t)arrays * q * q * sizeof(T) <= kLdsLimit;
+  };
+  auto with_mats = [&](int arrays, int nrc) {
+    return dense_lds(arrays, nrc) +
+           (mats_fit(arrays, nrc) ? (size_t)arrays * q * q * sizeof(T) : 0);
+  };
   if (ip.td) {
     const int nu = dense_nrc(2);
     if (dense_lds(2, nu) > kLdsLimit) return hipErrorInvalidConfiguration;
+    const int se = mats_fit(2, nu) ? 1 : 0;
     for (int L = lcols.depth - 1; L >= 0; --L) {
       const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
       if (nu == 32)
         hipLaunchKernelGGL((k_upward_transfer_dense<T, 32>), dim3(nc), dim3(kBlock),
-                           dense_lds(2, 32), stream, cols, q, cb, nrhs, buf.mom);
+                           with_mats(2, 32), stream, cols, q, cb, nrhs, se, buf.mom);
       else if (nu == 8)
         hipLaunchKernelGGL((k_upward_transfer_dense<T, 8>), dim3(nc), dim3(kBlock),
-                           dense_lds(2, 8), stream, cols, q, cb, nrhs, buf.mom);
+                           with_mats(2, 8), stream, cols, q, cb, nrhs, se, buf.mom);
       else
         hipLaunchKernelGGL((k_upward_transfer_dense<T, 1>), dim3(nc), dim3(kBlock),
-                           dense_lds(2, 1), stream, cols, q, cb, nrhs, buf.mom);
+                           with_mats(2, 1), stream, cols, q, cb, nrhs, se, buf.mom);
     }
   }
   auto basis_change = [&](int ncl, int transpose, T *x) {
     const int nb = dense_nrc(1);
+    const int se = mats_fit(1, nb) ? 1 : 0;
     if (nb == 32)
       hipLaunchKernelGGL((k_cluster_basis_change<T, 32>), dim3(ncl), dim3(kBlock),
-                         dense_lds(1, 32), stream, q, ip.invV, transpose, nrhs, x);
+                         with_mats(1, 32), stream, q, ip.invV, transpose, nrhs, se, x);
     else if (nb == 8)
       hipLaunchKernelGGL((k_cluster_basis_change<T, 8>), dim3(ncl), dim3(kBlock),
-                         dense_lds(1, 8), stream, q, ip.invV, transpose, nrhs, x);
+                         with_mats(1, 8), stream, q, ip.invV, transpose, nrhs, se, x);
     else
       hipLaunchKernelGGL((k_cluster_basis_change<T, 1>), dim3(ncl), dim3(kBlock),
-                         dense_lds(1, 1), stream, q, ip.invV, transpose, nrhs, x);
+                         with_mats(1, 1), stream, q, ip.invV, transpose, nrhs, se, x);
   };
   const int nrc_up = transfer_nrc(3), nrc_dn = transfer_nrc(2);
   if (!ip.td && (transfer_lds(3, nrc_up) > kLdsLimit || transfer_lds(2, nrc_dn) > kLdsLimit))
@@ -860,17 +891,18 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   mark(PH_DOWNWARD_TRANSFER);
   if (ip.td) {
     const int nd = dense_nrc(1);
+    const int se = mats_fit(1, nd) ? 1 : 0;
     for (int L = 1; L <= lrows.depth; ++L) {
       const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
       if (nd == 32)
         hipLaunchKernelGGL((k_downward_transfer_dense<T, 32>), dim3(nc), dim3(kBlock),
-                           dense_lds(1, 32), stream, rows, q, cb, nrhs, buf.tlhs);
+                           with_mats(1, 32), stream, rows, q, cb, nrhs, se, buf.tlhs);
       else if (nd == 8)
         hipLaunchKernelGGL((k_downward_transfer_dense<T, 8>), dim3(nc), dim3(kBlock),
-                           dense_lds(1, 8), stream, rows, q, cb, nrhs, buf.tlhs);
+                           with_mats(1, 8), stream, rows, q, cb, nrhs, se, buf.tlhs);
       else
         hipLaunchKernelGGL((k_downward_transfer_dense<T, 1>), dim3(nc), dim3(kBlock),
-                           dense_lds(1, 1), stream, rows, q, cb, nrhs, buf.tlhs);
+                           with_mats(1, 1), stream, rows, q, cb, nrhs, se, buf.tlhs);
     }
   }
   for (int L = 1; !ip.td && L <= lrows.depth; ++L) {
