@@ -108,7 +108,7 @@ struct DevTreeReals {
 
 struct DevPattern {
   DevBuf far_ptr, far_src, far_order, far_task, far_split;
-  DevBuf near_ptr, near_src, near_order, near_task, near_split;
+  DevBuf near_ptr, near_src, near_run, near_order, near_task, near_split;
   DevPatternView view(const Pattern &p) const {
     DevPatternView v;
     v.far_ptr = far_ptr.as<int32_t>();
@@ -121,6 +121,7 @@ struct DevPattern {
     v.nnear_leaves = (int)p.near_ptr.size() - 1;
     v.near_ptr = near_ptr.as<int32_t>();
     v.near_src = near_src.as<int32_t>();
+    v.near_run = near_run.as<int32_t>();
     v.nnear_order = (int)p.near_order.size();
     v.near_order = near_order.as<int32_t>();
     v.near_task = near_task.as<int32_t>();
@@ -137,6 +138,7 @@ struct DevPattern {
     if ((e = far_split.upload(p.far_split)) != hipSuccess) return e;
     if ((e = near_ptr.upload(p.near_ptr)) != hipSuccess) return e;
     if ((e = near_src.upload(p.near_src)) != hipSuccess) return e;
+    if ((e = near_run.upload(p.near_run)) != hipSuccess) return e;
     if ((e = near_order.upload(p.near_order)) != hipSuccess) return e;
     if ((e = near_task.upload(p.near_task)) != hipSuccess) return e;
     return near_split.upload(p.near_split);

@@ -46,6 +46,7 @@ struct DevPatternView {
   int nnear_leaves = 0;
   const int32_t *near_ptr = nullptr;    // nleaves_target + 1
   const int32_t *near_src = nullptr;
+  const int32_t *near_run = nullptr;    // contiguous sources from each entry on
   int nnear_order = 0;                  // leaf output grid size
   const int32_t *near_order = nullptr;  // workgroup -> leaf output task (-1: none)
   const int32_t *near_task = nullptr;   // target leaf index, near list range, flags

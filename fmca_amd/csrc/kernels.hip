@@ -616,10 +616,13 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
           }
         }
       }
-      // near field (unsymmetricNystromEvaluator.h:63-73, H2kernelActor.h:68-69)
-      for (int nb = n0; nb < n1; ++nb) {
-        const int s = pat.near_src[nb];
-        const int sb = cols.begin[s], ss = cols.size[s];
+      // near field (unsymmetricNystromEvaluator.h:63-73, H2kernelActor.h:68-69);
+      // a run of sources with adjacent point ranges is one source here
+      for (int nb = n0; nb < n1;) {
+        const int run = pat.near_run[nb];
+        const int s = pat.near_src[nb], sl = pat.near_src[nb + run - 1];
+        const int sb = cols.begin[s], ss = cols.begin[sl] + cols.size[sl] - sb;
+        nb += run;
         for (int j0 = 0; j0 < ss; j0 += kSrcTile) {
           const int nj = min(kSrcTile, ss - j0);
           __syncthreads();

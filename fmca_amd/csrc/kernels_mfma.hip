@@ -184,9 +184,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
       static_assert(kChunk * NC % 256 == 0, "staging layout");
       T py[KY], px[KX];
       int nb = n0, j0 = 0, pnj = 0;
+      // a run of sources with adjacent point ranges is staged as one source
       auto fetch = [&]() {
-        const int s = pat.near_src[nb];
-        const int sb = cols.begin[s], ss = cols.size[s];
+        const int run = pat.near_run[nb];
+        const int s = pat.near_src[nb], sl = pat.near_src[nb + run - 1];
+        const int sb = cols.begin[s], ss = cols.begin[sl] + cols.size[sl] - sb;
         pnj = min(kChunk, ss - j0);
 #pragma unroll
         for (int k = 0; k < KY; ++k) {
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
         }
         j0 += kChunk;
         if (j0 >= ss) {
-          ++nb;
+          nb += run;
           j0 = 0;
         }
       };

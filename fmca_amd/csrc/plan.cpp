@@ -686,6 +686,15 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
       }
     }
     pat.near_order = xcd_launch_order(wt, {{0, (int)wt.size()}}, kXcds);
+    // runs of contiguous sources (the lists are sorted by point range)
+    pat.near_run.assign(pat.near_src.size(), 1);
+    for (size_t t = 0; t < pat.near_task.size(); t += 4) {
+      const int a = pat.near_task[t + 1], b = pat.near_task[t + 2];
+      for (int e = b - 2; e >= a; --e) {
+        const int s = pat.near_src[e], sn = pat.near_src[e + 1];
+        if (C.begin[sn] == C.begin[s] + C.size[s]) pat.near_run[e] = pat.near_run[e + 1] + 1;
+      }
+    }
   }
 }
 

@@ -112,6 +112,10 @@ struct Pattern {
   // an output that is zeroed first); the first piece of a leaf also applies
   // the far field (kTaskFar).
   std::vector<int32_t> far_task, near_task;
+  // near_run[e]: the number of near-list entries from e on, within e's task,
+  // whose point ranges follow each other without a gap (1 if the next one
+  // does not): the leaf output stages such a run as one source
+  std::vector<int32_t> near_run;
   std::vector<int32_t> far_split, near_split;  // targets cut into several tasks
   // launch orders over the tasks (one workgroup per entry, -1 = nothing to
   // do, see xcd_launch_order)
