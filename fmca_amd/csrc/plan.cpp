@@ -327,11 +327,29 @@ namespace {
 struct WorkNode {
   int64_t begin = 0, size = 0;
   int32_t parent = -1, level = 0, son0 = -1, son1 = -1;
+  int32_t axis = -1;       // split axis of a refined node
   double bb[3 * kMaxDim];  // [min | max | size], each dim entries
 };
+
+// One build.  exact: every split sorts its range as FMCA does (std::sort by
+// key), so ties between equal coordinates resolve as in FMCA.  Otherwise the
+// splits only partition (std::nth_element) and each leaf is sorted once, by
+// its father's split key, at the end.  With pairwise distinct keys the two
+// give the same permutation: FMCA's order inside a leaf is the order of its
+// father's sort, and a partition at the median is the same set either way.
+// Equal keys at a split point or inside a leaf make the fast build return
+// false; the caller then builds exactly.
+bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize, bool exact);
 }  // namespace
 
 void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize) {
+  if (!build_tree_impl(T, dim, n, P, min_csize, false))
+    build_tree_impl(T, dim, n, P, min_csize, true);
+}
+
+namespace {
+bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize,
+                     bool exact) {
   T = Tree();
   T.dim = dim;
   T.npts = n;
@@ -367,6 +385,7 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
     root.bb[i + 2 * dim] = mx - mn;
   }
   const int64_t thr = min_csize >= 1 ? (2 * min_csize - 1) : 1;
+  std::atomic<bool> tie{false};
   std::vector<int32_t> level_ptr{0};
   int64_t lb = 0, le = 1;
   int level = 0;
@@ -380,29 +399,39 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
     // sorting indices through the coordinate, so std::sort yields the same
     // permutation, ties included.
     parallel_for(cnt, 1, [&](int64_t k) {
-      const WorkNode &nd = nodes[lb + k];
+      WorkNode &nd = nodes[lb + k];
       if (nd.size <= thr) return;
       split[k] = 1;
       int longest = 0;
       for (int i = 1; i < dim; ++i)
         if (nd.bb[i + 2 * dim] > nd.bb[longest + 2 * dim]) longest = i;
+      nd.axis = longest;
       std::vector<std::pair<double, uint32_t>> kv(nd.size);
       uint32_t *id = T.perm.data() + nd.begin;
       for (int64_t j = 0; j < nd.size; ++j) kv[j] = {P[longest + (size_t)dim * id[j]], id[j]};
-      std::sort(kv.begin(), kv.end(),
-                [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
-                  return a.first < b.first;
-                });
+      auto by_key = [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+        return a.first < b.first;
+      };
+      const int64_t half = nd.size / 2;
+      double left_max;
+      if (exact) {
+        std::sort(kv.begin(), kv.end(), by_key);
+        left_max = kv[half - 1].first;
+      } else {
+        std::nth_element(kv.begin(), kv.begin() + half, kv.end(), by_key);
+        left_max = std::max_element(kv.begin(), kv.begin() + half, by_key)->first;
+        if (!(left_max < kv[half].first)) tie = true;
+      }
       for (int64_t j = 0; j < nd.size; ++j) id[j] = kv[j].second;
       WorkNode &c1 = sons[2 * k];
       WorkNode &c2 = sons[2 * k + 1];
       std::memcpy(c1.bb, nd.bb, sizeof(nd.bb));
       std::memcpy(c2.bb, nd.bb, sizeof(nd.bb));
       c1.begin = nd.begin;
-      c1.size = nd.size / 2;
+      c1.size = half;
       c2.begin = nd.begin + c1.size;
       c2.size = nd.size - c1.size;
-      c1.bb[longest + dim] = kv[c1.size - 1].first;
+      c1.bb[longest + dim] = left_max;
       c1.bb[longest + 2 * dim] = c1.bb[longest + dim] - c1.bb[longest];
       c2.bb[longest] = kv[c1.size].first;
       c2.bb[longest + 2 * dim] = c2.bb[longest + dim] - c2.bb[longest];
@@ -420,6 +449,27 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
     lb = le;
     le = (int64_t)nodes.size();
     ++level;
+    if (tie) return false;
+  }
+  if (!exact) {
+    // leaves in their father's sort order
+    parallel_for((int64_t)nodes.size(), 16, [&](int64_t c) {
+      const WorkNode &nd = nodes[c];
+      if (nd.son0 >= 0 || nd.parent < 0 || nd.size < 2) return;
+      const int ax = nodes[nd.parent].axis;
+      uint32_t *id = T.perm.data() + nd.begin;
+      std::vector<std::pair<double, uint32_t>> kv(nd.size);
+      for (int64_t j = 0; j < nd.size; ++j) kv[j] = {P[ax + (size_t)dim * id[j]], id[j]};
+      std::sort(kv.begin(), kv.end(),
+                [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+                  return a.first < b.first;
+                });
+      for (int64_t j = 0; j < nd.size; ++j) {
+        id[j] = kv[j].second;
+        if (j && !(kv[j - 1].first < kv[j].first)) tie = true;
+      }
+    });
+    if (tie) return false;
   }
   const int ncl = (int)nodes.size();
   T.ncl = ncl;
@@ -487,7 +537,9 @@ void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize)
       }
     });
   }
+  return true;
 }
+}  // namespace
 
 // ---------------------------------------------------------------------------
 namespace {

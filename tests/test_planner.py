@@ -61,6 +61,28 @@ def test_planner_ties_and_duplicates():
     assert _block_set(*H.blocks()) == _block_set(*A.blocks())
 
 
+@pytest.mark.parametrize("kind", ["uniform", "grid-ties", "1d-ties"])
+@pytest.mark.parametrize("scheme", ["tensor", "total_degree"])
+def test_planner_fast_build_and_tie_fallback(kind, scheme):
+    """The planner partitions with nth_element and sorts each leaf once; on
+    equal coordinates at a split point or inside a leaf it rebuilds with
+    FMCA's full sorts.  Either way the permutation and blocks are FMCA's."""
+    rng = np.random.default_rng(9)
+    if kind == "uniform":
+        P = rng.random((3, 20000))
+    elif kind == "grid-ties":
+        P = rng.integers(0, 30, (3, 20000)).astype(float) / 30
+    else:
+        P = rng.integers(0, 500, (1, 5000)).astype(float) / 500
+    P = np.asfortranarray(P)
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 3, 0.8, scheme=scheme)
+    interp = "tensor_product" if scheme == "tensor" else "total_degree"
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("EXPONENTIAL", 0.5), P, P, 3, 0.8,
+                               interp)
+    assert np.array_equal(H.indices(0), A.indices(0))
+    assert _block_set(*H.blocks()) == _block_set(*A.blocks())
+
+
 def test_statistics_semantics():
     """H2MatrixBase::statistics (H2MatrixBase.h:70-105)."""
     P = uniform_points(8000, 3, seed=3)
