@@ -181,3 +181,56 @@ def test_multipole_function_evaluator():
     Pe2 = uniform_points(1000, 2, seed=44)
     ref2 = oracle.OracleH2(P, Pe2, "MATERN52", 0.3, 4, 0.8, scheme="total_degree").action(C)
     assert rel_err(ev.evaluate(P, Pe2, C), ref2) <= FP64_TOL
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_total_degree_variant_groups_equal_separate_actions(transpose):
+    """The fused-backward entry (16-column groups with different kernel
+    variants in one action) on the total-degree form: each group equals the
+    single-variant action of its columns."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print(f"[case] total-degree variant groups T={transpose}", flush=True)
+    n, m = 6000, 5000
+    P, Pe = W.uniform(n, 3, 31), W.uniform(m, 3, 32)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.4), Pe, P, 4, 0.8,
+                               interpolator="total_degree")
+    groups = [2, 0, 1, 2]
+    rows_in, rows_out = (m, n) if transpose else (n, m)
+    ncols = 16 * (len(groups) - 1) + 9
+    X = torch.randn(ncols, rows_in, dtype=torch.float64, device="cuda").t()
+    stream = torch.cuda.current_stream().cuda_stream
+    for prec, tol in ((fmca_amd.FP64, 1e-13), (fmca_amd.FP32, 1e-6)):
+        Y = torch.empty(ncols, rows_out, dtype=torch.float64, device="cuda").t()
+        H._a.action_variants_device(groups, transpose, X.data_ptr(), rows_in, ncols,
+                                    Y.data_ptr(), rows_out, prec, stream)
+        for g, v in enumerate(groups):
+            c0, c1 = 16 * g, min(16 * g + 16, ncols)
+            Xg = X[:, c0:c1].t().contiguous().t()
+            Yg = torch.empty(c1 - c0, rows_out, dtype=torch.float64, device="cuda").t()
+            H._a.action_variant_device(v, transpose, Xg.data_ptr(), rows_in, c1 - c0,
+                                       Yg.data_ptr(), rows_out, prec, stream)
+            err = (torch.linalg.norm(Y[:, c0:c1] - Yg) / torch.linalg.norm(Yg)).item()
+            assert err <= tol, (prec, g, v, err)
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_total_degree_cg_reaches_1e8(precision):
+    """pCG (PCG.h:15-44) on (K + ridge I) with the total-degree form: the FP64
+    residual, recomputed through the FP64 action, is <= 1e-8 for every column."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print(f"[case] total-degree cg {precision}", flush=True)
+    n, ridge = 20000, 1.0  # well above the H2 error of the deg-4 total-degree form
+    P = W.uniform(n, 3, 0)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.3), P, P, 4, 0.8,
+                               interpolator="total_degree")
+    b = torch.from_numpy(W.normal_rhs(n, 4)).cuda()
+    x, info = H.cg(b, x0=torch.zeros_like(b), ridge=ridge, tol=1e-8,
+                   precision=getattr(fmca_amd, precision), max_iter=4000)
+    print(" ", info, flush=True)
+    assert info["converged"]
+    r = b - H.action_device(x) - ridge * x
+    assert (torch.linalg.norm(r, dim=0) / torch.linalg.norm(b, dim=0)).max().item() <= 1e-8
