@@ -552,21 +552,38 @@ double radius(const Tree &T, int c) {
   return 0.5 * std::sqrt(s);
 }
 
+// What the admissibility test reads of a cluster, in one cache line: box min,
+// box max, radius and whether it is a leaf.
+struct alignas(64) ClusterGeo {
+  double mn[kMaxDim], mx[kMaxDim], rad;
+  int leaf;
+};
+
+std::vector<ClusterGeo> cluster_geo(const Tree &T) {
+  std::vector<ClusterGeo> g(T.ncl);
+  for (int c = 0; c < T.ncl; ++c) {
+    for (int i = 0; i < T.dim; ++i) {
+      g[c].mn[i] = T.bbmin[(size_t)c * T.dim + i];
+      g[c].mx[i] = T.bbmax[(size_t)c * T.dim + i];
+    }
+    g[c].rad = radius(T, c);
+    g[c].leaf = T.is_leaf(c) ? 1 : 0;
+  }
+  return g;
+}
+
 // CompareCluster::compare (Admissibility.h:95-115) with computeDistance (:19-29).
 // 0 = refine, 1 = low rank, 2 = dense.
-int admissibility(const Tree &R, int t, const Tree &C, int s, double eta) {
-  const int d = R.dim;
+int admissibility(const ClusterGeo &r, const ClusterGeo &c, int d, double eta) {
   double s2 = 0;
   for (int i = 0; i < d; ++i) {
-    const double v = 0.5 * (R.bbmin[(size_t)t * d + i] - C.bbmin[(size_t)s * d + i] +
-                            R.bbmax[(size_t)t * d + i] - C.bbmax[(size_t)s * d + i]);
+    const double v = 0.5 * (r.mn[i] - c.mn[i] + r.mx[i] - c.mx[i]);
     s2 += v * v;
   }
-  const double rr = radius(R, t), rc = radius(C, s);
-  double dist = std::sqrt(s2) - rr - rc;
+  double dist = std::sqrt(s2) - r.rad - c.rad;
   dist = dist > 0 ? dist : 0;
-  const double rad = rr > rc ? rr : rc;
-  if (rad > eta * dist) return (R.is_leaf(t) || C.is_leaf(s)) ? 2 : 0;
+  const double rad = r.rad > c.rad ? r.rad : c.rad;
+  if (rad > eta * dist) return (r.leaf || c.leaf) ? 2 : 0;
   return 1;
 }
 }  // namespace
@@ -577,6 +594,9 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
   // the one of FMCA's depth-first construction (H2MatrixBase.h:160-191).
   std::vector<std::pair<int32_t, int32_t>> frontier{{0, 0}};
   const int nth = num_workers();
+  const std::vector<ClusterGeo> gr = cluster_geo(R);
+  const std::vector<ClusterGeo> gc_own = &C == &R ? std::vector<ClusterGeo>() : cluster_geo(C);
+  const std::vector<ClusterGeo> &gc = &C == &R ? gr : gc_own;
   while (!frontier.empty()) {
     std::vector<std::vector<std::pair<int32_t, int32_t>>> next(nth);
     std::vector<std::vector<int32_t>> brow(nth), bcol(nth);
@@ -584,9 +604,14 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
     const int64_t F = (int64_t)frontier.size();
     const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nth, F / 256 + 1));
     parallel_chunks(F, nt, [&](int tid, int64_t lo, int64_t hi) {
+      // no reallocation while expanding (every pair refines into at most 4)
+      next[tid].reserve(4 * (hi - lo));
+      brow[tid].reserve(hi - lo);
+      bcol[tid].reserve(hi - lo);
+      blr[tid].reserve(hi - lo);
       for (int64_t k = lo; k < hi; ++k) {
         const int t = frontier[k].first, s = frontier[k].second;
-        const int adm = admissibility(R, t, C, s, eta);
+        const int adm = admissibility(gr[t], gc[s], R.dim, eta);
         if (adm == 0) {
           for (int j = 0; j < 2; ++j)
             for (int i = 0; i < 2; ++i)
