@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   constexpr int NCP = NC + 16;  // padded LDS row stride
   constexpr int kChunk = kChunkOf<T, NT>;
   extern __shared__ unsigned char smem_raw[];
-  const int P1 = ip.P1, q = ip.q;
+  const int q = ip.q;
   const int qpad = (q + 3) & ~3;
   T *U = reinterpret_cast<T *>(smem_raw);  // [D][qpad] unit coordinates of the nodes
   T *Yc = U + D * qpad;                    // [D][kChunk] coordinates of the chunk's rows
@@ -361,10 +361,11 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
     }
   }
   __syncthreads();  // U
-  // source slot and node of chunk row bl; node -1: a padding row
-  auto chunk_row = [&](int bl, int be, int nsrc, int &slot, int &node) {
-    slot = G > 1 ? (int)(((float)bl + 0.5f) * rqpad) : 0;
-    node = be + bl - slot * qpad;
+  // source slot and node of row bl of a chunk of nsrc whole sources (G >= 2);
+  // node -1: a padding row (its coordinates are then those of slot 0, node 0)
+  auto chunk_row = [&](int bl, int nsrc, int &slot, int &node) {
+    slot = (int)(((float)bl + 0.5f) * rqpad);
+    node = bl - slot * qpad;
     if (slot >= nsrc || node >= q) node = -1;
     if (slot >= nsrc) slot = 0;
   };
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
           const int d = e / kChunk, bl = e % kChunk;
           if (e < D * kChunk) {
             int slot, node;
-            chunk_row(bl, 0, nsrc, slot, node);
+            chunk_row(bl, nsrc, slot, node);
             const int s = pat.far_src[f + slot];
             py[k] = U[d * qpad + (node < 0 ? 0 : node)] * cols.bbsize[s * D + d] +
                     cols.bbmin[s * D + d];
