@@ -45,6 +45,12 @@ def device_memory(reset_peak: bool = False):
     return _C.device_memory(reset_peak)
 
 
+def set_device_memory_limit(limit_bytes: int):
+    """Cap the library's device memory (0: no cap).  Allocations beyond it
+    fail with an out-of-memory error, as a full HBM would."""
+    _C.set_device_memory_limit(int(limit_bytes))
+
+
 KERNEL_TYPES = (
     "EXPONENTIAL", "MATERN32", "MATERN52", "MATERN72", "MATERN92", "GAUSSIAN",
     "INVMULTIQUADRIC", "MULTIQUADRIC", "BIHARMONIC2D", "BIHARMONIC3D", "TRIHARMONIC3D",

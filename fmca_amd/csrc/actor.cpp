@@ -45,6 +45,9 @@ fmca_status from_hip(hipError_t e, const char *what) {
 
 // Device bytes held by all DevBufs of the process, and their high-water mark.
 std::atomic<int64_t> g_dev_live{0}, g_dev_peak{0};
+// Cap on g_dev_live set through fmca_set_device_memory_limit (0: none).  An
+// allocation that would exceed it fails as hipMalloc does when HBM is full.
+std::atomic<int64_t> g_dev_limit{0};
 
 void note_device_bytes(int64_t delta) {
   const int64_t now = g_dev_live.fetch_add(delta) + delta;
@@ -74,6 +77,11 @@ struct DevBuf {
     if (b <= bytes && p) return hipSuccess;
     release();
     const size_t h = std::max<size_t>(b, 256);
+    const int64_t cap = g_dev_limit.load();
+    if (cap > 0 && g_dev_live.load() + (int64_t)h > cap) {
+      p = nullptr;
+      return hipErrorOutOfMemory;
+    }
     hipError_t e = hipMalloc(&p, h);
     if (e == hipSuccess) {
       bytes = b;
@@ -1053,6 +1061,12 @@ fmca_status fmca_device_memory(int64_t *live_bytes, int64_t *peak_bytes, int res
   if (live_bytes) *live_bytes = g_dev_live.load();
   if (peak_bytes) *peak_bytes = g_dev_peak.load();
   if (reset_peak) g_dev_peak.store(g_dev_live.load());
+  return FMCA_OK;
+}
+
+fmca_status fmca_set_device_memory_limit(int64_t limit_bytes) {
+  if (limit_bytes < 0) return fail(FMCA_ERR_INVALID_ARGUMENT, "negative limit");
+  g_dev_limit.store(limit_bytes);
   return FMCA_OK;
 }
 

@@ -296,6 +296,7 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
                              reinterpret_cast<double *>(y), n, reinterpret_cast<void *>(stream)));
       },
       py::arg("fn"), py::arg("x"), py::arg("y"), py::arg("n"), py::arg("stream") = 0);
+  m.def("set_device_memory_limit", [](int64_t bytes) { check(fmca_set_device_memory_limit(bytes)); });
   m.def("device_memory", [](bool reset_peak) {
     int64_t live = 0, peak = 0;
     check(fmca_device_memory(&live, &peak, reset_peak ? 1 : 0));

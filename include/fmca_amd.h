@@ -261,6 +261,13 @@ fmca_status fmca_diag_math(int fn, const double *d_x, double *d_y, int64_t n, vo
  * footprint of a plan and its workspaces. */
 fmca_status fmca_device_memory(int64_t *live_bytes, int64_t *peak_bytes, int reset_peak);
 
+/* Caps the library's device memory (0 removes the cap): an allocation that
+ * would take the live bytes above the cap fails with FMCA_ERR_OUT_OF_MEMORY,
+ * as a full HBM would, and leaves the actor usable.  No FMCA counterpart; it
+ * lets callers and tests exercise the out-of-memory path, and bounds the
+ * library's share of HBM. */
+fmca_status fmca_set_device_memory_limit(int64_t limit_bytes);
+
 /* Static string for a status code. */
 const char *fmca_status_string(fmca_status status);
 
