@@ -93,6 +93,7 @@ def _points(P, name):
 
 INTERPOLATORS = {"tensor_product": _C.INTERP_TENSOR_PRODUCT,
                  "total_degree": _C.INTERP_TOTAL_DEGREE}
+TREE_BUILDERS = {"host": _C.TREE_HOST, "device": _C.TREE_DEVICE}
 
 
 class H2kernelActor:
@@ -103,24 +104,42 @@ class H2kernelActor:
     ``interpolator`` is the Interpolator of NystromMoments<Interpolator>
     (H2kernelActor.h:17-36): "tensor_product" (TensorProductInterpolator) or
     "total_degree" (TotalDegreeInterpolator, as in FMCA's own actor test,
-    tests/test_H2kernelActor.cpp:20-24).
+    tests/test_H2kernelActor.cpp:20-24).  ``tree_builder`` "device" builds the
+    cluster trees on the current GPU (the same trees as "host", which is
+    FMCA's ClusterTreeInitializer restated on the CPU); ``plan_report()`` says
+    where each was built and how long planning took.
     """
 
     def __init__(self, kernel: CovarianceKernel, Peval, P, mpole_deg: int = 3,
-                 eta: float = 0.8, interpolator: str = "tensor_product"):
+                 eta: float = 0.8, interpolator: str = "tensor_product",
+                 tree_builder: str = "host"):
         Peval = _points(Peval, "Peval")
         P = _points(P, "P")
         if interpolator not in INTERPOLATORS:
             raise ValueError(f"interpolator must be one of {sorted(INTERPOLATORS)}")
+        if tree_builder not in TREE_BUILDERS:
+            raise ValueError(f"tree_builder must be one of {sorted(TREE_BUILDERS)}")
         self.kernel = kernel
         self.deg = int(mpole_deg)
         self.eta = float(eta)
         self.interpolator = interpolator
         self._a = _C.Actor(kernel.kernelType(), kernel.l, kernel.c, kernel.nu, Peval, P,
-                           self.deg, self.eta, INTERPOLATORS[interpolator])
+                           self.deg, self.eta, INTERPOLATORS[interpolator],
+                           TREE_BUILDERS[tree_builder])
         self.m = self._a.m
         self.n = self._a.n
         self.dim = self._a.dim
+
+    def plan_report(self) -> dict:
+        """Planning seconds (tree_s, pattern_s) and where each tree was built
+        (rows_builder / cols_builder: "host", "device", or "host-fallback" when
+        the device was asked for but equal coordinates sent the build to the
+        host)."""
+        d = dict(self._a.plan_report())
+        names = {0: "host", 1: "device", 2: "host-fallback"}
+        d["rows_builder"] = names[d["rows_builder"]]
+        d["cols_builder"] = names[d["cols_builder"]]
+        return d
 
     # -- host-memory forms (FMCA semantics) ---------------------------------
     def action(self, rhs, precision: int = FP64) -> np.ndarray:
@@ -254,8 +273,8 @@ class H2Matrix(H2kernelActor):
     """
 
     def __init__(self, cov: CovarianceKernel, Pr, Pc, p: int = 3, eta: float = 0.8,
-                 interpolator: str = "total_degree"):
-        super().__init__(cov, Pr, Pc, p, eta, interpolator)
+                 interpolator: str = "total_degree", tree_builder: str = "host"):
+        super().__init__(cov, Pr, Pc, p, eta, interpolator, tree_builder)
 
 
 class MultipoleFunctionEvaluator:

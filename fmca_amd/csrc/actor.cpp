@@ -169,6 +169,10 @@ struct fmca_h2_actor {
   Interpolation ip;
   Tree rows_tree, cols_tree;
   bool shared = false;  // Peval and P are the same point set: one tree
+  // planning report: seconds in the tree builds and the pattern; where each
+  // tree was built (0 host, 1 device, 2 device asked, host built)
+  double tree_s = 0, pattern_s = 0;
+  int rows_builder = 0, cols_builder = 0;
   Pattern pat, pat_t;
   bool have_pat_t = false;
   std::vector<double> trans_rows, trans_cols;
@@ -532,7 +536,17 @@ fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
 fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64_t m,
                                   const double *Peval, int64_t n, const double *P, int deg,
                                   double eta, int interpolator, fmca_h2_actor **actor) {
+  return fmca_h2_create_ex(kernel, dim, m, Peval, n, P, deg, eta, interpolator, FMCA_TREE_HOST,
+                           actor);
+}
+
+fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m,
+                              const double *Peval, int64_t n, const double *P, int deg,
+                              double eta, int interpolator, int tree_builder,
+                              fmca_h2_actor **actor) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  if (tree_builder != FMCA_TREE_HOST && tree_builder != FMCA_TREE_DEVICE)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown tree builder");
   *actor = nullptr;
   if (!kernel || !kernel->ktype) return fail(FMCA_ERR_INVALID_ARGUMENT, "kernel is null");
   const int kt = parse_kernel_type(kernel->ktype);
@@ -569,9 +583,29 @@ fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64
     // H2ClusterTree::init (H2ClusterTree.h:65-74): minimum cluster size q
     a->shared = (P == Peval && m == n) ||
                 (m == n && std::memcmp(P, Peval, sizeof(double) * dim * n) == 0);
-    build_tree(a->rows_tree, dim, m, Peval, a->ip.q);
-    if (!a->shared) build_tree(a->cols_tree, dim, n, P, a->ip.q);
+    const auto t0 = std::chrono::steady_clock::now();
+    // ClusterTreeInitializer<ClusterTree>::init for both trees, on the host or
+    // on the current device
+    auto tree = [&](Tree &T, int64_t cnt, const double *pts, int &where) -> fmca_status {
+      if (tree_builder == FMCA_TREE_HOST) {
+        build_tree(T, dim, cnt, pts, a->ip.q);
+        where = 0;
+        return FMCA_OK;
+      }
+      std::string err;
+      const int r = build_tree_device(T, dim, cnt, pts, a->ip.q, &err);
+      if (r < 0) return fail(from_hip((hipError_t)(-r), "tree"), "device tree build: " + err);
+      where = r == 1 ? 1 : 2;
+      return FMCA_OK;
+    };
+    fmca_status st = tree(a->rows_tree, m, Peval, a->rows_builder);
+    if (st == FMCA_OK && !a->shared) st = tree(a->cols_tree, n, P, a->cols_builder);
+    if (a->shared) a->cols_builder = a->rows_builder;
+    if (st != FMCA_OK) return st;
+    const auto t1 = std::chrono::steady_clock::now();
     compute_pattern(a->pat, a->rows(), a->cols(), eta, a->ip.q);
+    a->tree_s = std::chrono::duration<double>(t1 - t0).count();
+    a->pattern_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     auto transfers = [&](std::vector<double> &T, const Tree &tr) {
       if (a->ip.scheme == IS_TOTAL_DEGREE)
         compute_transfers_dense(T, tr, a->ip);
@@ -588,6 +622,16 @@ fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64
     return fail(FMCA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
   }
   *actor = a.release();
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_plan_report(const fmca_h2_actor *actor, double *tree_seconds,
+                                double *pattern_seconds, int *rows_builder, int *cols_builder) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  if (tree_seconds) *tree_seconds = actor->tree_s;
+  if (pattern_seconds) *pattern_seconds = actor->pattern_s;
+  if (rows_builder) *rows_builder = actor->rows_builder;
+  if (cols_builder) *cols_builder = actor->cols_builder;
   return FMCA_OK;
 }
 

@@ -331,35 +331,86 @@ struct WorkNode {
   double bb[3 * kMaxDim];  // [min | max | size], each dim entries
 };
 
-// One build.  exact: every split sorts its range as FMCA does (std::sort by
-// key), so ties between equal coordinates resolve as in FMCA.  Otherwise the
-// splits only partition (std::nth_element) and each leaf is sorted once, by
-// its father's split key, at the end.  With pairwise distinct keys the two
-// give the same permutation: FMCA's order inside a leaf is the order of its
-// father's sort, and a partition at the median is the same set either way.
-// Equal keys at a split point or inside a leaf make the fast build return
-// false; the caller then builds exactly.
-bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize, bool exact);
+// The host splitter.  exact: every split sorts its range as FMCA does
+// (std::sort by key), so ties between equal coordinates resolve as in FMCA.
+// Otherwise the splits only partition (std::nth_element) and each leaf is
+// sorted once, by its father's split key, at the end.  With pairwise distinct
+// keys at every split point and inside every leaf the two give the same
+// permutation: FMCA's order inside a leaf is the order of its father's sort,
+// and a partition at the median is the same set either way.  Equal keys there
+// make the fast build report a tie; the caller then builds exactly.
+class HostSplitter final : public TreeSplitter {
+ public:
+  HostSplitter(int dim, int64_t n, const double *P, bool exact)
+      : dim_(dim), P_(P), exact_(exact), perm_(n) {
+    std::iota(perm_.begin(), perm_.end(), 0u);
+  }
+  bool split(const std::vector<SplitSeg> &segs, double *lmax, double *rmin) override {
+    std::atomic<bool> tie{false};
+    // Sorting (key, index) pairs by key performs the same comparisons as
+    // sorting indices through the coordinate, so std::sort yields the same
+    // permutation, ties included.
+    parallel_for((int64_t)segs.size(), 1, [&](int64_t k) {
+      const SplitSeg &sg = segs[k];
+      std::vector<std::pair<double, uint32_t>> kv(sg.size);
+      uint32_t *id = perm_.data() + sg.begin;
+      for (int64_t j = 0; j < sg.size; ++j) kv[j] = {P_[sg.axis + (size_t)dim_ * id[j]], id[j]};
+      const int64_t half = sg.size / 2;
+      if (exact_) {
+        std::sort(kv.begin(), kv.end(), by_key);
+        lmax[k] = kv[half - 1].first;
+      } else {
+        std::nth_element(kv.begin(), kv.begin() + half, kv.end(), by_key);
+        lmax[k] = std::max_element(kv.begin(), kv.begin() + half, by_key)->first;
+        if (!(lmax[k] < kv[half].first)) tie = true;
+      }
+      rmin[k] = kv[half].first;
+      for (int64_t j = 0; j < sg.size; ++j) id[j] = kv[j].second;
+    });
+    return !tie;
+  }
+  bool finish_leaves(const std::vector<SplitSeg> &leaves, double *mn, double *mx) override {
+    std::atomic<bool> tie{false};
+    parallel_for((int64_t)leaves.size(), 16, [&](int64_t k) {
+      const SplitSeg &lf = leaves[k];
+      uint32_t *id = perm_.data() + lf.begin;
+      if (!exact_ && lf.axis >= 0 && lf.size > 1) {
+        // leaves in their father's sort order
+        std::vector<std::pair<double, uint32_t>> kv(lf.size);
+        for (int64_t j = 0; j < lf.size; ++j) kv[j] = {P_[lf.axis + (size_t)dim_ * id[j]], id[j]};
+        std::sort(kv.begin(), kv.end(), by_key);
+        for (int64_t j = 0; j < lf.size; ++j) {
+          id[j] = kv[j].second;
+          if (j && !(kv[j - 1].first < kv[j].first)) tie = true;
+        }
+      }
+      double *a = mn + (size_t)k * dim_, *b = mx + (size_t)k * dim_;
+      const double *p0 = P_ + (size_t)dim_ * id[0];
+      for (int i = 0; i < dim_; ++i) a[i] = b[i] = p0[i];
+      for (int64_t j = 0; j < lf.size; ++j) {
+        const double *p = P_ + (size_t)dim_ * id[j];
+        for (int i = 0; i < dim_; ++i) {
+          a[i] = std::min(p[i], a[i]);
+          b[i] = std::max(p[i], b[i]);
+        }
+      }
+    });
+    return !tie;
+  }
+  void permutation(std::vector<uint32_t> &perm) override { perm.swap(perm_); }
+
+ private:
+  static bool by_key(const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+    return a.first < b.first;
+  }
+  int dim_;
+  const double *P_;
+  bool exact_;
+  std::vector<uint32_t> perm_;
+};
 }  // namespace
 
-void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize) {
-  if (!build_tree_impl(T, dim, n, P, min_csize, false))
-    build_tree_impl(T, dim, n, P, min_csize, true);
-}
-
-namespace {
-bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize,
-                     bool exact) {
-  T = Tree();
-  T.dim = dim;
-  T.npts = n;
-  T.perm.resize(n);
-  std::iota(T.perm.begin(), T.perm.end(), 0u);
-  std::vector<WorkNode> nodes(1);
-  WorkNode &root = nodes[0];
-  root.begin = 0;
-  root.size = n;
-  // eps-enlarged root box (ClusterTreeInitializer_ClusterTree.h:44-56)
+void point_box(int dim, int64_t n, const double *P, double *mn, double *mx) {
   const int nw = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), n / 65536 + 1));
   for (int i = 0; i < dim; ++i) {
     std::vector<double> tmn(nw, P[i]), tmx(nw, P[i]);
@@ -373,11 +424,37 @@ bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_c
       tmn[t] = a;
       tmx[t] = b;
     });
-    double mn = P[i], mx = P[i];
+    mn[i] = mx[i] = P[i];
     for (int t = 0; t < nw; ++t) {
-      mn = tmn[t] < mn ? tmn[t] : mn;
-      mx = tmx[t] > mx ? tmx[t] : mx;
+      mn[i] = tmn[t] < mn[i] ? tmn[t] : mn[i];
+      mx[i] = tmx[t] > mx[i] ? tmx[t] : mx[i];
     }
+  }
+}
+
+void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize) {
+  double mn[kMaxDim], mx[kMaxDim];
+  point_box(dim, n, P, mn, mx);
+  {
+    HostSplitter fast(dim, n, P, false);
+    if (build_tree_with(T, dim, n, P, mn, mx, min_csize, fast)) return;
+  }
+  HostSplitter exact(dim, n, P, true);
+  build_tree_with(T, dim, n, P, mn, mx, min_csize, exact);
+}
+
+bool build_tree_with(Tree &T, int dim, int64_t n, const double *P, const double *root_min,
+                     const double *root_max, int64_t min_csize, TreeSplitter &sp) {
+  T = Tree();
+  T.dim = dim;
+  T.npts = n;
+  std::vector<WorkNode> nodes(1);
+  WorkNode &root = nodes[0];
+  root.begin = 0;
+  root.size = n;
+  // eps-enlarged root box (ClusterTreeInitializer_ClusterTree.h:44-56)
+  for (int i = 0; i < dim; ++i) {
+    double mn = root_min[i], mx = root_max[i];
     mn -= kEps * std::abs(mn);
     mx += kEps * std::abs(mx);
     root.bb[i] = mn;
@@ -385,91 +462,56 @@ bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_c
     root.bb[i + 2 * dim] = mx - mn;
   }
   const int64_t thr = min_csize >= 1 ? (2 * min_csize - 1) : 1;
-  std::atomic<bool> tie{false};
   std::vector<int32_t> level_ptr{0};
   int64_t lb = 0, le = 1;
   int level = 0;
+  std::vector<SplitSeg> segs;
+  std::vector<int64_t> seg_node;
+  std::vector<double> lmax, rmin;
   while (lb < le) {
-    const int64_t cnt = le - lb;
-    std::vector<char> split(cnt, 0);
-    std::vector<WorkNode> sons(2 * cnt);
-    // Split every node of this level: sort its index range by the longest
-    // edge of its split box and halve it (ClusterTreeSplitter.h:198-219).
-    // Sorting (key, index) pairs by key performs the same comparisons as
-    // sorting indices through the coordinate, so std::sort yields the same
-    // permutation, ties included.
-    parallel_for(cnt, 1, [&](int64_t k) {
-      WorkNode &nd = nodes[lb + k];
-      if (nd.size <= thr) return;
-      split[k] = 1;
+    // Split every node of this level above the leaf size: sort its index
+    // range by the longest edge of its split box and halve it
+    // (ClusterTreeSplitter.h:198-219).
+    segs.clear();
+    seg_node.clear();
+    for (int64_t k = lb; k < le; ++k) {
+      WorkNode &nd = nodes[k];
+      if (nd.size <= thr) continue;
       int longest = 0;
       for (int i = 1; i < dim; ++i)
         if (nd.bb[i + 2 * dim] > nd.bb[longest + 2 * dim]) longest = i;
       nd.axis = longest;
-      std::vector<std::pair<double, uint32_t>> kv(nd.size);
-      uint32_t *id = T.perm.data() + nd.begin;
-      for (int64_t j = 0; j < nd.size; ++j) kv[j] = {P[longest + (size_t)dim * id[j]], id[j]};
-      auto by_key = [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
-        return a.first < b.first;
-      };
-      const int64_t half = nd.size / 2;
-      double left_max;
-      if (exact) {
-        std::sort(kv.begin(), kv.end(), by_key);
-        left_max = kv[half - 1].first;
-      } else {
-        std::nth_element(kv.begin(), kv.begin() + half, kv.end(), by_key);
-        left_max = std::max_element(kv.begin(), kv.begin() + half, by_key)->first;
-        if (!(left_max < kv[half].first)) tie = true;
-      }
-      for (int64_t j = 0; j < nd.size; ++j) id[j] = kv[j].second;
-      WorkNode &c1 = sons[2 * k];
-      WorkNode &c2 = sons[2 * k + 1];
-      std::memcpy(c1.bb, nd.bb, sizeof(nd.bb));
-      std::memcpy(c2.bb, nd.bb, sizeof(nd.bb));
-      c1.begin = nd.begin;
-      c1.size = half;
-      c2.begin = nd.begin + c1.size;
-      c2.size = nd.size - c1.size;
-      c1.bb[longest + dim] = left_max;
-      c1.bb[longest + 2 * dim] = c1.bb[longest + dim] - c1.bb[longest];
-      c2.bb[longest] = kv[c1.size].first;
-      c2.bb[longest + 2 * dim] = c2.bb[longest + dim] - c2.bb[longest];
+      segs.push_back({nd.begin, nd.size, longest});
+      seg_node.push_back(k);
+    }
+    lmax.assign(segs.size(), 0.);
+    rmin.assign(segs.size(), 0.);
+    if (!segs.empty() && !sp.split(segs, lmax.data(), rmin.data())) return false;
+    for (size_t s = 0; s < segs.size(); ++s) {
+      const int64_t k = seg_node[s];
+      const int ax = segs[s].axis;
+      WorkNode c1, c2;
+      std::memcpy(c1.bb, nodes[k].bb, sizeof(c1.bb));
+      std::memcpy(c2.bb, nodes[k].bb, sizeof(c2.bb));
+      c1.begin = nodes[k].begin;
+      c1.size = nodes[k].size / 2;
+      c2.begin = nodes[k].begin + c1.size;
+      c2.size = nodes[k].size - c1.size;
+      c1.bb[ax + dim] = lmax[s];
+      c1.bb[ax + 2 * dim] = c1.bb[ax + dim] - c1.bb[ax];
+      c2.bb[ax] = rmin[s];
+      c2.bb[ax + 2 * dim] = c2.bb[ax + dim] - c2.bb[ax];
       c1.level = c2.level = level + 1;
-      c1.parent = c2.parent = (int32_t)(lb + k);
-    });
-    for (int64_t k = 0; k < cnt; ++k)
-      if (split[k]) {
-        nodes[lb + k].son0 = (int32_t)nodes.size();
-        nodes.push_back(sons[2 * k]);
-        nodes[lb + k].son1 = (int32_t)nodes.size();
-        nodes.push_back(sons[2 * k + 1]);
-      }
+      c1.parent = c2.parent = (int32_t)k;
+      nodes[k].son0 = (int32_t)nodes.size();
+      nodes.push_back(c1);
+      nodes[k].son1 = (int32_t)nodes.size();
+      nodes.push_back(c2);
+    }
     level_ptr.push_back((int32_t)le);
     lb = le;
     le = (int64_t)nodes.size();
     ++level;
-    if (tie) return false;
-  }
-  if (!exact) {
-    // leaves in their father's sort order
-    parallel_for((int64_t)nodes.size(), 16, [&](int64_t c) {
-      const WorkNode &nd = nodes[c];
-      if (nd.son0 >= 0 || nd.parent < 0 || nd.size < 2) return;
-      const int ax = nodes[nd.parent].axis;
-      uint32_t *id = T.perm.data() + nd.begin;
-      std::vector<std::pair<double, uint32_t>> kv(nd.size);
-      for (int64_t j = 0; j < nd.size; ++j) kv[j] = {P[ax + (size_t)dim * id[j]], id[j]};
-      std::sort(kv.begin(), kv.end(),
-                [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
-                  return a.first < b.first;
-                });
-      for (int64_t j = 0; j < nd.size; ++j) {
-        id[j] = kv[j].second;
-        if (j && !(kv[j - 1].first < kv[j].first)) tie = true;
-      }
-    });
-    if (tie) return false;
   }
   const int ncl = (int)nodes.size();
   T.ncl = ncl;
@@ -484,6 +526,7 @@ bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_c
   T.bbmin.assign((size_t)ncl * dim, 0.);
   T.bbmax.assign((size_t)ncl * dim, 0.);
   T.bbsize.assign((size_t)ncl * dim, 0.);
+  std::vector<SplitSeg> leaves;
   for (int c = 0; c < ncl; ++c) {
     T.begin[c] = nodes[c].begin;
     T.size[c] = nodes[c].size;
@@ -491,37 +534,39 @@ bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_c
     T.son0[c] = nodes[c].son0;
     T.son1[c] = nodes[c].son1;
     T.parent[c] = nodes[c].parent;
-    if (nodes[c].son0 < 0) T.leaves.push_back(c);
+    if (nodes[c].son0 < 0) {
+      T.leaves.push_back(c);
+      const int fa = nodes[c].parent;
+      leaves.push_back({nodes[c].begin, nodes[c].size, fa >= 0 ? nodes[fa].axis : -1});
+    }
   }
+  // Leaf point boxes (and, for the fast build, the leaves' final order).
+  std::vector<double> lmn(leaves.size() * dim), lmx(leaves.size() * dim);
+  if (!sp.finish_leaves(leaves, lmn.data(), lmx.data())) return false;
+  sp.permutation(T.perm);
+  std::vector<int32_t> leaf_index(ncl, -1);
+  for (size_t k = 0; k < T.leaves.size(); ++k) leaf_index[T.leaves[k]] = (int32_t)k;
   // Shrink to tight boxes, deepest level first (ClusterTreeInitializer_ClusterTree.h:88-132).
+  // A leaf's box is the min / max over its points; a father's over its first
+  // point and its sons' (widened) boxes.
   for (int L = T.depth; L >= 0; --L) {
     const int cl0 = T.level_ptr[L];
     parallel_for(T.level_ptr[L + 1] - cl0, 16, [&](int64_t ci) {
       const int c = cl0 + (int)ci;
       double mn[kMaxDim], mx[kMaxDim];
-      if (T.size[c]) {
+      if (T.son0[c] >= 0) {
         const double *p0 = P + (size_t)dim * T.perm[T.begin[c]];
         for (int i = 0; i < dim; ++i) mn[i] = mx[i] = p0[i];
-        if (T.son0[c] >= 0) {
-          for (int s : {T.son0[c], T.son1[c]})
-            if (T.size[s])
-              for (int i = 0; i < dim; ++i) {
-                mn[i] = std::min(mn[i], T.bbmin[(size_t)s * dim + i]);
-                mx[i] = std::max(mx[i], T.bbmax[(size_t)s * dim + i]);
-              }
-        } else {
-          for (int64_t j = 0; j < T.size[c]; ++j) {
-            const double *p = P + (size_t)dim * T.perm[T.begin[c] + j];
-            for (int i = 0; i < dim; ++i) {
-              mn[i] = std::min(p[i], mn[i]);
-              mx[i] = std::max(p[i], mx[i]);
-            }
+        for (int s : {T.son0[c], T.son1[c]})
+          for (int i = 0; i < dim; ++i) {
+            mn[i] = std::min(mn[i], T.bbmin[(size_t)s * dim + i]);
+            mx[i] = std::max(mx[i], T.bbmax[(size_t)s * dim + i]);
           }
-        }
       } else {
+        const int k = leaf_index[c];
         for (int i = 0; i < dim; ++i) {
-          mn[i] = std::numeric_limits<double>::infinity();
-          mx[i] = -std::numeric_limits<double>::infinity();
+          mn[i] = lmn[(size_t)k * dim + i];
+          mx[i] = lmx[(size_t)k * dim + i];
         }
       }
       for (int i = 0; i < dim; ++i) {
@@ -539,7 +584,6 @@ bool build_tree_impl(Tree &T, int dim, int64_t n, const double *P, int64_t min_c
   }
   return true;
 }
-}  // namespace
 
 // ---------------------------------------------------------------------------
 namespace {

@@ -95,6 +95,46 @@ struct Tree {
 //  FMCA/src/Clustering/ClusterTreeSplitter.h:198-219).
 void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize);
 
+// A range of the cluster-order permutation and a coordinate axis.
+struct SplitSeg {
+  int64_t begin, size;
+  int axis;  // -1: none (a leaf without a father)
+};
+
+// The data-parallel part of a tree build: the host planner's splitter and
+// the device build each implement it; build_tree_with does the rest.
+struct TreeSplitter {
+  virtual ~TreeSplitter() = default;
+  // For every segment k: reorder its range so that its size / 2 points of
+  // smallest coordinate on the axis come first; lmax[k] = the largest of
+  // them, rmin[k] = the smallest of the others.  False if, for some k,
+  // lmax[k] == rmin[k] (equal coordinates at a split point) and the build
+  // must be redone exactly.
+  virtual bool split(const std::vector<SplitSeg> &segs, double *lmax, double *rmin) = 0;
+  // For every leaf k (axis: its father's split axis): put it in its father's
+  // sort order if that is still to be done (false on equal coordinates in the
+  // leaf), and store the min / max over its points in mn / mx[k * dim + i].
+  virtual bool finish_leaves(const std::vector<SplitSeg> &leaves, double *mn, double *mx) = 0;
+  // Hand over the final permutation.
+  virtual void permutation(std::vector<uint32_t> &perm) = 0;
+};
+
+// Min / max of each coordinate over n points.
+void point_box(int dim, int64_t n, const double *P, double *mn, double *mx);
+
+// build_tree with the O(n) work on the current HIP device (tree_device.hip).
+// Returns 1 if built there, 0 if built on the host instead (equal coordinates
+// at a split point or inside a leaf, a -0.0 coordinate, or leaves too large for
+// the device sort; the result is the same either way), minus the hipError_t
+// code on a HIP error (message in *err; T is then unspecified).
+int build_tree_device(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize,
+                      std::string *err);
+
+// The level loop of build_tree with a given splitter; root_min / root_max:
+// the points' coordinate box.  False if the splitter reported equal keys.
+bool build_tree_with(Tree &T, int dim, int64_t n, const double *P, const double *root_min,
+                     const double *root_max, int64_t min_csize, TreeSplitter &sp);
+
 // FMCA's leaf blocks (H2MatrixBase::computePattern, H2MatrixBase.h:160-191),
 // regrouped by target for a gather-style GPU schedule.
 struct Pattern {

@@ -79,6 +79,32 @@ fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64
                                   const double *Peval, int64_t n, const double *P, int deg,
                                   double eta, int interpolator, fmca_h2_actor **actor);
 
+/* Where fmca_h2_create_ex builds the cluster trees. */
+enum { FMCA_TREE_HOST = 0, FMCA_TREE_DEVICE = 1 };
+
+/* fmca_h2_create_interp with a choice of tree builder.  FMCA_TREE_DEVICE runs
+ * the O(n) work of ClusterTreeInitializer<ClusterTree>::init
+ * (ClusterTreeInitializer_ClusterTree.h:24-132, CardinalityBisection in
+ * ClusterTreeSplitter.h:60-81), which H2kernelActor's H2ClusterTree members
+ * run in its constructor (H2kernelActor.h:21-27), on the current HIP device:
+ * the points are copied there for the build and freed after it (this
+ * transient memory is not counted by fmca_set_device_memory_limit).  The trees
+ * equal the host build's bit for bit.  Where coordinates are equal at a split
+ * point or inside a leaf, or a coordinate is -0.0, the tree is built on the
+ * host instead (fmca_h2_plan_report says so).  FMCA_ERR_DEVICE /
+ * FMCA_ERR_NO_DEVICE / FMCA_ERR_OUT_OF_MEMORY if the device build fails. */
+fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m,
+                              const double *Peval, int64_t n, const double *P, int deg,
+                              double eta, int interpolator, int tree_builder,
+                              fmca_h2_actor **actor);
+
+/* Seconds the create call spent building the trees and the block pattern,
+ * and where each tree was built: 0 host, 1 device, 2 device asked for but
+ * built on the host.  Null outputs are skipped.  No reference counterpart
+ * (FMCA reports no setup times). */
+fmca_status fmca_h2_plan_report(const fmca_h2_actor *actor, double *tree_seconds,
+                                double *pattern_seconds, int *rows_builder, int *cols_builder);
+
 /* Interpolation data, as the interpolator's idcs(), Xi() and invV() return
  * them (TotalDegreeInterpolator.h:50-55, TensorProductInterpolator.h:83-85):
  * the interpolator id, q, the q multi-indices in FMCA's order (q x dim,
