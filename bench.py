@@ -337,13 +337,21 @@ def side_large(args):
     pts = torch.from_numpy(P).cuda()
     alpha = torch.from_numpy(rng.standard_normal((n, nrhs))).cuda().requires_grad_(True)
     l = torch.tensor(0.25, dtype=torch.float64, requires_grad=True)
-    spec = H2Spec("EXPONENTIAL", deg=4, eta=0.8, precision=fmca_amd.FP32)
+    K = fmca_amd.CovarianceKernel("EXPONENTIAL", 0.25)
+    Pf = np.asfortranarray(P)
+    plan = {}
+    for planner in ("host", "device"):
+        t0 = time.perf_counter()
+        Hp = fmca_amd.H2kernelActor(K, Pf, Pf, 4, 0.8, planner=planner)
+        plan[planner] = {"create_s": time.perf_counter() - t0, **Hp.plan_report()}
+        del Hp
+    spec = H2Spec("EXPONENTIAL", deg=4, eta=0.8, precision=fmca_amd.FP32, planner="device")
     t0 = time.perf_counter()
     y = h2_matvec(alpha, l, pts, spec)
     torch.cuda.synchronize()
     first = time.perf_counter() - t0
     res = {"workload": "large: n=8e6 uniform 3D, EXPONENTIAL l=0.25, deg 4, eta 0.8, 16 rhs, "
-                       "FP32 form", "first_forward_incl_plan_s": first,
+                       "FP32 form", "first_forward_incl_plan_s": first, "plan": plan,
            "info": spec._actor.info()}
     tf, tb = [], []
     for _ in range(max(2, args.steps // 2)):
@@ -387,9 +395,15 @@ def main():
     rng = np.random.default_rng(0)
     P = np.asfortranarray(rng.random((3, n)))
     K = fmca_amd.CovarianceKernel("MATERN32", 0.5)
+    torch.zeros(1, device="cuda")  # HIP runtime up before the planning clock starts
     t0 = time.perf_counter()
-    H = fmca_amd.H2kernelActor(K, P, P, 4, 0.8)
+    Hh = fmca_amd.H2kernelActor(K, P, P, 4, 0.8, planner="host")
+    setup_plan_host = time.perf_counter() - t0
+    del Hh
+    t0 = time.perf_counter()
+    H = fmca_amd.H2kernelActor(K, P, P, 4, 0.8, planner="device")
     setup_plan = time.perf_counter() - t0
+    plan_report = H.plan_report()
     t0 = time.perf_counter()
     H.upload()
     torch.cuda.synchronize()
@@ -492,7 +506,8 @@ def main():
                  "fp32_ms": t32_1 * 1e3, "fp64_ms": t64_1 * 1e3},
         "host_boundary": {"fp32_ms": t_host * 1e3, "mpts": n * nrhs / t_host / 1e6,
                           "note": "fmca_h2_action on host arrays: H2D copy + action + D2H"},
-        "setup_s": {"plan": setup_plan, "upload": setup_upload},
+        "setup_s": {"plan": setup_plan, "plan_host": setup_plan_host, "upload": setup_upload,
+                    "plan_report": plan_report},
         "device_memory": {"library_peak_bytes": fmca_amd.device_memory()[1],
                           "torch_peak_bytes": int(torch.cuda.max_memory_allocated())},
     }

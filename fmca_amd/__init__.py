@@ -93,7 +93,24 @@ def _points(P, name):
 
 INTERPOLATORS = {"tensor_product": _C.INTERP_TENSOR_PRODUCT,
                  "total_degree": _C.INTERP_TOTAL_DEGREE}
-TREE_BUILDERS = {"host": _C.TREE_HOST, "device": _C.TREE_DEVICE}
+PLANNERS = {"host": _C.PLAN_HOST, "device": _C.PLAN_DEVICE}
+_AUTO_PLANNER = None
+
+
+def _planner(name: str) -> int:
+    """"auto" plans on the GPU when one is visible, else on the host."""
+    global _AUTO_PLANNER
+    if name == "auto":
+        if _AUTO_PLANNER is None:
+            try:
+                import torch
+                _AUTO_PLANNER = "device" if torch.cuda.is_available() else "host"
+            except Exception:
+                _AUTO_PLANNER = "host"
+        name = _AUTO_PLANNER
+    if name not in PLANNERS:
+        raise ValueError(f"planner must be 'auto' or one of {sorted(PLANNERS)}")
+    return PLANNERS[name]
 
 
 class H2kernelActor:
@@ -104,41 +121,42 @@ class H2kernelActor:
     ``interpolator`` is the Interpolator of NystromMoments<Interpolator>
     (H2kernelActor.h:17-36): "tensor_product" (TensorProductInterpolator) or
     "total_degree" (TotalDegreeInterpolator, as in FMCA's own actor test,
-    tests/test_H2kernelActor.cpp:20-24).  ``tree_builder`` "device" builds the
-    cluster trees on the current GPU (the same trees as "host", which is
-    FMCA's ClusterTreeInitializer restated on the CPU); ``plan_report()`` says
-    where each was built and how long planning took.
+    tests/test_H2kernelActor.cpp:20-24).  ``planner`` "device" builds the
+    cluster trees and expands the block tree on the current GPU (the same plan
+    as "host", which restates FMCA's ClusterTreeInitializer and computePattern
+    on the CPU); "auto" (the default) takes the GPU when one is visible.
+    ``plan_report()`` says where each part ran and how long planning took.
     """
 
     def __init__(self, kernel: CovarianceKernel, Peval, P, mpole_deg: int = 3,
                  eta: float = 0.8, interpolator: str = "tensor_product",
-                 tree_builder: str = "host"):
+                 planner: str = "auto"):
         Peval = _points(Peval, "Peval")
         P = _points(P, "P")
         if interpolator not in INTERPOLATORS:
             raise ValueError(f"interpolator must be one of {sorted(INTERPOLATORS)}")
-        if tree_builder not in TREE_BUILDERS:
-            raise ValueError(f"tree_builder must be one of {sorted(TREE_BUILDERS)}")
+        planner_id = _planner(planner)
         self.kernel = kernel
         self.deg = int(mpole_deg)
         self.eta = float(eta)
         self.interpolator = interpolator
         self._a = _C.Actor(kernel.kernelType(), kernel.l, kernel.c, kernel.nu, Peval, P,
                            self.deg, self.eta, INTERPOLATORS[interpolator],
-                           TREE_BUILDERS[tree_builder])
+                           planner_id)
         self.m = self._a.m
         self.n = self._a.n
         self.dim = self._a.dim
 
     def plan_report(self) -> dict:
-        """Planning seconds (tree_s, pattern_s) and where each tree was built
-        (rows_builder / cols_builder: "host", "device", or "host-fallback" when
-        the device was asked for but equal coordinates sent the build to the
-        host)."""
+        """Planning seconds (tree_s, pattern_s) and where the row tree, the
+        column tree and the block expansion ran (rows_tree, cols_tree, blocks:
+        "host", "device", or "host-fallback" where the device was asked for
+        but equal coordinates, or an admissibility test within rounding of
+        eta, sent that part to the host)."""
         d = dict(self._a.plan_report())
         names = {0: "host", 1: "device", 2: "host-fallback"}
-        d["rows_builder"] = names[d["rows_builder"]]
-        d["cols_builder"] = names[d["cols_builder"]]
+        for k in ("rows_tree", "cols_tree", "blocks"):
+            d[k] = names[d[k]]
         return d
 
     # -- host-memory forms (FMCA semantics) ---------------------------------
@@ -273,8 +291,8 @@ class H2Matrix(H2kernelActor):
     """
 
     def __init__(self, cov: CovarianceKernel, Pr, Pc, p: int = 3, eta: float = 0.8,
-                 interpolator: str = "total_degree", tree_builder: str = "host"):
-        super().__init__(cov, Pr, Pc, p, eta, interpolator, tree_builder)
+                 interpolator: str = "total_degree", planner: str = "auto"):
+        super().__init__(cov, Pr, Pc, p, eta, interpolator, planner)
 
 
 class MultipoleFunctionEvaluator:

@@ -31,15 +31,17 @@ KERNEL_PHI = _C.KERNEL_PHI
 
 
 class H2Spec:
-    """Kernel type, interpolation degree, admissibility, precision and
-    interpolator ("tensor_product" or "total_degree") of an H2 matvec, plus a
-    cache of the plan for the current point set(s)."""
+    """Kernel type, interpolation degree, admissibility, precision,
+    interpolator ("tensor_product" or "total_degree") and planner ("auto",
+    "host" or "device") of an H2 matvec, plus a cache of the plan for the
+    current point set(s)."""
 
     def __init__(self, ktype: str = "MATERN32", deg: int = 4, eta: float = 0.8,
                  precision: int = FP64, c: float = 1.0, nu: float = 1.0,
-                 interpolator: str = "tensor_product"):
+                 interpolator: str = "tensor_product", planner: str = "auto"):
         self.ktype = ktype.upper()
         self.interpolator = interpolator
+        self.planner = planner
         self.deg = int(deg)
         self.eta = float(eta)
         self.precision = int(precision)
@@ -57,7 +59,8 @@ class H2Spec:
             T = P if targets is None else np.asfortranarray(
                 targets.detach().cpu().numpy().astype(np.float64))
             self._actor = H2kernelActor(CovarianceKernel(self.ktype, l, self.c, self.nu), T, P,
-                                        self.deg, self.eta, self.interpolator)
+                                        self.deg, self.eta, self.interpolator,
+                                        self.planner)
             self._key = key
         self._actor.set_kernel(CovarianceKernel(self.ktype, l, self.c, self.nu))
         return self._actor

@@ -169,10 +169,11 @@ struct fmca_h2_actor {
   Interpolation ip;
   Tree rows_tree, cols_tree;
   bool shared = false;  // Peval and P are the same point set: one tree
-  // planning report: seconds in the tree builds and the pattern; where each
-  // tree was built (0 host, 1 device, 2 device asked, host built)
+  // planning report: seconds in the tree builds and the pattern; where the
+  // trees and the block expansion ran (0 host, 1 device, 2 device asked for,
+  // host used)
   double tree_s = 0, pattern_s = 0;
-  int rows_builder = 0, cols_builder = 0;
+  int rows_builder = 0, cols_builder = 0, pattern_builder = 0;
   Pattern pat, pat_t;
   bool have_pat_t = false;
   std::vector<double> trans_rows, trans_cols;
@@ -536,17 +537,17 @@ fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
 fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64_t m,
                                   const double *Peval, int64_t n, const double *P, int deg,
                                   double eta, int interpolator, fmca_h2_actor **actor) {
-  return fmca_h2_create_ex(kernel, dim, m, Peval, n, P, deg, eta, interpolator, FMCA_TREE_HOST,
+  return fmca_h2_create_ex(kernel, dim, m, Peval, n, P, deg, eta, interpolator, FMCA_PLAN_HOST,
                            actor);
 }
 
 fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m,
                               const double *Peval, int64_t n, const double *P, int deg,
-                              double eta, int interpolator, int tree_builder,
+                              double eta, int interpolator, int planner,
                               fmca_h2_actor **actor) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
-  if (tree_builder != FMCA_TREE_HOST && tree_builder != FMCA_TREE_DEVICE)
-    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown tree builder");
+  if (planner != FMCA_PLAN_HOST && planner != FMCA_PLAN_DEVICE)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown planner");
   *actor = nullptr;
   if (!kernel || !kernel->ktype) return fail(FMCA_ERR_INVALID_ARGUMENT, "kernel is null");
   const int kt = parse_kernel_type(kernel->ktype);
@@ -587,7 +588,7 @@ fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m
     // ClusterTreeInitializer<ClusterTree>::init for both trees, on the host or
     // on the current device
     auto tree = [&](Tree &T, int64_t cnt, const double *pts, int &where) -> fmca_status {
-      if (tree_builder == FMCA_TREE_HOST) {
+      if (planner == FMCA_PLAN_HOST) {
         build_tree(T, dim, cnt, pts, a->ip.q);
         where = 0;
         return FMCA_OK;
@@ -603,7 +604,19 @@ fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m
     if (a->shared) a->cols_builder = a->rows_builder;
     if (st != FMCA_OK) return st;
     const auto t1 = std::chrono::steady_clock::now();
-    compute_pattern(a->pat, a->rows(), a->cols(), eta, a->ip.q);
+    if (planner == FMCA_PLAN_HOST) {
+      compute_pattern(a->pat, a->rows(), a->cols(), eta, a->ip.q);
+    } else {
+      std::string err;
+      const int r = expand_blocks_device(a->pat, a->rows(), a->cols(), eta, a->ip.q, &err);
+      if (r < 0) return fail(from_hip((hipError_t)(-r), "pattern"), "device block expansion: " + err);
+      if (r == 0) {
+        a->pat = Pattern();
+        expand_blocks(a->pat, a->rows(), a->cols(), eta, a->ip.q);
+      }
+      a->pattern_builder = r == 1 ? 1 : 2;
+      finish_pattern(a->pat, a->rows(), a->cols(), a->ip.q);
+    }
     a->tree_s = std::chrono::duration<double>(t1 - t0).count();
     a->pattern_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     auto transfers = [&](std::vector<double> &T, const Tree &tr) {
@@ -626,12 +639,15 @@ fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m
 }
 
 fmca_status fmca_h2_plan_report(const fmca_h2_actor *actor, double *tree_seconds,
-                                double *pattern_seconds, int *rows_builder, int *cols_builder) {
+                                double *pattern_seconds, int *where) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
   if (tree_seconds) *tree_seconds = actor->tree_s;
   if (pattern_seconds) *pattern_seconds = actor->pattern_s;
-  if (rows_builder) *rows_builder = actor->rows_builder;
-  if (cols_builder) *cols_builder = actor->cols_builder;
+  if (where) {
+    where[0] = actor->rows_builder;
+    where[1] = actor->cols_builder;
+    where[2] = actor->pattern_builder;
+  }
   return FMCA_OK;
 }
 

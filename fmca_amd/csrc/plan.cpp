@@ -632,8 +632,15 @@ int admissibility(const ClusterGeo &r, const ClusterGeo &c, int d, double eta) {
 }
 }  // namespace
 
+double cluster_radius(const Tree &T, int c) { return radius(T, c); }
+
 void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int q) {
   pat = Pattern();
+  expand_blocks(pat, R, C, eta, q);
+  finish_pattern(pat, R, C, q);
+}
+
+void expand_blocks(Pattern &pat, const Tree &R, const Tree &C, double eta, int q) {
   // Level-synchronous expansion of the block tree; the set of leaf blocks is
   // the one of FMCA's depth-first construction (H2MatrixBase.h:160-191).
   std::vector<std::pair<int32_t, int32_t>> frontier{{0, 0}};
@@ -682,7 +689,12 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
       pat.block_lr.insert(pat.block_lr.end(), blr[t].begin(), blr[t].end());
     }
   }
+}
+
+void finish_pattern(Pattern &pat, const Tree &R, const Tree &C, int q) {
   const int64_t nb = (int64_t)pat.block_row.size();
+  pat.n_far = pat.n_near = 0;
+  pat.near_entries = pat.far_entries = 0;
   // far field CSR by target cluster
   pat.far_ptr.assign(R.ncl + 1, 0);
   for (int64_t b = 0; b < nb; ++b)
@@ -697,9 +709,9 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
     for (int64_t b = 0; b < nb; ++b)
       if (pat.block_lr[b]) pat.far_src[fill[pat.block_row[b]]++] = pat.block_col[b];
   }
-  for (int c = 0; c < R.ncl; ++c) {
+  parallel_for(R.ncl, 64, [&](int64_t c) {
     std::sort(pat.far_src.begin() + pat.far_ptr[c], pat.far_src.begin() + pat.far_ptr[c + 1]);
-  }
+  });
   pat.far_entries = double(pat.n_far) * q * q;
   // near field CSR by target leaf; a dense block whose row cluster is not a
   // leaf is split over the leaves below it (their point ranges partition it).
@@ -711,10 +723,16 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
             [&](int a, int b) { return R.begin[a] < R.begin[b]; });
   std::vector<int64_t> begins(nl);
   for (int k = 0; k < nl; ++k) begins[k] = R.begin[by_begin[k]];
+  // the leaves below each cluster: [leaf_lo, leaf_hi) in by_begin
+  std::vector<int32_t> leaf_lo(R.ncl), leaf_hi(R.ncl);
+  parallel_for(R.ncl, 256, [&](int64_t t) {
+    leaf_lo[t] = int(std::lower_bound(begins.begin(), begins.end(), R.begin[t]) - begins.begin());
+    leaf_hi[t] = int(std::lower_bound(begins.begin(), begins.end(), R.begin[t] + R.size[t]) -
+                     begins.begin());
+  });
   auto leaves_under = [&](int t, int &lo, int &hi) {
-    lo = int(std::lower_bound(begins.begin(), begins.end(), R.begin[t]) - begins.begin());
-    hi = int(std::lower_bound(begins.begin(), begins.end(), R.begin[t] + R.size[t]) -
-             begins.begin());
+    lo = leaf_lo[t];
+    hi = leaf_hi[t];
   };
   pat.near_ptr.assign(nl + 1, 0);
   for (int64_t b = 0; b < nb; ++b)
@@ -737,9 +755,10 @@ void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int
           pat.near_src[fill[leaf_index[by_begin[k]]]++] = pat.block_col[b];
       }
   }
-  for (int k = 0; k < nl; ++k)
+  parallel_for(nl, 16, [&](int64_t k) {
     std::sort(pat.near_src.begin() + pat.near_ptr[k], pat.near_src.begin() + pat.near_ptr[k + 1],
               [&](int a, int b) { return C.begin[a] < C.begin[b]; });
+  });
   // Tasks and launch orders.  Coupling work = far blocks (plus the zero fill
   // of an empty list); leaf output work = near-field entries (plus the far
   // field V^T l_t).  Targets above three times the mean work are cut into

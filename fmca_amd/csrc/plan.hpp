@@ -166,6 +166,23 @@ struct Pattern {
 
 void compute_pattern(Pattern &pat, const Tree &rows, const Tree &cols, double eta, int q);
 
+// The two halves of compute_pattern.  expand_blocks: the leaf blocks
+// (block_row, block_col, block_lr), level by level, each level in the order
+// of its parents' refinements.  finish_pattern: everything else, from them.
+void expand_blocks(Pattern &pat, const Tree &rows, const Tree &cols, double eta, int q);
+void finish_pattern(Pattern &pat, const Tree &rows, const Tree &cols, int q);
+
+// expand_blocks on the current HIP device, with the same blocks in the same
+// order.  Returns 1 if expanded there; 0 if some cluster pair's admissibility
+// lies within rounding of eta (distance and radius compared on the device to
+// a margin; such a plan is expanded on the host instead); minus the hipError_t
+// code on a HIP error (message in *err).
+int expand_blocks_device(Pattern &pat, const Tree &rows, const Tree &cols, double eta, int q,
+                         std::string *err);
+
+// Half the box diagonal of cluster c (Admissibility.h:19-29).
+double cluster_radius(const Tree &T, int c);
+
 // Launch order of one workgroup per item on MI355X, whose 8 XCDs (one L2
 // each) receive the workgroups of a launch round-robin (workgroup b on XCD
 // b % 8).  Each range [lo, hi) of items, taken in the order given, is cut into

@@ -44,7 +44,7 @@ struct Actor {
 
 std::unique_ptr<Actor> create(const std::string &ktype, double l, double c, double nu,
                               const py::array &Peval_in, const py::array &P_in, int deg,
-                              double eta, int interpolator, int tree_builder) {
+                              double eta, int interpolator, int planner) {
   farr Peval = as_f64(Peval_in, "Peval"), P = as_f64(P_in, "P");
   if (Peval.ndim() != 2 || P.ndim() != 2 || Peval.shape(0) != P.shape(0))
     throw py::value_error("points must be (dim, n) arrays of equal dim");
@@ -56,7 +56,7 @@ std::unique_ptr<Actor> create(const std::string &ktype, double l, double c, doub
   {
     py::gil_scoped_release nogil;
     check(fmca_h2_create_ex(&k, a->dim, a->m, Peval.data(), a->n, P.data(), deg, eta,
-                            interpolator, tree_builder, &a->h));
+                            interpolator, planner, &a->h));
   }
   return a;
 }
@@ -140,23 +140,24 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
   m.attr("KERNEL_PHI") = (int)FMCA_KERNEL_PHI;
   m.attr("INTERP_TENSOR_PRODUCT") = (int)FMCA_INTERP_TENSOR_PRODUCT;
   m.attr("INTERP_TOTAL_DEGREE") = (int)FMCA_INTERP_TOTAL_DEGREE;
-  m.attr("TREE_HOST") = (int)FMCA_TREE_HOST;
-  m.attr("TREE_DEVICE") = (int)FMCA_TREE_DEVICE;
+  m.attr("PLAN_HOST") = (int)FMCA_PLAN_HOST;
+  m.attr("PLAN_DEVICE") = (int)FMCA_PLAN_DEVICE;
   py::class_<Actor>(m, "Actor")
       .def(py::init(&create), py::arg("ktype"), py::arg("l"), py::arg("c"), py::arg("nu"),
            py::arg("Peval"), py::arg("P"), py::arg("deg"), py::arg("eta"),
            py::arg("interpolator") = (int)FMCA_INTERP_TENSOR_PRODUCT,
-           py::arg("tree_builder") = (int)FMCA_TREE_HOST)
+           py::arg("planner") = (int)FMCA_PLAN_HOST)
       .def("plan_report",
            [](const Actor &a) {
              double ts = 0, ps = 0;
-             int rb = 0, cb = 0;
-             check(fmca_h2_plan_report(a.h, &ts, &ps, &rb, &cb));
+             int where[3] = {0, 0, 0};
+             check(fmca_h2_plan_report(a.h, &ts, &ps, where));
              py::dict d;
              d["tree_s"] = ts;
              d["pattern_s"] = ps;
-             d["rows_builder"] = rb;
-             d["cols_builder"] = cb;
+             d["rows_tree"] = where[0];
+             d["cols_tree"] = where[1];
+             d["blocks"] = where[2];
              return d;
            })
       .def_readonly("dim", &Actor::dim)

@@ -79,31 +79,36 @@ fmca_status fmca_h2_create_interp(const fmca_kernel_desc *kernel, int dim, int64
                                   const double *Peval, int64_t n, const double *P, int deg,
                                   double eta, int interpolator, fmca_h2_actor **actor);
 
-/* Where fmca_h2_create_ex builds the cluster trees. */
-enum { FMCA_TREE_HOST = 0, FMCA_TREE_DEVICE = 1 };
+/* Where fmca_h2_create_ex plans. */
+enum { FMCA_PLAN_HOST = 0, FMCA_PLAN_DEVICE = 1 };
 
-/* fmca_h2_create_interp with a choice of tree builder.  FMCA_TREE_DEVICE runs
- * the O(n) work of ClusterTreeInitializer<ClusterTree>::init
- * (ClusterTreeInitializer_ClusterTree.h:24-132, CardinalityBisection in
- * ClusterTreeSplitter.h:60-81), which H2kernelActor's H2ClusterTree members
- * run in its constructor (H2kernelActor.h:21-27), on the current HIP device:
- * the points are copied there for the build and freed after it (this
- * transient memory is not counted by fmca_set_device_memory_limit).  The trees
- * equal the host build's bit for bit.  Where coordinates are equal at a split
- * point or inside a leaf, or a coordinate is -0.0, the tree is built on the
- * host instead (fmca_h2_plan_report says so).  FMCA_ERR_DEVICE /
- * FMCA_ERR_NO_DEVICE / FMCA_ERR_OUT_OF_MEMORY if the device build fails. */
+/* fmca_h2_create_interp with a choice of planner.  FMCA_PLAN_DEVICE runs on
+ * the current HIP device what H2kernelActor's constructor runs on the CPU
+ * (H2kernelActor.h:21-30):
+ *  - the O(n) work of ClusterTreeInitializer<ClusterTree>::init for both trees
+ *    (ClusterTreeInitializer_ClusterTree.h:24-132, CardinalityBisection in
+ *    ClusterTreeSplitter.h:60-81): the points are copied there for the build
+ *    and freed after it;
+ *  - the level-by-level expansion of H2MatrixBase::computePattern
+ *    (H2MatrixBase.h:160-191) with CompareCluster (Admissibility.h:95-115).
+ * The device memory they use is transient and not counted by
+ * fmca_set_device_memory_limit.  The plan equals the host planner's bit for
+ * bit.  Where coordinates are equal at a split point or inside a leaf, or a
+ * coordinate is -0.0, that tree is built on the host; where a cluster pair's
+ * admissibility lies within rounding of eta, the expansion runs on the host
+ * (fmca_h2_plan_report says which).  FMCA_ERR_DEVICE / FMCA_ERR_NO_DEVICE /
+ * FMCA_ERR_OUT_OF_MEMORY if a device step fails. */
 fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m,
                               const double *Peval, int64_t n, const double *P, int deg,
-                              double eta, int interpolator, int tree_builder,
+                              double eta, int interpolator, int planner,
                               fmca_h2_actor **actor);
 
 /* Seconds the create call spent building the trees and the block pattern,
- * and where each tree was built: 0 host, 1 device, 2 device asked for but
- * built on the host.  Null outputs are skipped.  No reference counterpart
- * (FMCA reports no setup times). */
+ * and where (where[0] row tree, where[1] column tree, where[2] block
+ * expansion): 0 host, 1 device, 2 device asked for, host used.  Null outputs
+ * are skipped.  No reference counterpart (FMCA reports no setup times). */
 fmca_status fmca_h2_plan_report(const fmca_h2_actor *actor, double *tree_seconds,
-                                double *pattern_seconds, int *rows_builder, int *cols_builder);
+                                double *pattern_seconds, int *where);
 
 /* Interpolation data, as the interpolator's idcs(), Xi() and invV() return
  * them (TotalDegreeInterpolator.h:50-55, TensorProductInterpolator.h:83-85):

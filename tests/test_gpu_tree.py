@@ -1,4 +1,4 @@
-"""The cluster trees built on the GPU (tree_builder="device") against the host
+"""The cluster trees built on the GPU (planner="device") against the host
 planner, which is itself pinned bit for bit against the oracle's restatement
 of FMCA's ClusterTreeInitializer (tests/test_planner.py).
 
@@ -50,8 +50,9 @@ def _same_tree(Hd, Hh, which):
                               np.ascontiguousarray(ch[k]).view(np.uint64)), k
 
 
-def _block_set(r, c, lr):
-    return set(zip(r.tolist(), c.tolist(), lr.tolist()))
+def _same_blocks(Hd, Hh):
+    for a, b in zip(Hd.blocks(), Hh.blocks()):
+        assert np.array_equal(a, b)
 
 
 CASES = [
@@ -78,13 +79,14 @@ def test_device_tree_equals_host_tree(case):
     Pe = P if m is None else _points(kind, m, d, 2)
     ker = fmca_amd.CovarianceKernel("MATERN32", 0.4)
     Hh = fmca_amd.H2kernelActor(ker, Pe, P, deg, 0.8, interp)
-    Hd = fmca_amd.H2kernelActor(ker, Pe, P, deg, 0.8, interp, tree_builder="device")
+    Hd = fmca_amd.H2kernelActor(ker, Pe, P, deg, 0.8, interp, planner="device")
     rep = Hd.plan_report()
     print(f"  {rep}", flush=True)
-    assert rep["rows_builder"] == "device" and rep["cols_builder"] == "device"
+    assert rep["rows_tree"] == "device" and rep["cols_tree"] == "device"
+    assert rep["blocks"] == "device"
     for which in (0, 1):
         _same_tree(Hd, Hh, which)
-    assert _block_set(*Hd.blocks()) == _block_set(*Hh.blocks())
+    _same_blocks(Hd, Hh)
     if n <= 100_000 and deg <= 6:
         x = np.asfortranarray(np.random.default_rng(3).standard_normal((n, 8)))
         assert rel_err(Hd.action(x), Hh.action(x)) <= 1e-14
@@ -97,10 +99,30 @@ def test_device_tree_falls_back_to_host_on_ties(kind):
     P = _points(kind, 20_000, 3, 5)
     ker = fmca_amd.CovarianceKernel("EXPONENTIAL", 0.5)
     Hh = fmca_amd.H2kernelActor(ker, P, P, 3, 0.8)
-    Hd = fmca_amd.H2kernelActor(ker, P, P, 3, 0.8, tree_builder="device")
-    assert Hd.plan_report()["rows_builder"] == "host-fallback"
+    Hd = fmca_amd.H2kernelActor(ker, P, P, 3, 0.8, planner="device")
+    assert Hd.plan_report()["rows_tree"] == "host-fallback"
     _same_tree(Hd, Hh, 0)
-    assert _block_set(*Hd.blocks()) == _block_set(*Hh.blocks())
+    _same_blocks(Hd, Hh)
+
+
+@pytest.mark.parametrize("eta,where", [(2.5, "host-fallback"), (0.8, "device"), (0.0, "device")])
+def test_device_blocks_on_an_integer_grid(eta, where):
+    """Points 0, 1, ..., n-1 on a line: boxes, radii and distances are
+    (half-)integers.  At eta = 2.5, 2,046 of the pairs the expansion tests have
+    max(r_t, r_s) == eta * dist exactly (counted on the CPU from the host
+    tree), so the device expansion must hand over to the host.  At eta = 0.8
+    none does, and eta = 0 (MultipoleFunctionEvaluator's) never can."""
+    import fmca_amd
+    print(f"[case] device blocks integer grid eta={eta}", flush=True)
+    P = np.asfortranarray(np.arange(6000, dtype=float)[None, :])
+    ker = fmca_amd.CovarianceKernel("EXPONENTIAL", 50.0)
+    Hh = fmca_amd.H2kernelActor(ker, P, P, 3, eta)
+    Hd = fmca_amd.H2kernelActor(ker, P, P, 3, eta, planner="device")
+    rep = Hd.plan_report()
+    print(f"  {rep}", flush=True)
+    assert rep["rows_tree"] == "device" and rep["blocks"] == where
+    _same_tree(Hd, Hh, 0)
+    _same_blocks(Hd, Hh)
 
 
 def test_device_tree_at_one_million_points():
@@ -112,9 +134,11 @@ def test_device_tree_at_one_million_points():
     P = W.uniform(1_000_000, 3, 0)
     ker = fmca_amd.CovarianceKernel("MATERN32", 0.5)
     Hh = fmca_amd.H2kernelActor(ker, P, P, 4, 0.8)
-    Hd = fmca_amd.H2kernelActor(ker, P, P, 4, 0.8, tree_builder="device")
+    Hd = fmca_amd.H2kernelActor(ker, P, P, 4, 0.8, planner="device")
     print(f"  host {Hh.plan_report()}  device {Hd.plan_report()}", flush=True)
-    assert Hd.plan_report()["rows_builder"] == "device"
+    assert Hd.plan_report()["rows_tree"] == "device"
+    assert Hd.plan_report()["blocks"] == "device"
     _same_tree(Hd, Hh, 0)
+    _same_blocks(Hd, Hh)
     assert Hd.info()["far_blocks"] == Hh.info()["far_blocks"]
     assert Hd.info()["near_blocks"] == Hh.info()["near_blocks"]

@@ -304,3 +304,33 @@ def test_variant_groups_argument_checks_without_device():
     assert call([0, 1], 33) == 1         # more columns than groups
     assert call([0, 1], 16) == 1         # last group empty
     assert lib.fmca_h2_destroy(h) == 0
+
+
+def test_planner_choice_without_device():
+    """planner="auto" plans on the host when no GPU is visible; "device" then
+    fails with a status (no crash); the C ABI rejects an unknown planner."""
+    P = uniform_points(3000, 3, seed=4)
+    ker = fmca_amd.CovarianceKernel("MATERN32", 0.5)
+    H = fmca_amd.H2kernelActor(ker, P, P, 3, 0.8)
+    rep = H.plan_report()
+    assert rep["rows_tree"] == rep["cols_tree"] == rep["blocks"] == "host"
+    assert rep["tree_s"] >= 0 and rep["pattern_s"] >= 0
+    with pytest.raises(RuntimeError, match="device"):
+        fmca_amd.H2kernelActor(ker, P, P, 3, 0.8, planner="device")
+    with pytest.raises(ValueError):
+        fmca_amd.H2kernelActor(ker, P, P, 3, 0.8, planner="gpu")
+    lib = ctypes.CDLL(LIB)
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("ktype", ctypes.c_char_p), ("l", ctypes.c_double), ("c", ctypes.c_double),
+                    ("nu", ctypes.c_double)]
+    desc = Desc(b"matern32", 0.5, 1.0, 1.0)
+    h = ctypes.c_void_p()
+    ptr = P.ctypes.data_as(ctypes.c_void_p)
+    args = (ctypes.byref(desc), 3, ctypes.c_int64(3000), ptr, ctypes.c_int64(3000), ptr, 3,
+            ctypes.c_double(0.8), 0)
+    assert lib.fmca_h2_create_ex(*args, 7, ctypes.byref(h)) == 1
+    assert lib.fmca_h2_create_ex(*args, 0, ctypes.byref(h)) == 0 and h.value
+    where = (ctypes.c_int * 3)(9, 9, 9)
+    assert lib.fmca_h2_plan_report(h, None, None, where) == 0 and list(where) == [0, 0, 0]
+    assert lib.fmca_h2_destroy(h) == 0
