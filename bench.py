@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--ridge", type=float, default=100.0)
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
-                            "total-degree"],
+                            "total-degree", "plan"],
                    help="matvec: the headline line (default); the others print a side line "
                         "for the other configurations")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -322,6 +322,31 @@ def side_autograd(args):
     return res
 
 
+def side_plan(args):
+    """Planning (cluster tree + block pattern) of the n = 8e6 and n = 1e6
+    uniform 3-D workloads, degree 4, eta 0.8, on the host and on the GPU: the
+    create call's seconds and the plan report.  The GPU plan is timed twice
+    (the first call also loads the planner's kernels)."""
+    import torch
+    import fmca_amd
+    torch.zeros(1, device="cuda")
+    res = {"workload": "plan: uniform 3D, deg 4 (q=125), eta 0.8; trees + block pattern"}
+    for n in (8_000_000, 1_000_000):
+        P = np.asfortranarray(np.random.default_rng(0).random((3, n)))
+        K = fmca_amd.CovarianceKernel("EXPONENTIAL", 0.25)
+        runs = {}
+        for tag, planner in (("device_first", "device"), ("device", "device"), ("host", "host")):
+            t0 = time.perf_counter()
+            H = fmca_amd.H2kernelActor(K, P, P, 4, 0.8, planner=planner)
+            runs[tag] = {"create_s": time.perf_counter() - t0, **H.plan_report()}
+            info = H.info()
+            del H
+        runs["clusters"] = info["row_clusters"]
+        runs["blocks"] = info["far_blocks"] + info["near_blocks"]
+        res[f"n={n}"] = runs
+    return res
+
+
 def side_large(args):
     """n = 8e6 uniform 3-D, EXPONENTIAL l = 0.25, deg 4, eta 0.8, 16 rhs:
     plan time, FP32 forward, backward (d alpha, d l) with moments recomputed,
@@ -386,7 +411,7 @@ def main():
 
     side = {"small-dense": side_small_dense, "clustered-cg": side_clustered_cg,
             "autograd": side_autograd, "large": side_large,
-            "total-degree": side_total_degree}.get(args.workload)
+            "total-degree": side_total_degree, "plan": side_plan}.get(args.workload)
     if side is not None:
         print(json.dumps(side(args)), flush=True)
         return
