@@ -214,7 +214,10 @@ __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
 
 // Value, dk/dl and phi of one entry (ev[KV_VALUE], ev[KV_DL], ev[KV_PHI]) for
 // the base kernel KT (kp.kt % KT_COUNT at run time when KT < 0).  Evaluated
-// together, the three share the distance, square root and exponential.
+// together, the three share the distance, square root and exponential.  A
+// variant outside kp.vmask (uniform over the launch) is not needed by any
+// column tile: where it costs extra work (a division, a separate evaluation)
+// it is skipped and left 0.
 template <int KT, typename T>
 __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T ev[3]) {
   using M = Math<T>;
@@ -223,7 +226,8 @@ __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T
     const T r = M::sqrt_(r2), e = exp_neg_scaled(r, kp);
     ev[KV_VALUE] = e;
     ev[KV_DL] = r * (kp.s * kp.s) * e;
-    ev[KV_PHI] = r > T(0) ? -(kp.s / r) * e : T(0);
+    ev[KV_PHI] = T(0);
+    if (kp.vmask & (1 << KV_PHI)) ev[KV_PHI] = r > T(0) ? -(kp.s / r) * e : T(0);
   } else if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN) {
     // the shared exponential; dk/dl = -(r^2 / l) phi for these f(r / l)
     T phi;
@@ -244,16 +248,17 @@ __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T
     ev[KV_PHI] = phi;
     ev[KV_DL] = -r2 * kp.inv_l * phi;
   } else if constexpr (KT >= 0) {
-    ev[KV_VALUE] = kernel_r2<KT, T>(r2, kp);
-    ev[KV_DL] = kernel_r2<KT + KT_COUNT * KV_DL, T>(r2, kp);
-    ev[KV_PHI] = kernel_r2<KT + KT_COUNT * KV_PHI, T>(r2, kp);
+    ev[KV_VALUE] = ev[KV_DL] = ev[KV_PHI] = T(0);
+    if (kp.vmask & (1 << KV_VALUE)) ev[KV_VALUE] = kernel_r2<KT, T>(r2, kp);
+    if (kp.vmask & (1 << KV_DL)) ev[KV_DL] = kernel_r2<KT + KT_COUNT * KV_DL, T>(r2, kp);
+    if (kp.vmask & (1 << KV_PHI)) ev[KV_PHI] = kernel_r2<KT + KT_COUNT * KV_PHI, T>(r2, kp);
   } else {
     KernelParams<T> k = kp;
     const int base = kp.kt % KT_COUNT;
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
       k.kt = base + KT_COUNT * v;
-      ev[v] = kernel_r2<-1, T>(r2, k);
+      ev[v] = (kp.vmask & (1 << v)) ? kernel_r2<-1, T>(r2, k) : T(0);
     }
   }
 }
