@@ -147,6 +147,15 @@ class H2kernelActor:
         self.n = self._a.n
         self.dim = self._a.dim
 
+    def assemble(self, precision: int = FP64) -> int:
+        """H2Matrix::computeH2Matrix: store every coupling matrix, dense block
+        and leaf basis in device memory (returns the bytes held); the plain
+        actions of that precision then stream them (operator*)."""
+        return self._a.assemble(precision)
+
+    def release_assembled(self) -> None:
+        self._a.release_assembled()
+
     def plan_report(self) -> dict:
         """Planning seconds (tree_s, pattern_s) and where the row tree, the
         column tree and the block expansion ran (rows_tree, cols_tree, blocks:

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-SOURCES_HIP = ["kernels.hip", "kernels_mfma.hip", "blas1.hip", "tree_device.hip",
+SOURCES_HIP = ["kernels.hip", "kernels_mfma.hip", "blas1.hip", "tree_device.hip", "assembled.hip",
                "pattern_device.hip"]
 SOURCES_CPP = ["plan.cpp", "actor.cpp"]
 

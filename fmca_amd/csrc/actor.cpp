@@ -191,6 +191,16 @@ struct fmca_h2_actor {
   Workspace<float> ws32;
   DevBuf stage_in, stage_out;
   DevBuf cg_ws, cg_scal;  // CG vectors and per-column scalars
+  // assembled blocks (fmca_h2_assemble) of precision st_prec (-1: none)
+  DevBuf st_far, st_near, st_near_off, st_v_off, st_far_tgt, st_near_leaf;
+  int st_prec = -1, st_Rq = 0, st_Kq = 0;
+  int64_t st_bytes = 0;
+  void release_stored() {
+    for (DevBuf *b : {&st_far, &st_near, &st_near_off, &st_v_off, &st_far_tgt, &st_near_leaf})
+      b->release();
+    st_prec = -1;
+    st_bytes = 0;
+  }
   hipStream_t stream = nullptr;
   std::mutex mtx;
   // per-phase profiling (events recorded on the launch stream)
@@ -436,10 +446,74 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
     a->pending.push_back(ev);
     pe = &a->pending.back();
   }
+  // the assembled blocks serve plain forward actions of their precision
+  StoredBlocks<T> st;
+  const bool use_stored = a->st_prec == (sizeof(T) == 8 ? FMCA_PRECISION_FP64 : FMCA_PRECISION_FP32) &&
+                          !transpose && groups.uniform() && groups.v[0] == KV_VALUE;
+  if (use_stored) {
+    st.far = a->st_far.template as<T>();
+    st.near = a->st_near.template as<T>();
+    st.near_off = a->st_near_off.template as<int64_t>();
+    st.v_off = a->st_v_off.template as<int64_t>();
+    st.Rq = a->st_Rq;
+    st.Kq = a->st_Kq;
+  }
   hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
                                   interp_params<T>(a), kernel_params<T>(a, groups), d_rhs, ld_rhs,
-                                  (int)nrhs, d_out, ld_out, natural, buf, stream, pe);
+                                  (int)nrhs, d_out, ld_out, natural, buf, stream, pe,
+                                  use_stored ? &st : nullptr);
   return from_hip(e, "launch_action");
+}
+
+// Writes the assembled blocks of precision T (fmca_h2_assemble).
+template <typename T>
+fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
+                     const DevTreeReals<T> &r_cols) {
+  const Tree &tr = a->rows();
+  const Pattern &pat = a->pat;
+  const int q = a->ip.q;
+  const int Rq = (q + 15) & ~15, Kq = stored_kpad<T>(q);
+  const int nl = (int)tr.leaves.size();
+  // block -> target tables and the offsets of the leaf blocks
+  std::vector<int32_t> far_tgt(pat.far_src.size()), near_leaf(pat.near_src.size());
+  for (int t = 0; t < tr.ncl; ++t)
+    for (int b = pat.far_ptr[t]; b < pat.far_ptr[t + 1]; ++b) far_tgt[b] = t;
+  std::vector<int64_t> near_off(pat.near_src.size()), v_off(nl);
+  int64_t off = 0;
+  const Tree &tc = a->cols();
+  for (int k = 0; k < nl; ++k) {
+    const int64_t Rt = (tr.size[tr.leaves[k]] + 15) & ~15;
+    v_off[k] = off;
+    off += Rt * Kq;
+    for (int e = pat.near_ptr[k]; e < pat.near_ptr[k + 1]; ++e) {
+      near_leaf[e] = k;
+      near_off[e] = off;
+      off += Rt * stored_kpad<T>((int)tc.size[pat.near_src[e]]);
+    }
+  }
+  const int64_t nfar = (int64_t)pat.far_src.size(), nnear = (int64_t)pat.near_src.size();
+  HIP_TRY(a->st_far.alloc((size_t)nfar * Rq * Kq * sizeof(T)));
+  HIP_TRY(a->st_near.alloc((size_t)off * sizeof(T)));
+  HIP_TRY(a->st_near_off.upload(near_off));
+  HIP_TRY(a->st_v_off.upload(v_off));
+  HIP_TRY(a->st_far_tgt.upload(far_tgt));
+  HIP_TRY(a->st_near_leaf.upload(near_leaf));
+  const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
+  DevTreeView<T> v_rows = make_view<T>(tr, a->d_rows_i, r_rows);
+  DevTreeView<T> v_cols = make_view<T>(tc, i_cols, r_cols);
+  HIP_TRY(launch_assemble<T>(v_rows, v_cols, a->d_pat.view(pat), interp_params<T>(a),
+                             kernel_params<T>(a, VariantGroups::single(KV_VALUE)), nfar,
+                             a->st_far_tgt.template as<int32_t>(), nnear,
+                             a->st_near_leaf.template as<int32_t>(), a->st_far.template as<T>(),
+                             a->st_near.template as<T>(), a->st_near_off.template as<int64_t>(),
+                             a->st_v_off.template as<int64_t>(), Rq, Kq, a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  a->st_Rq = Rq;
+  a->st_Kq = Kq;
+  a->st_bytes = (int64_t)((size_t)nfar * Rq * Kq + off) * (int64_t)sizeof(T) +
+                (int64_t)(near_off.size() + v_off.size()) * 8 +
+                (int64_t)(far_tgt.size() + near_leaf.size()) * 4;
+  return FMCA_OK;
 }
 
 fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
@@ -712,10 +786,46 @@ fmca_status fmca_h2_set_kernel(fmca_h2_actor *actor, const fmca_kernel_desc *ker
   if (!(kernel->l > 0) || !std::isfinite(kernel->l))
     return fail(FMCA_ERR_INVALID_ARGUMENT, "lengthscale must be positive");
   std::lock_guard<std::mutex> lock(actor->mtx);
+  // assembled blocks hold the old kernel's entries
+  if (kt != actor->kt || kernel->l != actor->l || kernel->c != actor->c || kernel->nu != actor->nu)
+    actor->release_stored();
   actor->kt = kt;
   actor->l = kernel->l;
   actor->c = kernel->c;
   actor->nu = kernel->nu;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_assemble(fmca_h2_actor *actor, int precision, int64_t *bytes) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (actor->ip.q > kStoredMaxRows / 2)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "assembled blocks need q <= 128");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  fmca_status st = ensure_device(actor);
+  if (st != FMCA_OK) return st;
+  actor->release_stored();
+  st = precision == FMCA_PRECISION_FP64
+           ? assemble<double>(actor, actor->d_rows_64,
+                              actor->shared ? actor->d_rows_64 : actor->d_cols_64)
+           : assemble<float>(actor, actor->d_rows_32,
+                             actor->shared ? actor->d_rows_32 : actor->d_cols_32);
+  if (st != FMCA_OK) {
+    const std::string msg = fmca_last_error();
+    actor->release_stored();
+    return fail(st, msg);
+  }
+  actor->st_prec = precision;
+  if (bytes) *bytes = actor->st_bytes;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_release_assembled(fmca_h2_actor *actor) {
+  if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  if (actor->uploaded) HIP_TRY(hipSetDevice(actor->device));
+  actor->release_stored();
   return FMCA_OK;
 }
 

@@ -129,8 +129,33 @@ struct PhaseEvents {
   hipEvent_t ev[PH_COUNT + 1];
 };
 
-// Enqueues the whole matrix-free action out = K(rows, cols) * rhs on `stream`.
-// rhs / out are FP64 column-major device arrays; perm_in / perm_out select the
+// The blocks of an assembled H2 matrix (fmca_h2_assemble), kept in HBM as
+// v_mfma 16x16x4 left-operand tiles.  In a block of R rows (a multiple of 16)
+// and K columns (a multiple of 4 G), the 16 x 4 tile (rt, ks) gives lane
+// (j % 4) * 16 + i % 16 element (i, j), the value the instruction takes from
+// that lane; the G = 16 / sizeof(T) consecutive k-steps ks = g G + u of a
+// lane are adjacent, at ((rt * K / (4 G) + g) * 64 + lane) * G + u, so one
+// wave reads G tiles with one 16-byte load per lane.  Padding is zero.
+constexpr int kStoredMaxRows = 256;  // rows of a stored block (q and leaf sizes)
+template <typename T>
+constexpr int kStoredGroup = 16 / (int)sizeof(T);
+template <typename T>
+__host__ __device__ inline int stored_kpad(int k) {
+  constexpr int m = 4 * kStoredGroup<T>;
+  return (k + m - 1) / m * m;
+}
+template <typename T>
+struct StoredBlocks {
+  const T *far = nullptr;             // far block b (far list order): S_ts, Rq x Kq
+  const T *near = nullptr;            // per leaf: V_t^T (Rt x Kq), then its near blocks
+  const int64_t *near_off = nullptr;  // near list entry e -> offset of K(P_t, P_s), Rt x kpad|s|
+  const int64_t *v_off = nullptr;     // leaf index k -> offset of V_t^T
+  int Rq = 0, Kq = 0;                 // pad16(q), stored_kpad(q)
+};
+
+// Enqueues the whole action out = K(rows, cols) * rhs on `stream`: matrix-free,
+// or with the coupling and leaf-output phases streaming stored blocks.
+// rhs / out are FP64 column-major device arrays; natural_order selects the
 // natural order (true) or FMCA's cluster order (false).
 template <typename T>
 hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
@@ -138,7 +163,26 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const DevPatternView &pat, const InterpParams<T> &ip,
                          const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
-                         hipStream_t stream, const PhaseEvents *events = nullptr);
+                         hipStream_t stream, const PhaseEvents *events = nullptr,
+                         const StoredBlocks<T> *stored = nullptr);
+
+// Writes every block of the assembled matrix (assembled.hip).  far_tgt: far
+// block -> target cluster; near_leaf: near list entry -> leaf index.
+template <typename T>
+hipError_t launch_assemble(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                           const DevPatternView &pat, const InterpParams<T> &ip,
+                           const KernelParams<T> &kp, int64_t nfar, const int32_t *far_tgt,
+                           int64_t nnear, const int32_t *near_leaf, T *far, T *near,
+                           const int64_t *near_off, const int64_t *v_off, int Rq, int Kq,
+                           hipStream_t stream);
+
+// The coupling (output = false: tlhs) or the leaf output (output = true: out)
+// from stored blocks.
+template <typename T>
+hipError_t launch_stored(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,
+                         const DevPatternView &pat, const StoredBlocks<T> &st, int q, int nrhs,
+                         const T *xs, const T *mom, T *tlhs, double *out, int64_t ldo,
+                         bool natural, hipStream_t stream);
 
 // Matrix-core (MFMA) forms of the coupling (output = false) and the leaf
 // output (output = true), used from kMfmaMinRhs right-hand sides on when

@@ -775,7 +775,8 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                     const HostLevels &lrows, const HostLevels &lcols, const DevPatternView &pat,
                     const InterpParams<T> &ip, const KernelParams<T> &kp, const double *rhs,
                     int64_t ld_rhs, int nrhs, double *out, int64_t ld_out, bool natural,
-                    ActionBuffers<T> buf, hipStream_t stream, const PhaseEvents *ev) {
+                    ActionBuffers<T> buf, hipStream_t stream, const PhaseEvents *ev,
+                    const StoredBlocks<T> *stored) {
   const int q = ip.q, P1 = ip.P1;
   hipError_t err;
   auto mark = [&](int k) {
@@ -883,12 +884,19 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   // 4. coupling (every target cluster is written, empty far lists give zeros)
   mark(PH_COUPLING);
   if (ip.td) basis_change(cols.ncl, 1, buf.mom);
-  if (pat.nfar_split > 0)
-    hipLaunchKernelGGL((k_zero_clusters<T>), dim3(pat.nfar_split), dim3(kBlock), 0, stream,
-                       pat.far_split, (int64_t)q * nrhs, buf.tlhs);
-  if ((err = dispatch_nr<T, D>(false, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
-                               stream)) != hipSuccess)
-    return err;
+  if (stored) {
+    // stored coupling matrices; one workgroup per target writes all of it
+    if ((err = launch_stored<T>(false, rows, cols, pat, *stored, q, nrhs, buf.xs, buf.mom,
+                                buf.tlhs, out, ld_out, natural, stream)) != hipSuccess)
+      return err;
+  } else {
+    if (pat.nfar_split > 0)
+      hipLaunchKernelGGL((k_zero_clusters<T>), dim3(pat.nfar_split), dim3(kBlock), 0, stream,
+                         pat.far_split, (int64_t)q * nrhs, buf.tlhs);
+    if ((err = dispatch_nr<T, D>(false, rows, cols, pat, ip, kp, nrhs, buf, out, ld_out, natural,
+                                 stream)) != hipSuccess)
+      return err;
+  }
   if (ip.td) basis_change(rows.ncl, 0, buf.tlhs);
   // 5. downward transfers, top level first
   mark(PH_DOWNWARD_TRANSFER);
@@ -924,6 +932,12 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 6. leaf output: far field through V^T plus the near field
   mark(PH_LEAF_OUTPUT);
+  if (stored) {
+    err = launch_stored<T>(true, rows, cols, pat, *stored, q, nrhs, buf.xs, buf.mom, buf.tlhs,
+                           out, ld_out, natural, stream);
+    mark(PH_COUNT);
+    return err;
+  }
   if (pat.nnear_split > 0)
     hipLaunchKernelGGL((k_zero_leaf_rows<T>), dim3(pat.nnear_split), dim3(kBlock), 0, stream,
                        rows, pat.near_split, nrhs, out, ld_out, natural ? 1 : 0);
@@ -967,17 +981,18 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const DevPatternView &pat, const InterpParams<T> &ip,
                          const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
-                         hipStream_t stream, const PhaseEvents *events) {
+                         hipStream_t stream, const PhaseEvents *events,
+                         const StoredBlocks<T> *stored) {
   switch (rows.dim) {
     case 1:
       return action_d<T, 1>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream, events);
+                            ld_out, natural_order, buf, stream, events, stored);
     case 2:
       return action_d<T, 2>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream, events);
+                            ld_out, natural_order, buf, stream, events, stored);
     case 3:
       return action_d<T, 3>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream, events);
+                            ld_out, natural_order, buf, stream, events, stored);
   }
   return hipErrorInvalidValue;
 }
@@ -987,12 +1002,14 @@ template hipError_t launch_action<double>(const DevTreeView<double> &, const Dev
                                           const DevPatternView &, const InterpParams<double> &,
                                           const KernelParams<double> &, const double *, int64_t,
                                           int, double *, int64_t, bool, ActionBuffers<double>,
-                                          hipStream_t, const PhaseEvents *);
+                                          hipStream_t, const PhaseEvents *,
+                                          const StoredBlocks<double> *);
 template hipError_t launch_action<float>(const DevTreeView<float> &, const DevTreeView<float> &,
                                          const HostLevels &, const HostLevels &,
                                          const DevPatternView &, const InterpParams<float> &,
                                          const KernelParams<float> &, const double *, int64_t, int,
                                          double *, int64_t, bool, ActionBuffers<float>,
-                                         hipStream_t, const PhaseEvents *);
+                                         hipStream_t, const PhaseEvents *,
+                                         const StoredBlocks<float> *);
 
 }  // namespace fmca_amd

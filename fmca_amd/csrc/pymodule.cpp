@@ -147,6 +147,16 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
            py::arg("Peval"), py::arg("P"), py::arg("deg"), py::arg("eta"),
            py::arg("interpolator") = (int)FMCA_INTERP_TENSOR_PRODUCT,
            py::arg("planner") = (int)FMCA_PLAN_HOST)
+      .def("assemble",
+           [](Actor &a, int precision) {
+             int64_t bytes = 0;
+             {
+               py::gil_scoped_release nogil;
+               check(fmca_h2_assemble(a.h, precision, &bytes));
+             }
+             return bytes;
+           })
+      .def("release_assembled", [](Actor &a) { check(fmca_h2_release_assembled(a.h)); })
       .def("plan_report",
            [](const Actor &a) {
              double ts = 0, ps = 0;

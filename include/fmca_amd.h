@@ -110,6 +110,23 @@ fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m
 fmca_status fmca_h2_plan_report(const fmca_h2_actor *actor, double *tree_seconds,
                                 double *pattern_seconds, int *where);
 
+/* H2Matrix::computeH2Matrix (H2MatrixBase.h:222-255): generate every coupling
+ * matrix S_ts = K(xi_t, xi_s), every dense block K(P_t, P_s) and every leaf
+ * basis V_t once and keep them in device memory, in the given precision, as
+ * matrix-core operand tiles.  Afterwards the plain actions of that precision
+ * (fmca_h2_action*, natural or cluster order; not the transposed, variant or
+ * CG forms) stream them instead of generating entries, as operator*
+ * (H2MatrixBase.h:107-109, matrix_vector_product_impl.h:17-98) applies the
+ * stored blocks.  One precision at a time; fmca_h2_set_kernel with other
+ * kernel parameters, and fmca_h2_release_assembled, free them.  *bytes (if
+ * not null): the device bytes held (about 4 B per entry in FP32, 8 in FP64,
+ * entries padded to 16-row / 4-column tiles).  FMCA_ERR_NOT_SUPPORTED for
+ * q > 128; FMCA_ERR_OUT_OF_MEMORY if they do not fit. */
+fmca_status fmca_h2_assemble(fmca_h2_actor *actor, int precision, int64_t *bytes);
+
+/* Frees the assembled blocks; later actions generate entries again. */
+fmca_status fmca_h2_release_assembled(fmca_h2_actor *actor);
+
 /* Interpolation data, as the interpolator's idcs(), Xi() and invV() return
  * them (TotalDegreeInterpolator.h:50-55, TensorProductInterpolator.h:83-85):
  * the interpolator id, q, the q multi-indices in FMCA's order (q x dim,
