@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--ridge", type=float, default=100.0)
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
-                            "total-degree", "plan"],
+                            "total-degree", "plan", "assembled"],
                    help="matvec: the headline line (default); the others print a side line "
                         "for the other configurations")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -322,6 +322,53 @@ def side_autograd(args):
     return res
 
 
+def side_assembled(args):
+    """The assembled H2 matrix (computeH2Matrix + operator*) against the
+    matrix-free action, uniform 3-D, MATERN32 l = 0.5, deg 4 (q = 125), eta
+    0.8: FP32 at n = 1e6 (32 and 1 rhs) and FP64 at n = 5e5 (32 rhs; the FP64
+    blocks of n = 1e6 would not fit next to the rest).  Per configuration:
+    assembly seconds, bytes held, both action times, the streamed bandwidth
+    (stored bytes / stored action time) and the relative difference."""
+    import torch
+    import fmca_amd
+    res = {"workload": "assembled: uniform 3D, MATERN32 l=0.5, deg 4 (q=125), eta 0.8"}
+    for n, prec, nrhs_list in ((1_000_000, fmca_amd.FP32, (32, 1)),
+                               (500_000, fmca_amd.FP64, (32,))):
+        P = np.asfortranarray(np.random.default_rng(0).random((3, n)))
+        H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 4, 0.8)
+        name = "fp32" if prec == fmca_amd.FP32 else "fp64"
+        xs = {}
+        free = {}
+        for nrhs in nrhs_list:
+            x = torch.from_numpy(np.asfortranarray(
+                np.random.default_rng(1).standard_normal((n, nrhs)))).cuda()
+            x = x.t().contiguous().t() if nrhs > 1 else x
+            y = torch.empty(nrhs, n, dtype=torch.float64, device="cuda").t()
+            H.action_device(x, out=y, precision=prec)  # sizes the workspaces
+            free[nrhs] = _sync_time(lambda: H.action_device(x, out=y, precision=prec),
+                                    reps=max(3, args.steps))[0]
+            xs[nrhs] = (x, y.clone())
+        t0 = time.perf_counter()
+        nbytes = H.assemble(prec)
+        torch.cuda.synchronize()
+        t_asm = time.perf_counter() - t0
+        for nrhs in nrhs_list:
+            x, yf = xs[nrhs]
+            y = torch.empty_like(yf.t()).t()
+            H.action_device(x, out=y, precision=prec)
+            ts = _sync_time(lambda: H.action_device(x, out=y, precision=prec),
+                            reps=max(3, args.steps))[0]
+            diff = (torch.linalg.norm(y - yf) / torch.linalg.norm(yf)).item()
+            res[f"{name}_n{n}_rhs{nrhs}"] = {
+                "matrix_free_ms": free[nrhs] * 1e3, "assembled_ms": ts * 1e3,
+                "speedup": free[nrhs] / ts, "assemble_s": t_asm, "bytes": nbytes,
+                "streamed_TBps": nbytes / ts / 1e12, "rel_diff": diff,
+                "mpts": n * nrhs / ts / 1e6}
+        H.release_assembled()
+        del H
+    return res
+
+
 def side_plan(args):
     """Planning (cluster tree + block pattern) of the n = 8e6 and n = 1e6
     uniform 3-D workloads, degree 4, eta 0.8, on the host and on the GPU: the
@@ -411,7 +458,8 @@ def main():
 
     side = {"small-dense": side_small_dense, "clustered-cg": side_clustered_cg,
             "autograd": side_autograd, "large": side_large,
-            "total-degree": side_total_degree, "plan": side_plan}.get(args.workload)
+            "total-degree": side_total_degree, "plan": side_plan,
+            "assembled": side_assembled}.get(args.workload)
     if side is not None:
         print(json.dumps(side(args)), flush=True)
         return
