@@ -334,3 +334,17 @@ def test_planner_choice_without_device():
     where = (ctypes.c_int * 3)(9, 9, 9)
     assert lib.fmca_h2_plan_report(h, None, None, where) == 0 and list(where) == [0, 0, 0]
     assert lib.fmca_h2_destroy(h) == 0
+
+
+def test_assemble_without_device_reports_status():
+    """fmca_h2_assemble needs the device: without one it returns a status
+    (no crash), and the actor stays usable for host queries."""
+    P = uniform_points(2000, 3, seed=8)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 3, 0.8)
+    with pytest.raises(RuntimeError):
+        H.assemble(fmca_amd.FP64)
+    H.release_assembled()
+    assert H.info()["n"] == 2000
+    lib = ctypes.CDLL(LIB)
+    assert lib.fmca_h2_assemble(None, 0, None) == 1
+    assert lib.fmca_h2_release_assembled(None) == 1
