@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -192,11 +193,13 @@ struct fmca_h2_actor {
   DevBuf stage_in, stage_out;
   DevBuf cg_ws, cg_scal;  // CG vectors and per-column scalars
   // assembled blocks (fmca_h2_assemble) of precision st_prec (-1: none)
-  DevBuf st_far, st_near, st_near_off, st_v_off, st_far_tgt, st_near_leaf;
+  DevBuf st_far, st_near, st_near_off, st_v_off, st_far_tgt, st_near_leaf, st_far_order,
+      st_near_order;
   int st_prec = -1, st_Rq = 0, st_Kq = 0;
   int64_t st_bytes = 0;
   void release_stored() {
-    for (DevBuf *b : {&st_far, &st_near, &st_near_off, &st_v_off, &st_far_tgt, &st_near_leaf})
+    for (DevBuf *b : {&st_far, &st_near, &st_near_off, &st_v_off, &st_far_tgt, &st_near_leaf,
+                      &st_far_order, &st_near_order})
       b->release();
     st_prec = -1;
     st_bytes = 0;
@@ -455,6 +458,8 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
     st.near = a->st_near.template as<T>();
     st.near_off = a->st_near_off.template as<int64_t>();
     st.v_off = a->st_v_off.template as<int64_t>();
+    st.far_order = a->st_far_order.template as<int32_t>();
+    st.near_order = a->st_near_order.template as<int32_t>();
     st.Rq = a->st_Rq;
     st.Kq = a->st_Kq;
   }
@@ -491,6 +496,18 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
       off += Rt * stored_kpad<T>((int)tc.size[pat.near_src[e]]);
     }
   }
+  // launch orders of the streaming kernels: longest list first (stable)
+  std::vector<int32_t> far_order(tr.ncl), near_order(nl);
+  std::iota(far_order.begin(), far_order.end(), 0);
+  std::stable_sort(far_order.begin(), far_order.end(), [&](int x, int y) {
+    return pat.far_ptr[x + 1] - pat.far_ptr[x] > pat.far_ptr[y + 1] - pat.far_ptr[y];
+  });
+  std::vector<int64_t> leaf_work(nl);
+  for (int k = 0; k < nl; ++k)
+    leaf_work[k] = (k + 1 < nl ? v_off[k + 1] : off) - v_off[k];
+  std::iota(near_order.begin(), near_order.end(), 0);
+  std::stable_sort(near_order.begin(), near_order.end(),
+                   [&](int x, int y) { return leaf_work[x] > leaf_work[y]; });
   const int64_t nfar = (int64_t)pat.far_src.size(), nnear = (int64_t)pat.near_src.size();
   HIP_TRY(a->st_far.alloc((size_t)nfar * Rq * Kq * sizeof(T)));
   HIP_TRY(a->st_near.alloc((size_t)off * sizeof(T)));
@@ -498,6 +515,8 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
   HIP_TRY(a->st_v_off.upload(v_off));
   HIP_TRY(a->st_far_tgt.upload(far_tgt));
   HIP_TRY(a->st_near_leaf.upload(near_leaf));
+  HIP_TRY(a->st_far_order.upload(far_order));
+  HIP_TRY(a->st_near_order.upload(near_order));
   const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
   DevTreeView<T> v_rows = make_view<T>(tr, a->d_rows_i, r_rows);
   DevTreeView<T> v_cols = make_view<T>(tc, i_cols, r_cols);
@@ -512,7 +531,8 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
   a->st_Kq = Kq;
   a->st_bytes = (int64_t)((size_t)nfar * Rq * Kq + off) * (int64_t)sizeof(T) +
                 (int64_t)(near_off.size() + v_off.size()) * 8 +
-                (int64_t)(far_tgt.size() + near_leaf.size()) * 4;
+                (int64_t)(far_tgt.size() + near_leaf.size() + far_order.size() +
+                          near_order.size()) * 4;
   return FMCA_OK;
 }
 

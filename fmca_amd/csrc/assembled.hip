@@ -244,7 +244,8 @@ __device__ __forceinline__ void stream_item(const T *__restrict__ A, int K, int 
   }
 }
 
-// One workgroup per target: OUTPUT = false, a target cluster t and its far
+// One workgroup per target (in the order far_order / near_order, longest
+// list first): OUTPUT = false, a target cluster t and its far
 // list (tlhs_t = sum S_ts mom_s); OUTPUT = true, a target leaf and [V_t^T,
 // then its near blocks] (y_t = V_t^T tlhs_t + sum K(P_t, P_s) x_s), written
 // to out in natural (through perm) or cluster order.  The four waves form
@@ -266,14 +267,14 @@ __global__ __launch_bounds__(kTB) void k_stream(DevTreeView<T> rows, DevTreeView
   const int grp = wave / WS, wig = wave % WS;
   int t, nrt, nitems, bt = 0, sz = 0, k = 0;
   if (OUTPUT) {
-    k = blockIdx.x;
+    k = st.near_order[blockIdx.x];
     t = rows.leaves[k];
     bt = rows.begin[t];
     sz = rows.size[t];
     nrt = (sz + 15) / 16;
     nitems = 1 + pat.near_ptr[k + 1] - pat.near_ptr[k];
   } else {
-    t = blockIdx.x;
+    t = st.far_order[blockIdx.x];
     nrt = st.Rq / 16;
     nitems = pat.far_ptr[t + 1] - pat.far_ptr[t];
   }

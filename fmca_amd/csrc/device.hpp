@@ -150,6 +150,8 @@ struct StoredBlocks {
   const T *near = nullptr;            // per leaf: V_t^T (Rt x Kq), then its near blocks
   const int64_t *near_off = nullptr;  // near list entry e -> offset of K(P_t, P_s), Rt x kpad|s|
   const int64_t *v_off = nullptr;     // leaf index k -> offset of V_t^T
+  // launch orders, longest list first: workgroup -> target cluster / leaf index
+  const int32_t *far_order = nullptr, *near_order = nullptr;
   int Rq = 0, Kq = 0;                 // pad16(q), stored_kpad(q)
 };
 
