@@ -227,9 +227,10 @@ namespace {
 std::vector<double> cluster_order_points(const Tree &T, const double *P) {
   const int d = T.dim;
   std::vector<double> out((size_t)d * T.npts);
-  for (int i = 0; i < d; ++i)
-    for (int64_t j = 0; j < T.npts; ++j)
-      out[(size_t)i * T.npts + j] = P[i + (size_t)d * T.perm[j]];
+  parallel_ranges(T.npts, [&](int64_t lo, int64_t hi) {
+    for (int i = 0; i < d; ++i)
+      for (int64_t j = lo; j < hi; ++j) out[(size_t)i * T.npts + j] = P[i + (size_t)d * T.perm[j]];
+  });
   return out;
 }
 
@@ -242,7 +243,9 @@ hipError_t upload_ints(DevTreeInts &d, const Tree &T) {
   hipError_t e;
   std::vector<int32_t> b(T.begin.begin(), T.begin.end()), s(T.size.begin(), T.size.end());
   std::vector<uint32_t> pinv(T.perm.size());
-  for (size_t j = 0; j < T.perm.size(); ++j) pinv[T.perm[j]] = (uint32_t)j;
+  parallel_ranges((int64_t)T.perm.size(), [&](int64_t lo, int64_t hi) {
+    for (int64_t j = lo; j < hi; ++j) pinv[T.perm[j]] = (uint32_t)j;
+  });
   if ((e = d.perm.upload(T.perm)) != hipSuccess) return e;
   if ((e = d.pinv.upload(pinv)) != hipSuccess) return e;
   if ((e = d.begin.upload(b)) != hipSuccess) return e;
@@ -262,21 +265,25 @@ std::vector<double> leaf_unit_coords(const Tree &tr, const std::vector<double> &
   const int d = tr.dim;
   const size_t n = (size_t)tr.npts;
   std::vector<double> u(pts.size());
-  for (int c : tr.leaves)
-    for (int i = 0; i < d; ++i) {
-      const double mn = tr.bbmin[(size_t)c * d + i], sz = tr.bbsize[(size_t)c * d + i];
-      for (int64_t j = tr.begin[c]; j < tr.begin[c] + tr.size[c]; ++j)
-        u[i * n + j] = (pts[i * n + j] - mn) / sz;
+  parallel_ranges((int64_t)tr.leaves.size(), [&](int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; ++k) {
+      const int c = tr.leaves[k];
+      for (int i = 0; i < d; ++i) {
+        const double mn = tr.bbmin[(size_t)c * d + i], sz = tr.bbsize[(size_t)c * d + i];
+        for (int64_t j = tr.begin[c]; j < tr.begin[c] + tr.size[c]; ++j)
+          u[i * n + j] = (pts[i * n + j] - mn) / sz;
+      }
     }
+  });
   return u;
 }
 
 template <typename T>
 hipError_t upload_reals(DevTreeReals<T> &d, const Tree &tr, const std::vector<double> &pts,
-                        const std::vector<double> &transfer) {
+                        const std::vector<double> &unit, const std::vector<double> &transfer) {
   hipError_t e;
   if ((e = d.pts.upload(convert<T>(pts))) != hipSuccess) return e;
-  if ((e = d.unit.upload(convert<T>(leaf_unit_coords(tr, pts)))) != hipSuccess) return e;
+  if ((e = d.unit.upload(convert<T>(unit))) != hipSuccess) return e;
   if ((e = d.bbmin.upload(convert<T>(tr.bbmin))) != hipSuccess) return e;
   if ((e = d.bbsize.upload(convert<T>(tr.bbsize))) != hipSuccess) return e;
   return d.transfer.upload(convert<T>(transfer));
@@ -317,12 +324,16 @@ fmca_status ensure_device(fmca_h2_actor *a) {
   HIP_TRY(hipGetDevice(&a->device));
   HIP_TRY(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
   HIP_TRY(upload_ints(a->d_rows_i, a->rows()));
-  HIP_TRY(upload_reals<double>(a->d_rows_64, a->rows(), a->pts_rows, a->trans_rows));
-  HIP_TRY(upload_reals<float>(a->d_rows_32, a->rows(), a->pts_rows, a->trans_rows));
+  {
+    const std::vector<double> unit = leaf_unit_coords(a->rows(), a->pts_rows);
+    HIP_TRY(upload_reals<double>(a->d_rows_64, a->rows(), a->pts_rows, unit, a->trans_rows));
+    HIP_TRY(upload_reals<float>(a->d_rows_32, a->rows(), a->pts_rows, unit, a->trans_rows));
+  }
   if (!a->shared) {
     HIP_TRY(upload_ints(a->d_cols_i, a->cols()));
-    HIP_TRY(upload_reals<double>(a->d_cols_64, a->cols(), a->pts_cols, a->trans_cols));
-    HIP_TRY(upload_reals<float>(a->d_cols_32, a->cols(), a->pts_cols, a->trans_cols));
+    const std::vector<double> unit = leaf_unit_coords(a->cols(), a->pts_cols);
+    HIP_TRY(upload_reals<double>(a->d_cols_64, a->cols(), a->pts_cols, unit, a->trans_cols));
+    HIP_TRY(upload_reals<float>(a->d_cols_32, a->cols(), a->pts_cols, unit, a->trans_cols));
   }
   HIP_TRY(a->d_pat.upload(a->pat));
   if (a->ip.scheme == IS_TOTAL_DEGREE) {

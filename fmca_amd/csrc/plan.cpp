@@ -410,6 +410,11 @@ class HostSplitter final : public TreeSplitter {
 };
 }  // namespace
 
+void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)> &f) {
+  const int nw = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), n / 65536 + 1));
+  parallel_chunks(n, nw, [&](int, int64_t lo, int64_t hi) { f(lo, hi); });
+}
+
 void point_box(int dim, int64_t n, const double *P, double *mn, double *mx) {
   const int nw = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), n / 65536 + 1));
   for (int i = 0; i < dim; ++i) {

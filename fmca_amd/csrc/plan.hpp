@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -118,6 +119,10 @@ struct TreeSplitter {
   // Hand over the final permutation.
   virtual void permutation(std::vector<uint32_t> &perm) = 0;
 };
+
+// f(lo, hi) over contiguous chunks of [0, n) on the planner's worker threads
+// (FMCA_AMD_THREADS, else OMP_NUM_THREADS, else the hardware concurrency).
+void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)> &f);
 
 // Min / max of each coordinate over n points.
 void point_box(int dim, int64_t n, const double *P, double *mn, double *mx);
