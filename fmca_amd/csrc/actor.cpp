@@ -604,6 +604,26 @@ fmca_status host_action(fmca_h2_actor *a, const double *rhs, int64_t ld_rhs, int
 
 }  // namespace
 
+namespace fmca_amd {
+int device_malloc(void **p, size_t bytes) {
+  const size_t h = std::max<size_t>(bytes, 256);
+  const int64_t cap = g_dev_limit.load();
+  if (cap > 0 && g_dev_live.load() + (int64_t)h > cap) {
+    *p = nullptr;
+    return (int)hipErrorOutOfMemory;
+  }
+  const hipError_t e = hipMalloc(p, h);
+  if (e == hipSuccess) note_device_bytes((int64_t)h);
+  return (int)e;
+}
+
+void device_free(void *p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  note_device_bytes(-(int64_t)std::max<size_t>(bytes, 256));
+}
+}  // namespace fmca_amd
+
 extern "C" {
 
 const char *fmca_status_string(fmca_status s) {
@@ -631,6 +651,7 @@ const char *fmca_status_string(fmca_status s) {
 }
 
 const char *fmca_last_error(void) { return g_last_error.c_str(); }
+
 
 fmca_status fmca_h2_create(const fmca_kernel_desc *kernel, int dim, int64_t m,
                            const double *Peval, int64_t n, const double *P, int deg, double eta,
@@ -700,6 +721,11 @@ fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m
       }
       std::string err;
       const int r = build_tree_device(T, dim, cnt, pts, a->ip.q, &err);
+      if (r == -(int)hipErrorOutOfMemory) {  // no room on the device: plan on the host
+        build_tree(T, dim, cnt, pts, a->ip.q);
+        where = 2;
+        return FMCA_OK;
+      }
       if (r < 0) return fail(from_hip((hipError_t)(-r), "tree"), "device tree build: " + err);
       where = r == 1 ? 1 : 2;
       return FMCA_OK;
@@ -714,8 +740,9 @@ fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m
     } else {
       std::string err;
       const int r = expand_blocks_device(a->pat, a->rows(), a->cols(), eta, a->ip.q, &err);
-      if (r < 0) return fail(from_hip((hipError_t)(-r), "pattern"), "device block expansion: " + err);
-      if (r == 0) {
+      if (r < 0 && r != -(int)hipErrorOutOfMemory)
+        return fail(from_hip((hipError_t)(-r), "pattern"), "device block expansion: " + err);
+      if (r <= 0) {  // near the threshold, or no room on the device
         a->pat = Pattern();
         expand_blocks(a->pat, a->rows(), a->cols(), eta, a->ip.q);
       }

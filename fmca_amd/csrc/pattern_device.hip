@@ -161,16 +161,14 @@ template <typename T>
 struct DevArray {
   T *p = nullptr;
   size_t n = 0;
-  ~DevArray() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevArray() { device_free(p, n * sizeof(T)); }
   void reserve(size_t k) {
     if (k <= n) return;
-    if (p) hip_check(hipFree(p), "hipFree");
+    device_free(p, n * sizeof(T));
     p = nullptr;
     n = 0;
     k = std::max<size_t>(k + k / 4, 1024);
-    hip_check(hipMalloc(&p, k * sizeof(T)), "hipMalloc");
+    hip_check((hipError_t)device_malloc((void **)&p, k * sizeof(T)), "device allocation");
     n = k;
   }
 };

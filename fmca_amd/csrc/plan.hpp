@@ -6,6 +6,7 @@
 // planner can be exercised on a host without one.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <functional>
 #include <string>
@@ -126,6 +127,13 @@ void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)> &f);
 
 // Min / max of each coordinate over n points.
 void point_box(int dim, int64_t n, const double *P, double *mn, double *mx);
+
+// Device memory of the library's own accounting (actor.cpp): device_malloc
+// counts the bytes against the cap of fmca_set_device_memory_limit and
+// returns a hipError_t (hipErrorOutOfMemory beyond the cap); device_free
+// releases what device_malloc gave.
+int device_malloc(void **p, size_t bytes);
+void device_free(void *p, size_t bytes);
 
 // build_tree with the O(n) work on the current HIP device (tree_device.hip).
 // Returns 1 if built there, 0 if built on the host instead (equal coordinates

@@ -386,16 +386,15 @@ template <typename T>
 struct DevArray {
   T *p = nullptr;
   size_t n = 0;
-  ~DevArray() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevArray() { device_free(p, n * sizeof(T)); }
   // capacity for k elements; returns true if (re)allocated
   bool reserve(size_t k) {
     if (k <= n) return false;
-    if (p) hip_check(hipFree(p), "hipFree");
+    device_free(p, n * sizeof(T));
     p = nullptr;
     n = 0;
-    hip_check(hipMalloc(&p, std::max<size_t>(k, 1) * sizeof(T)), "hipMalloc");
+    hip_check((hipError_t)device_malloc((void **)&p, std::max<size_t>(k, 1) * sizeof(T)),
+              "device allocation");
     n = k;
     return true;
   }

@@ -91,13 +91,14 @@ enum { FMCA_PLAN_HOST = 0, FMCA_PLAN_DEVICE = 1 };
  *    and freed after it;
  *  - the level-by-level expansion of H2MatrixBase::computePattern
  *    (H2MatrixBase.h:160-191) with CompareCluster (Admissibility.h:95-115).
- * The device memory they use is transient and not counted by
+ * The device memory they use is transient and counted against
  * fmca_set_device_memory_limit.  The plan equals the host planner's bit for
  * bit.  Where coordinates are equal at a split point or inside a leaf, or a
  * coordinate is -0.0, that tree is built on the host; where a cluster pair's
- * admissibility lies within rounding of eta, the expansion runs on the host
- * (fmca_h2_plan_report says which).  FMCA_ERR_DEVICE / FMCA_ERR_NO_DEVICE /
- * FMCA_ERR_OUT_OF_MEMORY if a device step fails. */
+ * admissibility lies within rounding of eta, the expansion runs on the host;
+ * where the device memory (or the cap) has no room, that part also runs on
+ * the host (fmca_h2_plan_report says which).  FMCA_ERR_DEVICE /
+ * FMCA_ERR_NO_DEVICE if a device step fails otherwise. */
 fmca_status fmca_h2_create_ex(const fmca_kernel_desc *kernel, int dim, int64_t m,
                               const double *Peval, int64_t n, const double *P, int deg,
                               double eta, int interpolator, int planner,

@@ -37,3 +37,31 @@ def test_memory_limit_argument_check():
     with pytest.raises(ValueError):
         fmca_amd.set_device_memory_limit(-1)
     fmca_amd.set_device_memory_limit(0)
+
+
+@pytest.mark.gpu
+def test_device_planner_falls_back_to_host_under_the_memory_cap():
+    """The device planner's transient buffers count against the cap; with no
+    room left, both trees and the block expansion are planned on the host
+    (reported as host-fallback), the plan is the host's, and the accounting
+    returns to where it was."""
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print("[case] device planner under a memory cap", flush=True)
+    P = W.uniform(50_000, 3, 4)
+    ker = fmca_amd.CovarianceKernel("MATERN32", 0.5)
+    Hh = fmca_amd.H2kernelActor(ker, P, P, 3, 0.8, planner="host")
+    live, _ = fmca_amd.device_memory()
+    fmca_amd.set_device_memory_limit(live + 1024)  # below any buffer of the build
+    try:
+        Hd = fmca_amd.H2kernelActor(ker, P, P, 3, 0.8, planner="device")
+    finally:
+        fmca_amd.set_device_memory_limit(0)
+    rep = Hd.plan_report()
+    assert rep["rows_tree"] == "host-fallback" and rep["blocks"] == "host-fallback"
+    assert np.array_equal(Hd.indices(0), Hh.indices(0))
+    for a, b in zip(Hd.blocks(), Hh.blocks()):
+        assert np.array_equal(a, b)
+    assert fmca_amd.device_memory()[0] == live
+    x = np.asfortranarray(np.random.default_rng(1).standard_normal((50_000, 4)))
+    assert rel_err(Hd.action(x), Hh.action(x)) <= 1e-14
