@@ -637,6 +637,43 @@ int admissibility(const ClusterGeo &r, const ClusterGeo &c, int d, double eta) {
 }
 }  // namespace
 
+namespace {
+// CSR of the (key, value) pairs that items b in [0, nb) emit: emit(b, put)
+// calls put(key, value) for each pair of item b.  The values of one key keep
+// the order of their items, as a serial fill would: the items are cut into
+// one contiguous chunk per worker, counted per chunk, and each chunk then
+// fills from its own offsets.
+template <typename Emit>
+void build_csr(int64_t nb, int nkeys, Emit emit, std::vector<int32_t> &ptr,
+               std::vector<int32_t> &out) {
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), nb / 65536 + 1));
+  std::vector<std::vector<int32_t>> cnt(nt, std::vector<int32_t>(nkeys, 0));
+  parallel_chunks(nb, nt, [&](int t, int64_t lo, int64_t hi) {
+    int32_t *c = cnt[t].data();
+    for (int64_t b = lo; b < hi; ++b) emit(b, [&](int k, int32_t) { ++c[k]; });
+  });
+  ptr.assign(nkeys + 1, 0);
+  for (int k = 0; k < nkeys; ++k) {
+    int32_t sum = 0;
+    for (int t = 0; t < nt; ++t) sum += cnt[t][k];
+    ptr[k + 1] = ptr[k] + sum;
+  }
+  parallel_for(nkeys, 1024, [&](int64_t k) {  // counts -> each chunk's first position
+    int32_t o = ptr[k];
+    for (int t = 0; t < nt; ++t) {
+      const int32_t c = cnt[t][k];
+      cnt[t][k] = o;
+      o += c;
+    }
+  });
+  out.assign(ptr[nkeys], 0);
+  parallel_chunks(nb, nt, [&](int t, int64_t lo, int64_t hi) {
+    int32_t *f = cnt[t].data();
+    for (int64_t b = lo; b < hi; ++b) emit(b, [&](int k, int32_t v) { out[f[k]++] = v; });
+  });
+}
+}  // namespace
+
 double cluster_radius(const Tree &T, int c) { return radius(T, c); }
 
 void compute_pattern(Pattern &pat, const Tree &R, const Tree &C, double eta, int q) {
@@ -701,19 +738,13 @@ void finish_pattern(Pattern &pat, const Tree &R, const Tree &C, int q) {
   pat.n_far = pat.n_near = 0;
   pat.near_entries = pat.far_entries = 0;
   // far field CSR by target cluster
-  pat.far_ptr.assign(R.ncl + 1, 0);
-  for (int64_t b = 0; b < nb; ++b)
-    if (pat.block_lr[b]) {
-      ++pat.far_ptr[pat.block_row[b] + 1];
-      ++pat.n_far;
-    }
-  for (int c = 0; c < R.ncl; ++c) pat.far_ptr[c + 1] += pat.far_ptr[c];
-  pat.far_src.assign(pat.far_ptr[R.ncl], 0);
-  {
-    std::vector<int32_t> fill(pat.far_ptr.begin(), pat.far_ptr.end() - 1);
-    for (int64_t b = 0; b < nb; ++b)
-      if (pat.block_lr[b]) pat.far_src[fill[pat.block_row[b]]++] = pat.block_col[b];
-  }
+  build_csr(
+      nb, R.ncl,
+      [&](int64_t b, auto put) {
+        if (pat.block_lr[b]) put(pat.block_row[b], pat.block_col[b]);
+      },
+      pat.far_ptr, pat.far_src);
+  pat.n_far = pat.far_ptr[R.ncl];
   parallel_for(R.ncl, 64, [&](int64_t c) {
     std::sort(pat.far_src.begin() + pat.far_ptr[c], pat.far_src.begin() + pat.far_ptr[c + 1]);
   });
@@ -739,27 +770,36 @@ void finish_pattern(Pattern &pat, const Tree &R, const Tree &C, int q) {
     lo = leaf_lo[t];
     hi = leaf_hi[t];
   };
-  pat.near_ptr.assign(nl + 1, 0);
-  for (int64_t b = 0; b < nb; ++b)
-    if (!pat.block_lr[b]) {
-      ++pat.n_near;
-      pat.near_entries += double(R.size[pat.block_row[b]]) * C.size[pat.block_col[b]];
-      int lo, hi;
-      leaves_under(pat.block_row[b], lo, hi);
-      for (int k = lo; k < hi; ++k) ++pat.near_ptr[leaf_index[by_begin[k]] + 1];
-    }
-  for (int k = 0; k < nl; ++k) pat.near_ptr[k + 1] += pat.near_ptr[k];
-  pat.near_src.assign(pat.near_ptr[nl], 0);
   {
-    std::vector<int32_t> fill(pat.near_ptr.begin(), pat.near_ptr.end() - 1);
-    for (int64_t b = 0; b < nb; ++b)
-      if (!pat.block_lr[b]) {
+    // dense blocks and their entries (integer sums: exact in any order)
+    std::vector<int64_t> cnt(num_workers(), 0), ent(num_workers(), 0);
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), nb / 65536 + 1));
+    parallel_chunks(nb, nt, [&](int t, int64_t lo, int64_t hi) {
+      int64_t c = 0, en = 0;  // local: the per-thread slots share cache lines
+      for (int64_t b = lo; b < hi; ++b)
+        if (!pat.block_lr[b]) {
+          ++c;
+          en += R.size[pat.block_row[b]] * C.size[pat.block_col[b]];
+        }
+      cnt[t] = c;
+      ent[t] = en;
+    });
+    int64_t e = 0;
+    for (int t = 0; t < nt; ++t) {
+      pat.n_near += cnt[t];
+      e += ent[t];
+    }
+    pat.near_entries = double(e);
+  }
+  build_csr(
+      nb, nl,
+      [&](int64_t b, auto put) {
+        if (pat.block_lr[b]) return;
         int lo, hi;
         leaves_under(pat.block_row[b], lo, hi);
-        for (int k = lo; k < hi; ++k)
-          pat.near_src[fill[leaf_index[by_begin[k]]]++] = pat.block_col[b];
-      }
-  }
+        for (int k = lo; k < hi; ++k) put(leaf_index[by_begin[k]], pat.block_col[b]);
+      },
+      pat.near_ptr, pat.near_src);
   parallel_for(nl, 16, [&](int64_t k) {
     std::sort(pat.near_src.begin() + pat.near_ptr[k], pat.near_src.begin() + pat.near_ptr[k + 1],
               [&](int a, int b) { return C.begin[a] < C.begin[b]; });
