@@ -67,6 +67,10 @@ class CovarianceKernel {
 
 enum class Precision : int { FP64 = FMCA_PRECISION_FP64, FP32 = FMCA_PRECISION_FP32 };
 
+// Where the constructor builds the cluster trees and the block tree
+// (fmca_h2_create_ex): the host planner, or the current HIP device.
+enum class Planner : int { Host = FMCA_PLAN_HOST, Device = FMCA_PLAN_DEVICE };
+
 // Moments tags for callers without FMCA's headers.
 struct TensorProductMoments {};
 struct TotalDegreeMoments {
@@ -89,18 +93,21 @@ template <typename Covariance = CovarianceKernel, typename Moments = TensorProdu
           typename... FmcaPolicies>
 class H2kernelActor {
  public:
+  // FMCA's constructor (H2kernelActor.h:21-36); the planner argument is an
+  // addition with a default, so FMCA's calls compile unchanged.
   template <typename PointMatrix>
   H2kernelActor(const Covariance &kernel, const PointMatrix &Peval, const PointMatrix &P,
-                unsigned mpole_deg, double eta)
+                unsigned mpole_deg, double eta, Planner planner = Planner::Host)
       : m_((int64_t)Peval.cols()), n_((int64_t)P.cols()) {
     const std::string kt = kernel.kernelType();
     fmca_kernel_desc desc{kt.c_str(), (double)kernel.l(), (double)kernel.c(),
                           (double)kernel.nu()};
     if (Peval.rows() != P.rows()) throw Error(FMCA_ERR_INVALID_ARGUMENT, "point dimensions");
     fmca_h2_actor *h = nullptr;
-    check(fmca_h2_create_interp(&desc, (int)P.rows(), m_, Peval.data(), n_, P.data(),
-                                (int)mpole_deg, eta, moments_interpolator<Moments>::value, &h),
-          "fmca_h2_create_interp");
+    check(fmca_h2_create_ex(&desc, (int)P.rows(), m_, Peval.data(), n_, P.data(),
+                            (int)mpole_deg, eta, moments_interpolator<Moments>::value,
+                            (int)planner, &h),
+          "fmca_h2_create_ex");
     h_.reset(h, [](fmca_h2_actor *p) { fmca_h2_destroy(p); });
   }
 
@@ -138,6 +145,18 @@ class H2kernelActor {
     std::vector<double> out(6);
     check(fmca_h2_statistics(h_.get(), out.data()), "fmca_h2_statistics");
     return out;
+  }
+
+  // H2Matrix::computeH2Matrix (H2MatrixBase.h:222-255): store every block in
+  // device memory; later actions of this precision stream them (operator*).
+  // Returns the bytes held.
+  int64_t assemble(Precision precision = Precision::FP64) {
+    int64_t bytes = 0;
+    check(fmca_h2_assemble(h_.get(), (int)precision, &bytes), "fmca_h2_assemble");
+    return bytes;
+  }
+  void release_assembled() {
+    check(fmca_h2_release_assembled(h_.get()), "fmca_h2_release_assembled");
   }
 
   int64_t rows() const { return m_; }

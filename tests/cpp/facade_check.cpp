@@ -2,7 +2,7 @@
 // FMCA::H2kernelActor: construct from (kernel, Peval, P, deg, eta), call
 // action(rhs) on a column-major matrix.  Reads points and rhs from a binary
 // file, writes the result; "plan" mode only builds the actor.
-//   facade_check plan|action|plan_td|action_td <in.bin> <out.bin>
+//   facade_check plan|action|plan_td|action_td|action_dev|action_td_dev <in.bin> <out.bin>
 // (the _td modes instantiate the actor with a Moments type shaped like
 // FMCA's NystromMoments<TotalDegreeInterpolator>)
 // in.bin: int64 dim, n, nrhs, deg; double eta, l; char[16] kernel;
@@ -49,6 +49,7 @@ int run(Actor &act, const char *mode, const Mat &rhs, const char *path) {
     out.write((const char *)st.data(), sizeof(double) * st.size());
     return 0;
   }
+  if (std::strstr(mode, "_dev") && act.assemble(fmca_amd::Precision::FP64) <= 0) return 4;
   Mat y = act.action(rhs);
   out.write((const char *)y.data(), sizeof(double) * y.rows() * y.cols());
   return 0;
@@ -81,16 +82,19 @@ int main(int argc, char **argv) {
   try {
     const MyKernel kernel{std::string(kname), par[1]};
     const bool td = std::strstr(argv[1], "_td") != nullptr;
+    // "_dev": plan on the device and stream assembled blocks
+    const auto planner = std::strstr(argv[1], "_dev") ? fmca_amd::Planner::Device
+                                                       : fmca_amd::Planner::Host;
     // FMCA's four template arguments: kernel, Moments, tree, evaluator
     if (td) {
       fmca_amd::H2kernelActor<MyKernel, NystromMomentsLike<TDInterp>, struct H2ClusterTree,
                               struct Evaluator>
-          act(kernel, P, P, (unsigned)hdr[3], par[0]);
+          act(kernel, P, P, (unsigned)hdr[3], par[0], planner);
       return run(act, argv[1], rhs, argv[3]);
     }
     fmca_amd::H2kernelActor<MyKernel, NystromMomentsLike<TPInterp>, struct H2ClusterTree,
                             struct Evaluator>
-        act(kernel, P, P, (unsigned)hdr[3], par[0]);
+        act(kernel, P, P, (unsigned)hdr[3], par[0], planner);
     return run(act, argv[1], rhs, argv[3]);
   } catch (const fmca_amd::Error &e) {
     std::cerr << e.what() << std::endl;

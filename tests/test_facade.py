@@ -93,3 +93,19 @@ def test_facade_total_degree_action_matches_oracle(tmp_path):
     y = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(3, 5000).T
     ref = oracle.OracleH2(P, P, "EXPONENTIAL", 1.0, 4, 0.8, scheme="total_degree").action(rhs)
     assert rel_err(y, ref) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,scheme", [("action_dev", "tensor"), ("action_td_dev", "total_degree")])
+def test_facade_device_planner_and_assembled_blocks(tmp_path, mode, scheme):
+    """Planner::Device at construction and assemble() before the action: the
+    same result as FMCA's restated action."""
+    P = uniform_points(6000, 3, seed=26)
+    rhs = np.asfortranarray(np.random.default_rng(27).standard_normal((6000, 4)))
+    _write(tmp_path / "in.bin", P, rhs, 3, 0.8, "matern32", 0.5)
+    r = subprocess.run([_binary(), mode, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    y = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(4, 6000).T
+    ref = oracle.OracleH2(P, P, "MATERN32", 0.5, 3, 0.8, scheme=scheme).action(rhs)
+    assert rel_err(y, ref) <= 1e-12
