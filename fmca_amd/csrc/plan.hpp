@@ -1,9 +1,11 @@
-// Host-side planner for the MI355X H2-matrix kernel action.
+// Planner for the MI355X H2-matrix kernel action.
 //
-// Builds FMCA's cluster tree, block pattern and tensor-product Chebyshev data,
-// flattened into structure-of-arrays that are uploaded to HBM once.  Cluster
-// ids are FMCA's level-order block ids.  Nothing here touches the GPU, so the
-// planner can be exercised on a host without one.
+// Builds FMCA's cluster tree, block pattern and interpolation data, flattened
+// into structure-of-arrays that are uploaded to HBM once.  Cluster ids are
+// FMCA's level-order block ids.  plan.cpp runs on the host alone, so the
+// planner can be exercised without a GPU; build_tree_device and
+// expand_blocks_device (tree_device.hip, pattern_device.hip) move its O(n)
+// and O(#blocks) parts onto the device with the same result.
 #pragma once
 
 #include <cstddef>
