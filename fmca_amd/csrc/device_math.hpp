@@ -65,6 +65,7 @@ struct Math<double> {
     g = __builtin_fma(d, h, g);
     return x > 0.0 ? g : 0.0;
   }
+  static __device__ __forceinline__ double div_(double a, double b) { return a / b; }
   static __device__ __forceinline__ double log_(double x) { return log(x); }
   static __device__ __forceinline__ double eps() { return 2.220446049250313e-16; }
 };
@@ -76,6 +77,11 @@ struct Math<float> {
   }
   static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float log_(float x) { return logf(x); }
+  // a / b through the hardware reciprocal (about 1 ulp; the FP32 form's
+  // entries carry more error than that already)
+  static __device__ __forceinline__ float div_(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
   static __device__ __forceinline__ float eps() { return 1.1920928955078125e-07f; }
 };
 
@@ -154,7 +160,7 @@ __device__ __forceinline__ T kernel_r2(T r2, const KernelParams<T> &kp) {
     case KT_EXPONENTIAL: {
       const T r = M::sqrt_(r2), e = M::exp_(-r * il);
       if (var == KV_DL) return r * il2 * e;
-      phi = r > T(0) ? -(il / r) * e : T(0);
+      phi = r > T(0) ? -M::div_(il, r) * e : T(0);
       break;
     }
     case KT_MATERN32: {
@@ -227,7 +233,7 @@ __device__ __forceinline__ void kernel_r2_all(T r2, const KernelParams<T> &kp, T
     ev[KV_VALUE] = e;
     ev[KV_DL] = r * (kp.s * kp.s) * e;
     ev[KV_PHI] = T(0);
-    if (kp.vmask & (1 << KV_PHI)) ev[KV_PHI] = r > T(0) ? -(kp.s / r) * e : T(0);
+    if (kp.vmask & (1 << KV_PHI)) ev[KV_PHI] = r > T(0) ? -M::div_(kp.s, r) * e : T(0);
   } else if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN) {
     // the shared exponential; dk/dl = -(r^2 / l) phi for these f(r / l)
     T phi;
