@@ -3,19 +3,24 @@
 Workload (the configuration the headline metric is quoted on):
   n = 1,000,000 points uniform in [0,1]^3 (numpy default_rng(0)), Matern-3/2
   kernel with lengthscale 0.5, tensor Chebyshev degree 4, eta = 0.8,
-  32 right-hand sides with i.i.d. N(0,1) entries.
+  32 right-hand sides x with i.i.d. N(0,1) entries.
 
-One step = one matrix-free H2 matvec y = K x of the precision form (FP32 kernel
-entries and contractions on the matrix cores, FP64 in/out) on the 32 resident
-right-hand sides.  Beside it, on the same inputs: the plain FP64 action (the
-ratio and the relative difference of the two results), the single-rhs
-actions, the host-buffer boundary, and the CG solve of (K + 100 I) a = x to a
-1e-8 relative FP64 residual with FP32 matvecs + FP64 reliable updates and with
-plain FP64 CG (iterations, time, solution difference).
+One step = one CG solve of (K + 100 I) a = x for the 32 resident columns to a
+1e-8 relative FP64 residual: FP32 matvecs (FP32 kernel entries and
+contractions on the matrix cores) drive the recurrence, the residual is
+replaced by the FP64 action's each time it has dropped by 10x, and vectors,
+inner products and updates are FP64 (the precision form with FP64
+refinement).  value = n * nrhs * (iterations plain FP64 CG needs for the same
+tolerance) / seconds per refined solve: the FP64-equivalent matvec throughput
+of the refined solver.  Plain FP64 CG is timed beside it on the same system
+(its own value, the ratio, the solution difference), and so are single
+matvecs of both precisions (1 and 32 rhs, per phase), the host-buffer
+boundary, the differentiation side results (n = 2e5 2-D and n = 8e6 3-D:
+forward, backward, ratio, memory) and the FP64 CPU oracle on the host cores.
 
 Prints ONE JSON line.  Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                  [--no-solve] [--calibrate-copy-mib M]
+                  [--no-side] [--workload NAME] [--calibrate-copy-mib M]
 """
 from __future__ import annotations
 
@@ -39,12 +44,13 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=1_000_000)
     p.add_argument("--nrhs", type=int, default=32)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--no-solve", action="store_true")
+    p.add_argument("--no-side", action="store_true",
+                   help="skip the differentiation side results (n = 2e5 2-D and n = 8e6 3-D)")
     p.add_argument("--ridge", type=float, default=100.0)
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
@@ -100,33 +106,27 @@ def traffic_from_profiles(kernel_name):
         return None
 
 
-def solve(H, x, ridge):
-    """CG on (K + ridge I) a = x to a 1e-8 relative FP64 residual, FP32 matvecs
-    with FP64 reliable updates and plain FP64; both from a = 0."""
-    import torch
-    import fmca_amd
-    out = {"ridge": ridge, "tol": 1e-8, "nrhs": x.shape[1]}
-    sol = {}
-    for name, prec in (("fp32_refined", fmca_amd.FP32), ("fp64", fmca_amd.FP64)):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8, precision=prec,
-                       max_iter=2000)
-        torch.cuda.synchronize()
-        info["wall_s"] = time.perf_counter() - t0
-        out[name] = info
-        sol[name] = a
-    out["solution_rel_diff"] = float(torch.linalg.norm(sol["fp32_refined"] - sol["fp64"])
-                                     / torch.linalg.norm(sol["fp64"]))
-    out["speedup_fp32_refined_vs_fp64"] = out["fp64"]["wall_s"] / out["fp32_refined"]["wall_s"]
-    return out
+def host_threads():
+    """Threads for the CPU baseline: every core this process may run on
+    (its affinity mask), unless OMP_NUM_THREADS caps it, as it does on hosts
+    that give each GPU a fixed share of the cores.  Returns (threads, note)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() else affinity
+    note = (f"{threads} threads: nproc reports {os.cpu_count()}, the process may run on "
+            f"{affinity} cores, OMP_NUM_THREADS={omp or 'unset'} (the host's CPU share for "
+            f"this GPU)")
+    return threads, note
 
 
 def cpu_baseline(P, x, info, budget_seconds):
     """FMCA's OpenMP CPU H2 action as restated by the FP64 oracle, timed on a
     bounded sample of the column groups and scaled by kernel entries."""
     import oracle
-    nthreads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    nthreads, cores_note = host_threads()
     A = oracle.OracleH2(P, P, "MATERN32", 0.5, 4, 0.8)
     total = info["near_entries"] + info["far_entries"]
     # calibration pass, then a pass sized to the time budget
@@ -140,6 +140,7 @@ def cpu_baseline(P, x, info, budget_seconds):
         "value": P.shape[1] * nrhs / t_full / 1e6,
         "unit": "Mpoint*RHS/s",
         "cores": nthreads,
+        "cores_note": cores_note,
         "kind": "port",
         "sample": (f"oracle (FP64 C++/OpenMP restatement of FMCA's action, same tree/blocks) "
                    f"on the same n={P.shape[1]} points and {nrhs} rhs: forward pass "
@@ -195,7 +196,7 @@ def side_small_dense(args):
                      "rel_err_vs_dense": float(torch.linalg.norm(y - yd) / torch.linalg.norm(yd)),
                      "graph_equals_eager": bool(torch.equal(yg, y))}
     A = oracle.OracleH2(P, P, "EXPONENTIAL", 0.5, 3, 0.8)
-    nthreads = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    nthreads, _ = host_threads()
     t0 = time.perf_counter()
     yo = A.action(xh, nthreads)
     res["cpu_oracle"] = {"ms": (time.perf_counter() - t0) * 1e3, "threads": nthreads,
@@ -447,6 +448,53 @@ def side_large(args):
     return res
 
 
+def refined_solves(H, x, ridge, steps, warmup):
+    """The headline step: one CG solve of (K + ridge I) a = x for all columns
+    to a 1e-8 relative FP64 residual, FP32 matvecs driving the recurrence and
+    FP64 reliable updates (fmca_h2_cg_device), from a = 0.  Profiling events
+    (pooled, created before the timed region) stay on through the timed
+    solves, so that the FP32 leaf-output launch time of the roofline is
+    measured on the launches of the timed region itself."""
+    import torch
+    import fmca_amd
+    infos = []
+
+    def one():
+        a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8,
+                       precision=fmca_amd.FP32, max_iter=2000)
+        return a, info
+    H.set_profiling(True)
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    H.phase_times(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a, info = one()
+        infos.append(info)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    H.set_profiling(False)
+    ph32 = H.phase_times(reset=True, precision=fmca_amd.FP32)
+    H.phase_times(reset=True)
+    return t, a, infos, ph32
+
+
+def fp64_solve(H, x, ridge):
+    """Plain FP64 CG on the same system, from a = 0, timed once."""
+    import torch
+    import fmca_amd
+    H.cg(x[:, :1].contiguous(), x0=torch.zeros_like(x[:, :1].contiguous()), ridge=ridge,
+         tol=1e-8, precision=fmca_amd.FP64, max_iter=3)  # sizes nothing new, warms the path
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8, precision=fmca_amd.FP64,
+                   max_iter=2000)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, a, info
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -488,21 +536,32 @@ def main():
     out32 = torch.empty((nrhs, n), dtype=torch.float64, device="cuda").t()
     out64 = torch.empty((nrhs, n), dtype=torch.float64, device="cuda").t()
 
-    t32, ph32 = timed_steps(H, x, out32, fmca_amd.FP32, args.steps, args.warmup, True)
-    t64, ph64 = timed_steps(H, x, out64, fmca_amd.FP64, max(1, args.steps // 2),
-                            min(1, args.warmup), True)
+    # single matvecs of both precisions (per-phase times, FP32 vs FP64 difference)
+    mv_steps = 5
+    t32, ph32m = timed_steps(H, x, out32, fmca_amd.FP32, mv_steps, 1, True)
+    t64, ph64m = timed_steps(H, x, out64, fmca_amd.FP64, 3, 1, True)
     err = float(torch.linalg.norm(out32 - out64) / torch.linalg.norm(out64))
-    # single right-hand side
+    del out32, out64
     x1 = x[:, :1].contiguous()
     o1 = torch.empty((n, 1), dtype=torch.float64, device="cuda")
-    t32_1, _ = timed_steps(H, x1, o1, fmca_amd.FP32, args.steps, 1, False)
-    t64_1, _ = timed_steps(H, x1, o1, fmca_amd.FP64, max(1, args.steps // 2), 1, False)
+    t32_1, _ = timed_steps(H, x1, o1, fmca_amd.FP32, mv_steps, 1, False)
+    t64_1, _ = timed_steps(H, x1, o1, fmca_amd.FP64, 3, 1, False)
+    del o1, x1
     # the C-ABI host-buffer boundary (fmca_h2_action): H2D + action + D2H
     H.action(xh, precision=fmca_amd.FP32)
     t0 = time.perf_counter()
     H.action(xh, precision=fmca_amd.FP32)
     t_host = time.perf_counter() - t0
-    solve_res = None if args.no_solve else solve(H, x, args.ridge)
+
+    # the headline: refined solves (timed steps), then plain FP64 CG once
+    t_ref, a_ref, infos, ph_solve = refined_solves(H, x, args.ridge, args.steps, args.warmup)
+    t_64, a_64, info64 = fp64_solve(H, x, args.ridge)
+    sol_diff = float(torch.linalg.norm(a_ref - a_64) / torch.linalg.norm(a_64))
+    del a_ref, a_64
+    iters64 = info64["iterations"]
+    value = n * nrhs * iters64 / t_ref / 1e6
+    value_fp64 = n * nrhs * iters64 / t_64 / 1e6
+
     if args.calibrate_copy_mib > 0:
         nel = args.calibrate_copy_mib * (1 << 20) // 8
         src = torch.ones(nel, dtype=torch.float64, device="cuda")
@@ -512,17 +571,15 @@ def main():
         del src, dst
 
     E_near, E_far, q = info["near_entries"], info["far_entries"], int(info["q"])
-    value = n * nrhs / t32 / 1e6
 
-    # Dominant kernel of the timed step: the larger of the leaf output (near
-    # field + V^T l_t) and the coupling, each one launch per action, timed with
-    # HIP events on the launch stream.  Algorithmic flops per launch:
-    #   leaf output  2 E_near nrhs + 2 q n nrhs   (near entries, far basis)
-    #   coupling     2 E_far nrhs                 (q x q block products)
+    # Dominant kernel of the timed step: the FP32 leaf output (near field +
+    # V^T l_t), one launch per FP32 matvec of the solves, timed with HIP
+    # events on the launch stream over the timed solves.  Algorithmic flops
+    # per launch: 2 E_near nrhs (near entries) + 2 q n nrhs (far basis).
     def avg(ph, k):
         return ph[k][0] / max(ph[k][1], 1)
-    dom = max(("leaf_output", "coupling"), key=lambda k: avg(ph32, k))
-    t_dom = avg(ph32, dom) / 1e3
+    dom = max(("leaf_output", "coupling"), key=lambda k: avg(ph_solve, k))
+    t_dom = avg(ph_solve, dom) / 1e3
     nt = mfma_tiles(nrhs)
     if dom == "leaf_output":
         flops = 2.0 * E_near * nrhs + 2.0 * q * n * nrhs
@@ -541,7 +598,11 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
         "traffic": traffic_from_profiles(kname),
+        "traffic_source": ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                           "runs of this workload's FP32 matvec, scaled by the counters' reading "
+                           "for a copy kernel of known size; not measured in this run"),
         "avg_launch_ms": t_dom * 1e3,
+        "launches_timed": int(ph_solve[dom][1]),
         "algorithmic_flops_per_launch": flops,
         "kernel_entries_per_launch": entries,
         "kernel_entries_per_s": entries / t_dom,
@@ -553,39 +614,55 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t32 * 1e3,
+        "ms_per_step": t_ref * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+f64",
         "data": "synthetic: uniform points in [0,1]^3 and N(0,1) rhs, numpy default_rng(0)",
         "config": {
-            "workload": (f"H2 matvec, n={n} uniform 3D points, MATERN32 l=0.5, tensor "
-                         f"Chebyshev deg 4 (q={q}), eta=0.8, {nrhs} rhs, FP32 precision form"),
+            "workload": (f"CG solve of (K + {args.ridge:g} I) a = x to a 1e-8 relative FP64 "
+                         f"residual, n={n} uniform 3D points, MATERN32 l=0.5, tensor Chebyshev "
+                         f"deg 4 (q={q}), eta=0.8, {nrhs} rhs N(0,1); FP32 matvecs + FP64 "
+                         f"reliable updates; value = n * nrhs * (iterations of plain FP64 CG "
+                         f"to the same tolerance) / seconds per refined solve"),
             "n": n, "nrhs": nrhs, "kernel": "MATERN32", "lengthscale": 0.5, "degree": 4,
-            "eta": 0.8, "far_blocks": int(info["far_blocks"]),
+            "eta": 0.8, "ridge": args.ridge, "tol": 1e-8,
+            "far_blocks": int(info["far_blocks"]),
             "near_blocks": int(info["near_blocks"]), "near_entries": E_near,
             "far_entries": E_far,
         },
         "roofline": roofline,
-        "fp64": {
-            "value": n * nrhs / t64 / 1e6, "ms_per_step": t64 * 1e3,
-            "fp32_speedup": t64 / t32,
-            "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph64.items()},
+        "solve": {
+            "fp32_refined": {**infos[-1], "seconds_mean": t_ref,
+                             "iterations_all_steps": [i["iterations"] for i in infos],
+                             "rel_residual_max_all_steps": max(i["rel_residual"] for i in infos)},
+            "fp64": {**info64, "wall_s": t_64, "value": value_fp64},
+            "solution_rel_diff": sol_diff,
+            "speedup_fp32_refined_vs_fp64": t_64 / t_ref,
         },
-        "fp32_vs_fp64_rel_err": err,
-        "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph32.items()},
-        "rhs1": {"fp32_mpts": n / t32_1 / 1e6, "fp64_mpts": n / t64_1 / 1e6,
-                 "fp32_ms": t32_1 * 1e3, "fp64_ms": t64_1 * 1e3},
-        "host_boundary": {"fp32_ms": t_host * 1e3, "mpts": n * nrhs / t_host / 1e6,
-                          "note": "fmca_h2_action on host arrays: H2D copy + action + D2H"},
+        "matvec": {
+            "fp32": {"ms": t32 * 1e3, "mpts": n * nrhs / t32 / 1e6,
+                     "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph32m.items()}},
+            "fp64": {"ms": t64 * 1e3, "mpts": n * nrhs / t64 / 1e6,
+                     "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph64m.items()}},
+            "fp32_speedup": t64 / t32,
+            "fp32_vs_fp64_rel_err": err,
+            "rhs1": {"fp32_mpts": n / t32_1 / 1e6, "fp64_mpts": n / t64_1 / 1e6,
+                     "fp32_ms": t32_1 * 1e3, "fp64_ms": t64_1 * 1e3},
+            "host_boundary": {"fp32_ms": t_host * 1e3, "mpts": n * nrhs / t_host / 1e6,
+                              "note": "fmca_h2_action on host arrays: H2D copy + action + D2H"},
+        },
         "setup_s": {"plan": setup_plan, "plan_host": setup_plan_host, "upload": setup_upload,
                     "plan_report": plan_report},
         "device_memory": {"library_peak_bytes": fmca_amd.device_memory()[1],
                           "torch_peak_bytes": int(torch.cuda.max_memory_allocated())},
     }
-    if solve_res is not None:
-        res["solve"] = solve_res
+    del H, x
+    torch.cuda.empty_cache()
+    if not args.no_side:
+        res["autograd_2d"] = side_autograd(args)
+        res["large_8e6"] = side_large(args)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(P, xh, info, args.cpu_seconds)
     print(json.dumps(res), flush=True)

@@ -261,9 +261,10 @@ class H2kernelActor:
         """Record HIP events around the six phases of every action."""
         self._a.set_profiling(bool(enable))
 
-    def phase_times(self, reset: bool = False) -> dict:
-        """{phase: (total ms, number of actions)} of the profiled actions."""
-        return self._a.phase_times(reset)
+    def phase_times(self, reset: bool = False, precision: int | None = None) -> dict:
+        """{phase: (total ms, number of actions)} of the profiled actions, of
+        one precision (FP64 / FP32) or of both (None)."""
+        return self._a.phase_times(reset, -1 if precision is None else int(precision))
 
     def clusters(self, which: int) -> dict:
         d = self._a.clusters(which)

@@ -49,12 +49,22 @@ class H2Spec:
         self.nu = float(nu)
         self._key = None
         self._actor = None
+        self._held = None  # (points, targets) the cached plan was built from
 
     def actor(self, points: torch.Tensor, targets, l: float) -> H2kernelActor:
-        key = (points.data_ptr(), points._version, tuple(points.shape),
-               None if targets is None else (targets.data_ptr(), targets._version,
+        """The actor planned on `points` / `targets`, re-planned when they
+        change.  The spec holds the tensors the plan was built from and a copy
+        of their values: a fresh tensor that reuses the memory of a freed one
+        (same address, version 0) is a different object, and values changed
+        behind the version counter (through .data) differ from the copy."""
+        key = (id(points), points._version, tuple(points.shape),
+               None if targets is None else (id(targets), targets._version,
                                              tuple(targets.shape)))
-        if self._actor is None or key != self._key:
+        same = (self._actor is not None and key == self._key
+                and self._held[0] is points and self._held[1] is targets
+                and torch.equal(self._copy[0], points.detach())
+                and (targets is None or torch.equal(self._copy[1], targets.detach())))
+        if not same:
             P = np.asfortranarray(points.detach().cpu().numpy().astype(np.float64))
             T = P if targets is None else np.asfortranarray(
                 targets.detach().cpu().numpy().astype(np.float64))
@@ -62,6 +72,9 @@ class H2Spec:
                                         self.deg, self.eta, self.interpolator,
                                         self.planner)
             self._key = key
+            self._held = (points, targets)
+            self._copy = (points.detach().clone(),
+                          None if targets is None else targets.detach().clone())
         self._actor.set_kernel(CovarianceKernel(self.ktype, l, self.c, self.nu))
         return self._actor
 
@@ -133,6 +146,7 @@ class _H2Matvec(torch.autograd.Function):
         a2 = alpha if alpha.dim() == 2 else alpha.reshape(-1, 1)
         y = _variant(actor, KERNEL_VALUE, a2, False, spec.precision)
         ctx.actor = actor
+        ctx.l = l
         ctx.spec = spec
         ctx.vector = alpha.dim() == 1
         ctx.symmetric = targets is None
@@ -144,6 +158,10 @@ class _H2Matvec(torch.autograd.Function):
     def backward(ctx, grad):
         alpha, lengthscale, points, targets = ctx.saved_tensors
         actor, prec = ctx.actor, ctx.spec.precision
+        # the actor is shared with later forwards of the same spec, each of
+        # which sets its own lengthscale: restore this forward's
+        spec = ctx.spec
+        actor.set_kernel(CovarianceKernel(spec.ktype, ctx.l, spec.c, spec.nu))
         G = (grad.reshape(-1, 1) if ctx.vector else grad).to(torch.float64)
         A = (alpha.reshape(-1, 1) if ctx.vector else alpha).to(torch.float64)
         need_a, need_l = ctx.needs_input_grad[0], ctx.needs_input_grad[1]

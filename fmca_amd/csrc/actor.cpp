@@ -206,18 +206,30 @@ struct fmca_h2_actor {
   }
   hipStream_t stream = nullptr;
   std::mutex mtx;
+  // Cross-stream order of the actor's workspaces: every action records
+  // order_ev on its stream, and an action on another stream first waits for
+  // it.  Work captured into a graph is left out (its order is the graph's).
+  hipEvent_t order_ev = nullptr;
+  hipStream_t order_stream = nullptr;
+  bool order_valid = false;
   // per-phase profiling (events recorded on the launch stream)
   bool profiling = false;
-  std::vector<PhaseEvents> pending;
-  double phase_ms[PH_COUNT] = {0};
-  int64_t phase_count[PH_COUNT] = {0};
+  // events of the profiled actions not yet read, with each action's
+  // precision; read events go back to ev_pool, so that a profiled action
+  // records events but creates none once the pool is warm
+  std::vector<std::pair<int, PhaseEvents>> pending;
+  std::vector<hipEvent_t> ev_pool;
+  double phase_ms[2][PH_COUNT] = {{0}};
+  int64_t phase_count[2][PH_COUNT] = {{0}};
 
   const Tree &rows() const { return rows_tree; }
   const Tree &cols() const { return shared ? rows_tree : cols_tree; }
 
   ~fmca_h2_actor() {
     for (auto &pe : pending)
-      for (auto &e : pe.ev) (void)hipEventDestroy(e);
+      for (auto &e : pe.second.ev) (void)hipEventDestroy(e);
+    for (auto &e : ev_pool) (void)hipEventDestroy(e);
+    if (order_ev) (void)hipEventDestroy(order_ev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -323,6 +335,7 @@ fmca_status ensure_device(fmca_h2_actor *a) {
     return fail(FMCA_ERR_NO_DEVICE, "no HIP device available");
   HIP_TRY(hipGetDevice(&a->device));
   HIP_TRY(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&a->order_ev, hipEventDisableTiming));
   HIP_TRY(upload_ints(a->d_rows_i, a->rows()));
   {
     const std::vector<double> unit = leaf_unit_coords(a->rows(), a->pts_rows);
@@ -456,9 +469,16 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
   PhaseEvents *pe = nullptr;
   if (a->profiling) {
     PhaseEvents ev;
-    for (auto &x : ev.ev) HIP_TRY(hipEventCreate(&x));
-    a->pending.push_back(ev);
-    pe = &a->pending.back();
+    for (auto &x : ev.ev) {
+      if (a->ev_pool.empty()) {
+        HIP_TRY(hipEventCreate(&x));
+      } else {
+        x = a->ev_pool.back();
+        a->ev_pool.pop_back();
+      }
+    }
+    a->pending.emplace_back(sizeof(T) == 8 ? FMCA_PRECISION_FP64 : FMCA_PRECISION_FP32, ev);
+    pe = &a->pending.back().second;
   }
   // the assembled blocks serve plain forward actions of their precision
   StoredBlocks<T> st;
@@ -547,10 +567,36 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
   return FMCA_OK;
 }
 
+fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs,
+                                  int64_t nrhs, double *d_out, int64_t ld_out, int precision,
+                                  bool natural, bool transpose, hipStream_t stream,
+                                  const VariantGroups &groups);
+
+// One action on `stream`, ordered after the actor's previous action when that
+// ran on another stream: the workspaces (permuted rhs, moments, sums) are
+// shared by every action of the actor.
 fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
                         double *d_out, int64_t ld_out, int precision, bool natural,
                         bool transpose, hipStream_t stream,
                         const VariantGroups &groups = VariantGroups::single(KV_VALUE)) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(stream, &cs));
+  const bool captured = cs != hipStreamCaptureStatusNone;
+  if (!captured && a->order_valid && a->order_stream != stream)
+    HIP_TRY(hipStreamWaitEvent(stream, a->order_ev, 0));
+  fmca_status st = enqueue_any_unordered(a, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision,
+                                         natural, transpose, stream, groups);
+  if (st != FMCA_OK || captured) return st;
+  HIP_TRY(hipEventRecord(a->order_ev, stream));
+  a->order_stream = stream;
+  a->order_valid = true;
+  return FMCA_OK;
+}
+
+fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs,
+                                  int64_t nrhs, double *d_out, int64_t ld_out, int precision,
+                                  bool natural, bool transpose, hipStream_t stream,
+                                  const VariantGroups &groups) {
   if (transpose && !a->shared && !a->have_pat_t) {
     compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
     HIP_TRY(a->d_pat_t.upload(a->pat_t));
@@ -1260,29 +1306,44 @@ fmca_status fmca_h2_set_profiling(fmca_h2_actor *actor, int enable) {
   return FMCA_OK;
 }
 
-fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t count[6], int reset) {
+fmca_status fmca_h2_phase_times_precision(fmca_h2_actor *actor, int precision, double ms[6],
+                                          int64_t count[6], int reset) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
+  if (precision != -1 && precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
   std::lock_guard<std::mutex> lock(actor->mtx);
-  for (auto &pe : actor->pending) {
+  for (auto &pp : actor->pending) {
+    PhaseEvents &pe = pp.second;
     HIP_TRY(hipEventSynchronize(pe.ev[PH_COUNT]));
     for (int k = 0; k < PH_COUNT; ++k) {
       float t = 0;
       HIP_TRY(hipEventElapsedTime(&t, pe.ev[k], pe.ev[k + 1]));
-      actor->phase_ms[k] += t;
-      actor->phase_count[k] += 1;
+      actor->phase_ms[pp.first][k] += t;
+      actor->phase_count[pp.first][k] += 1;
     }
-    for (auto &e : pe.ev) (void)hipEventDestroy(e);
+    for (auto &e : pe.ev) actor->ev_pool.push_back(e);
   }
   actor->pending.clear();
   for (int k = 0; k < PH_COUNT; ++k) {
-    if (ms) ms[k] = actor->phase_ms[k];
-    if (count) count[k] = actor->phase_count[k];
-    if (reset) {
-      actor->phase_ms[k] = 0;
-      actor->phase_count[k] = 0;
+    double t = 0;
+    int64_t c = 0;
+    for (int p = 0; p < 2; ++p) {
+      if (precision != -1 && p != precision) continue;
+      t += actor->phase_ms[p][k];
+      c += actor->phase_count[p][k];
+      if (reset) {
+        actor->phase_ms[p][k] = 0;
+        actor->phase_count[p][k] = 0;
+      }
     }
+    if (ms) ms[k] = t;
+    if (count) count[k] = c;
   }
   return FMCA_OK;
+}
+
+fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t count[6], int reset) {
+  return fmca_h2_phase_times_precision(actor, -1, ms, count, reset);
 }
 
 fmca_status fmca_device_memory(int64_t *live_bytes, int64_t *peak_bytes, int reset_peak) {

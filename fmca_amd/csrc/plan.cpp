@@ -476,7 +476,7 @@ bool build_tree_with(Tree &T, int dim, int64_t n, const double *P, const double 
   while (lb < le) {
     // Split every node of this level above the leaf size: sort its index
     // range by the longest edge of its split box and halve it
-    // (ClusterTreeSplitter.h:198-219).
+    // (ClusterTreeSplitter.h:60-81).
     segs.clear();
     seg_node.clear();
     for (int64_t k = lb; k < le; ++k) {

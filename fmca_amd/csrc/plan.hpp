@@ -96,7 +96,7 @@ struct Tree {
 
 // ClusterTree with CardinalityBisection and shrink-to-fit boxes
 // (FMCA/src/Clustering/ClusterTreeInitializer_ClusterTree.h:24-132,
-//  FMCA/src/Clustering/ClusterTreeSplitter.h:198-219).
+//  FMCA/src/Clustering/ClusterTreeSplitter.h:60-81).
 void build_tree(Tree &T, int dim, int64_t n, const double *P, int64_t min_csize);
 
 // A range of the cluster-order permutation and a coordinate axis.

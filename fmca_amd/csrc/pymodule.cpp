@@ -279,12 +279,12 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
       .def("set_profiling", [](Actor &a, bool on) { check(fmca_h2_set_profiling(a.h, on ? 1 : 0)); })
       .def(
           "phase_times",
-          [](Actor &a, bool reset) {
+          [](Actor &a, bool reset, int precision) {
             double ms[6];
             int64_t cnt[6];
             {
               py::gil_scoped_release nogil;
-              check(fmca_h2_phase_times(a.h, ms, cnt, reset ? 1 : 0));
+              check(fmca_h2_phase_times_precision(a.h, precision, ms, cnt, reset ? 1 : 0));
             }
             py::dict d;
             const char *names[6] = {"permute",  "upward_leaf",        "upward_transfer",
@@ -292,7 +292,7 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
             for (int k = 0; k < 6; ++k) d[names[k]] = py::make_tuple(ms[k], cnt[k]);
             return d;
           },
-          py::arg("reset") = false)
+          py::arg("reset") = false, py::arg("precision") = -1)
       .def("info", &info)
       .def("interpolation",
            [](const Actor &a) {

@@ -114,11 +114,14 @@ fmca_status fmca_h2_plan_report(const fmca_h2_actor *actor, double *tree_seconds
 /* H2Matrix::computeH2Matrix (H2MatrixBase.h:222-255): generate every coupling
  * matrix S_ts = K(xi_t, xi_s), every dense block K(P_t, P_s) and every leaf
  * basis V_t once and keep them in device memory, in the given precision, as
- * matrix-core operand tiles.  Afterwards the plain actions of that precision
- * (fmca_h2_action*, natural or cluster order; not the transposed, variant or
- * CG forms) stream them instead of generating entries, as operator*
+ * matrix-core operand tiles.  Afterwards every forward action of that
+ * precision with the kernel value -- fmca_h2_action* in natural or cluster
+ * order, the matvecs of fmca_h2_cg_device and the FMCA_KERNEL_VALUE forward
+ * calls of fmca_h2_action_variant(s)_device (so the differentiable forward
+ * too) -- streams them instead of generating entries, as operator*
  * (H2MatrixBase.h:107-109, matrix_vector_product_impl.h:17-98) applies the
- * stored blocks.  One precision at a time; fmca_h2_set_kernel with other
+ * stored blocks; transposed actions and the derivative variants still
+ * generate entries.  One precision at a time; fmca_h2_set_kernel with other
  * kernel parameters, and fmca_h2_release_assembled, free them.  *bytes (if
  * not null): the device bytes held (about 4 B per entry in FP32, 8 in FP64,
  * entries padded to 16-row / 4-column tiles).  FMCA_ERR_NOT_SUPPORTED for
@@ -160,7 +163,12 @@ fmca_status fmca_h2_action_cluster_order(fmca_h2_actor *actor, const double *rhs
 /* Device-memory form of the two calls above: d_rhs / d_out are device
  * pointers, the work is enqueued on `stream` (a hipStream_t, NULL = default
  * stream) and the call returns without synchronising.  cluster_order != 0
- * selects the H2MatrixBase::action ordering. */
+ * selects the H2MatrixBase::action ordering.  Every action of an actor uses
+ * the actor's workspaces, so an action enqueued on another stream than the
+ * actor's previous action first waits for that action (an event on the
+ * previous stream); this holds for every device entry point below.  Work
+ * captured into a HIP graph records no such event: while a stream captures
+ * actions of an actor, no other stream may drive it. */
 fmca_status fmca_h2_action_device(fmca_h2_actor *actor, const double *d_rhs, int64_t ld_rhs,
                                   int64_t nrhs, double *d_out, int64_t ld_out, int precision,
                                   int cluster_order, void *stream);
@@ -298,6 +306,14 @@ fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *co
  * milliseconds and the number of actions per phase; reset != 0 clears them. */
 fmca_status fmca_h2_set_profiling(fmca_h2_actor *actor, int enable);
 fmca_status fmca_h2_phase_times(fmca_h2_actor *actor, double ms[6], int64_t count[6], int reset);
+
+/* fmca_h2_phase_times restricted to the actions of one precision
+ * (FMCA_PRECISION_FP64 / _FP32; -1: both, as fmca_h2_phase_times); reset
+ * clears that precision's sums.  The events of read actions are kept for
+ * later profiled actions, so that profiling records events but creates none
+ * once warm (no FMCA counterpart). */
+fmca_status fmca_h2_phase_times_precision(fmca_h2_actor *actor, int precision, double ms[6],
+                                          int64_t count[6], int reset);
 
 /* Diagnostic (no FMCA counterpart): evaluates the device exp (fn 0 FP64,
  * 2 FP32) or sqrt (fn 1 FP64, 3 FP32) that the kernel-entry loops use, on n

@@ -3,12 +3,14 @@
 // Drop-in for FMCA::H2kernelActor<Covariance, Moments, H2ClusterTree,
 // MatrixEvaluator> (FMCA/src/H2Matrix/H2kernelActor.h:17-94): the same
 // constructor (kernel, Peval, P, mpole_deg, eta) and `M action(const M &rhs)`.
-// The second template argument is the Moments type: FMCA's
-// NystromMoments<TotalDegreeInterpolator> (recognised by its interpolator's
-// idcs(), TotalDegreeInterpolator.h:56) or fmca_amd::TotalDegreeMoments selects
-// the total-degree form, anything else the tensor-product Chebyshev moments.
-// The arguments after it are accepted and ignored (the tree is a ClusterTree
-// with CardinalityBisection, the evaluator unsymmetric Nystrom).
+// The second template argument is the Moments type:
+// FMCA::NystromMoments<TensorProductInterpolator> (or
+// fmca_amd::TensorProductMoments) selects the tensor-product Chebyshev
+// moments, FMCA::NystromMoments<TotalDegreeInterpolator> (or
+// fmca_amd::TotalDegreeMoments) the total-degree form; any other Moments type
+// (e.g. NystromMoments<MonomialInterpolator>) fails to compile.  The
+// arguments after it are accepted and ignored (the tree is a ClusterTree with
+// CardinalityBisection, the evaluator unsymmetric Nystrom).
 //
 // Matrix types: anything column-major with rows(), cols(), data() and a
 // (rows, cols) constructor -- Eigen::MatrixXd qualifies.  Kernel types:
@@ -73,25 +75,56 @@ enum class Planner : int { Host = FMCA_PLAN_HOST, Device = FMCA_PLAN_DEVICE };
 
 // Moments tags for callers without FMCA's headers.
 struct TensorProductMoments {};
-struct TotalDegreeMoments {
-  struct Interpolator {
-    int idcs() const { return 0; }
-  };
-  Interpolator interp() const { return {}; }
-};
+struct TotalDegreeMoments {};
 
-// FMCA_INTERP_* of a Moments type: total degree when moments.interp().idcs()
-// exists (TotalDegreeInterpolator), tensor product otherwise.
-template <typename Moments, typename = void>
-struct moments_interpolator : std::integral_constant<int, FMCA_INTERP_TENSOR_PRODUCT> {};
+}  // namespace fmca_amd
+
+// FMCA's interpolators and moments class (FMCA/src/Interpolator/
+// TensorProductInterpolator.h:31, TotalDegreeInterpolator.h:19,
+// FMCA/src/Moments/NystromMoments.h:16-17), declared so that the mapping
+// below names them whether or not FMCA's headers are included.
+namespace FMCA {
+class TensorProductInterpolator;
+class TotalDegreeInterpolator;
+template <typename Interpolator>
+class NystromMoments;
+}  // namespace FMCA
+
+namespace fmca_amd {
+
+// FMCA_INTERP_* of an interpolator type; -1 for an interpolator this library
+// does not build (MonomialInterpolator, WeightedTotalDegreeInterpolator, ...).
+template <typename Interpolator>
+struct interpolator_id : std::integral_constant<int, -1> {};
+template <>
+struct interpolator_id<FMCA::TensorProductInterpolator>
+    : std::integral_constant<int, FMCA_INTERP_TENSOR_PRODUCT> {};
+template <>
+struct interpolator_id<FMCA::TotalDegreeInterpolator>
+    : std::integral_constant<int, FMCA_INTERP_TOTAL_DEGREE> {};
+
+// FMCA_INTERP_* of a Moments type: NystromMoments<I> maps through I, the two
+// tags above map to themselves; any other Moments type is -1.
 template <typename Moments>
-struct moments_interpolator<Moments,
-                            std::void_t<decltype(std::declval<const Moments &>().interp().idcs())>>
+struct moments_interpolator : std::integral_constant<int, -1> {};
+template <typename Interpolator>
+struct moments_interpolator<FMCA::NystromMoments<Interpolator>>
+    : interpolator_id<Interpolator> {};
+template <>
+struct moments_interpolator<TensorProductMoments>
+    : std::integral_constant<int, FMCA_INTERP_TENSOR_PRODUCT> {};
+template <>
+struct moments_interpolator<TotalDegreeMoments>
     : std::integral_constant<int, FMCA_INTERP_TOTAL_DEGREE> {};
 
 template <typename Covariance = CovarianceKernel, typename Moments = TensorProductMoments,
           typename... FmcaPolicies>
 class H2kernelActor {
+  static_assert(moments_interpolator<Moments>::value >= 0,
+                "fmca_amd::H2kernelActor: Moments must be NystromMoments<TensorProductInterpolator>, "
+                "NystromMoments<TotalDegreeInterpolator>, TensorProductMoments or "
+                "TotalDegreeMoments");
+
  public:
   // FMCA's constructor (H2kernelActor.h:21-36); the planner argument is an
   // addition with a default, so FMCA's calls compile unchanged.

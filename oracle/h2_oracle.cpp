@@ -8,7 +8,7 @@
 // Each piece cites the reference file:line it follows (paths relative to the
 // FMCA repository root):
 //   tree build      FMCA/src/Clustering/ClusterTreeInitializer_ClusterTree.h:24-132
-//   splitter        FMCA/src/Clustering/ClusterTreeSplitter.h:198-219
+//   splitter        FMCA/src/Clustering/ClusterTreeSplitter.h:60-81
 //   level order     FMCA/src/util/IDDFSForwardIterator.h:37-65
 //   multi-indices   FMCA/src/util/MultiIndexSet.h:87-111,147-187
 //   interpolation   FMCA/src/Interpolator/TensorProductInterpolator.h:39-81,
@@ -480,7 +480,7 @@ static void root_box(Tree &T, Index n) {
   }
 }
 
-// CardinalityBisection (ClusterTreeSplitter.h:198-219) applied to node `t`,
+// CardinalityBisection (ClusterTreeSplitter.h:60-81) applied to node `t`,
 // whose sons `a`, `b` start as copies of t.
 static void cardinality_split(Tree &T, int a, int b) {
   const int d = T.dim;

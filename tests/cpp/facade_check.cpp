@@ -26,18 +26,15 @@ struct Mat {
   std::vector<double> v_;
 };
 
-// stand-ins shaped like FMCA::NystromMoments<TotalDegreeInterpolator> and
-// NystromMoments<TensorProductInterpolator>: only interp().idcs() is looked at
-struct TDInterp {
-  struct Set {};
-  Set idcs() const { return {}; }
-};
-struct TPInterp {};
-template <typename I>
-struct NystromMomentsLike {
-  const I &interp() const { return i_; }
-  I i_;
-};
+// stand-ins with FMCA's names for NystromMoments<TotalDegreeInterpolator> and
+// NystromMoments<TensorProductInterpolator> (FMCA's headers need Eigen): the
+// facade maps the Moments type by the interpolator it is instantiated with
+namespace FMCA {
+class TensorProductInterpolator {};
+class TotalDegreeInterpolator {};
+template <typename Interpolator>
+class NystromMoments {};
+}  // namespace FMCA
 
 template <typename Actor>
 int run(Actor &act, const char *mode, const Mat &rhs, const char *path) {
@@ -87,12 +84,14 @@ int main(int argc, char **argv) {
                                                        : fmca_amd::Planner::Host;
     // FMCA's four template arguments: kernel, Moments, tree, evaluator
     if (td) {
-      fmca_amd::H2kernelActor<MyKernel, NystromMomentsLike<TDInterp>, struct H2ClusterTree,
+      fmca_amd::H2kernelActor<MyKernel, FMCA::NystromMoments<FMCA::TotalDegreeInterpolator>,
+                              struct H2ClusterTree,
                               struct Evaluator>
           act(kernel, P, P, (unsigned)hdr[3], par[0], planner);
       return run(act, argv[1], rhs, argv[3]);
     }
-    fmca_amd::H2kernelActor<MyKernel, NystromMomentsLike<TPInterp>, struct H2ClusterTree,
+    fmca_amd::H2kernelActor<MyKernel, FMCA::NystromMoments<FMCA::TensorProductInterpolator>,
+                            struct H2ClusterTree,
                             struct Evaluator>
         act(kernel, P, P, (unsigned)hdr[3], par[0], planner);
     return run(act, argv[1], rhs, argv[3]);

@@ -109,3 +109,43 @@ def test_facade_device_planner_and_assembled_blocks(tmp_path, mode, scheme):
     y = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(4, 6000).T
     ref = oracle.OracleH2(P, P, "MATERN32", 0.5, 3, 0.8, scheme=scheme).action(rhs)
     assert rel_err(y, ref) <= 1e-12
+
+
+_MOMENTS_SNIPPET = r"""
+#include "fmca_amd/H2kernelActor.hpp"
+namespace FMCA {
+class TensorProductInterpolator {};
+class TotalDegreeInterpolator {};
+class MonomialInterpolator {};
+template <typename I> class NystromMoments {};
+}
+struct K { std::string kernelType() const { return "EXPONENTIAL"; }
+           double l() const { return 1; } double c() const { return 1; }
+           double nu() const { return 1; } };
+int main() { return sizeof(fmca_amd::H2kernelActor<K, MOMENTS>); }
+"""
+
+
+@pytest.mark.parametrize("moments,interp", [
+    ("FMCA::NystromMoments<FMCA::TensorProductInterpolator>", 0),
+    ("FMCA::NystromMoments<FMCA::TotalDegreeInterpolator>", 1),
+    ("fmca_amd::TensorProductMoments", 0),
+    ("fmca_amd::TotalDegreeMoments", 1),
+    ("FMCA::NystromMoments<FMCA::MonomialInterpolator>", None),
+    ("int", None),
+])
+def test_facade_moments_type_is_checked(tmp_path, moments, interp):
+    """The facade maps FMCA's Moments type to the interpolator it was
+    instantiated with (NystromMoments.h:16-17) and refuses to compile any
+    other: NystromMoments<MonomialInterpolator> (MonomialInterpolator.h:20)
+    has no counterpart here and must not silently run another basis."""
+    src = tmp_path / "m.cpp"
+    src.write_text(_MOMENTS_SNIPPET.replace("MOMENTS", moments)
+                   + (f"static_assert(fmca_amd::moments_interpolator<{moments}>::value == "
+                      f"{interp}, \"id\");\n" if interp is not None else ""))
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
+                       capture_output=True, text=True)
+    if interp is None:
+        assert r.returncode != 0 and "Moments must be" in r.stderr
+    else:
+        assert r.returncode == 0, r.stderr

@@ -25,6 +25,18 @@ CASES = [
     ("matern72-3d", 3000, None, 3, 2, "MATERN72", 0.6, 0.8, 1),
     ("matern92-eta04", 3000, None, 2, 3, "MATERN92", 0.6, 0.4, 1),
     ("small-all-dense", 100, None, 3, 4, "EXPONENTIAL", 1.0, 0.8, 2),
+    # the kernels that take the run-time switch (CovarianceKernel.h:56-61,103-116),
+    # on the VALU (< 8 rhs) and matrix-core (>= 8 rhs) forms
+    ("multiquadric-3d-valu", 3000, None, 3, 3, "MULTIQUADRIC", 0.6, 0.8, 3),
+    ("multiquadric-rect-mfma", 2500, 3000, 3, 3, "MULTIQUADRIC", 0.6, 0.8, 16),
+    ("biharmonic2d-valu", 4000, None, 2, 4, "BIHARMONIC2D", 0.5, 0.8, 1),
+    ("biharmonic2d-mfma", 4000, None, 2, 4, "BIHARMONIC2D", 0.5, 0.8, 24),
+    ("biharmonic3d-valu", 3000, 2500, 3, 3, "BIHARMONIC3D", 0.5, 0.8, 2),
+    ("biharmonic3d-mfma", 3000, None, 3, 3, "BIHARMONIC3D", 0.5, 0.8, 8),
+    ("triharmonic3d-valu", 3000, None, 3, 3, "TRIHARMONIC3D", 0.7, 0.8, 1),
+    ("triharmonic3d-mfma", 3000, 2000, 3, 3, "TRIHARMONIC3D", 0.7, 0.8, 32),
+    ("tps2d-valu", 4000, None, 2, 4, "TPS2D", 1.0, 0.8, 5),
+    ("tps2d-mfma", 4000, 3500, 2, 4, "TPS2D", 1.0, 0.8, 16),
     ("rhs-40", 3000, None, 3, 3, "MATERN32", 0.5, 0.8, 40),
     ("q216-16rhs-two-row-passes", 4000, None, 3, 5, "EXPONENTIAL", 0.5, 0.8, 16),
     ("q343-9rhs-valu-coupling", 3000, None, 3, 6, "GAUSSIAN", 0.3, 0.8, 9),

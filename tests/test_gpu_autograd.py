@@ -231,3 +231,59 @@ def test_variant_groups_equal_separate_actions(kernel, dim, deg, transpose, grou
                                        Yg.data_ptr(), rows_out, prec, stream)
             err = (torch.linalg.norm(Y[:, c0:c1] - Yg) / torch.linalg.norm(Yg)).item()
             assert err <= tol, (prec, g, v, err)
+
+
+def test_spec_replans_for_fresh_points_at_a_reused_address():
+    """One H2Spec across steps whose points tensor is freshly computed each
+    time (P = P0 + shift): the caching allocator may hand the new tensor the
+    freed one's address with version 0, so the plan cache must not key on the
+    address.  Every step's value equals a fresh spec's on the same points."""
+    import torch
+    from fmca_amd.autograd import H2Spec, h2_matvec
+    print("[case] spec reuse with fresh points tensors", flush=True)
+    P0, alpha, _ = _problem(n=3000, d=2, nrhs=4)
+    l = torch.tensor(0.3, dtype=torch.float64)
+    spec = H2Spec("MATERN32", 4, 0.8)
+    for k in range(3):
+        P = P0 + 0.05 * k  # freed after the step; the next one may reuse its memory
+        y = h2_matvec(alpha, l, P, spec)
+        ref = h2_matvec(alpha, l, P, H2Spec("MATERN32", 4, 0.8))
+        assert rel_err(y.cpu().numpy(), ref.cpu().numpy()) <= 1e-13, k
+        del P, y
+    # in-place change behind the version counter (through .data)
+    P = P0.clone()
+    y0 = h2_matvec(alpha, l, P, spec)
+    P.data.mul_(0.5)
+    y1 = h2_matvec(alpha, l, P, spec)
+    ref = h2_matvec(alpha, l, P0 * 0.5, H2Spec("MATERN32", 4, 0.8))
+    assert rel_err(y1.cpu().numpy(), ref.cpu().numpy()) <= 1e-13
+    assert rel_err(y0.cpu().numpy(), y1.cpu().numpy()) > 1e-3
+
+
+def test_backward_uses_its_own_forward_lengthscale():
+    """Two forwards of one spec with different lengthscales, then the backward
+    of the first: its gradients equal those of a lone forward at that l."""
+    import torch
+    from fmca_amd.autograd import H2Spec, h2_matvec
+    print("[case] backward after a second forward", flush=True)
+    P, alpha, y = _problem(n=3000, d=2, nrhs=4)
+
+    def grads(l1, second):
+        spec = H2Spec("MATERN32", 4, 0.8)
+        Pg = P.clone().requires_grad_(True)
+        ag = alpha.clone().requires_grad_(True)
+        lt = torch.tensor(l1, dtype=torch.float64, requires_grad=True)
+        loss = 0.5 * ((h2_matvec(ag, lt, Pg, spec) - y) ** 2).sum()
+        actor0 = [spec._actor]
+        if second:
+            # the same points object: the spec reuses its actor and sets l = 0.9 on it
+            h2_matvec(ag.detach(), torch.tensor(0.9, dtype=torch.float64), Pg, spec)
+            assert spec._actor is actor0[0]
+        loss.backward()
+        return ag.grad, lt.grad, Pg.grad
+
+    a0, l0, p0 = grads(0.3, False)
+    a1, l1, p1 = grads(0.3, True)
+    assert torch.allclose(a1, a0, rtol=0, atol=1e-13 * a0.abs().max().item())
+    assert abs(l1.item() - l0.item()) <= 1e-12 * abs(l0.item())
+    assert torch.allclose(p1, p0, rtol=0, atol=1e-12 * p0.abs().max().item())

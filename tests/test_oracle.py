@@ -118,6 +118,73 @@ def test_tiny_tree_by_hand():
     assert cl["bbmax"][:, 0].tolist() == [0.9, 0.4, 0.9, 0.2, 0.4, 0.7, 0.9]
 
 
+# Hand-worked trees in 2-D and 3-D that pin the split-axis rule of
+# CardinalityBisection (ClusterTreeSplitter.h:60-81): the axis is the longest
+# edge of the node's *split box* (the root's box widened by eps |coordinate|,
+# ClusterTreeInitializer_ClusterTree.h:42-52, then cut at the boundary points
+# of every split, :72-80), not of the tight point box, and maxCoeff takes the
+# FIRST axis on equal edges (ClusterTreeSplitter.h:67).  degree 0 gives q = 1,
+# so nodes split down to single points.  Every expected value below was worked
+# out by hand from the points; the comments give the steps.
+#
+# 2-D: A (0.0, .55)  B (.2, .45)  C (.4, .5)  D (.7, 0.)  E (.9, 1.)  F (1., .6),
+# stored in the order E B F A D C (E = 0, B = 1, F = 2, A = 3, D = 4, C = 5).
+#  root  split box x [0, 1+eps], y [0, 1+eps]: equal edges -> x (first).
+#        by x: A B C | D E F
+#  c1 {A B C}  split box x [0, .4], y [0, 1+eps] -> y (the tight box
+#        x .4 > y .1 would say x).  by y: B | C A
+#  c2 {D E F}  split box x [.7, 1+eps], y [0, 1+eps] -> y.  by y: D | F E
+#  c12 {C A}   split box x [0, .4], y [.5, 1+eps] -> y (tight: x .4 > y .05).
+#        by y: C | A   (by x it would be A | C)
+#  c22 {F E}   split box x [.7, 1+eps], y [.6, 1+eps] -> y.  by y: F | E
+#        (by x it would be E | F)
+#  level order: 0 root, 1 c1, 2 c2, 3 c11, 4 c12, 5 c21, 6 c22, 7-8 sons of c12,
+#  9-10 sons of c22; leaves left to right B C A D F E.
+HAND_2D = dict(
+    P=[[0.9, 1.0], [0.2, 0.45], [1.0, 0.6], [0.0, 0.55], [0.7, 0.0], [0.4, 0.5]],
+    perm=[1, 5, 3, 4, 2, 0],
+    size=[6, 3, 3, 1, 2, 1, 2, 1, 1, 1, 1],
+    begin=[0, 0, 3, 0, 1, 3, 4, 1, 2, 4, 5],
+    level=[0, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3],
+    sons=[[1, 2], [3, 4], [5, 6], [-1, -1], [7, 8], [-1, -1], [9, 10]] + [[-1, -1]] * 4)
+# 3-D: A (0., .45, 0.)  B (.1, .4, 1.)  C (.6, 0., .3)  D (1., 1., .2),
+# stored in the order C A D B (C = 0, A = 1, D = 2, B = 3).
+#  root  split box [0, 1+eps] on every axis: three equal edges -> x.
+#        by x: A B | C D
+#  c1 {A B}  split box x [0, .1], y [0, 1+eps], z [0, 1+eps]: y and z equal
+#        -> y (first).  The tight box (x .1, y .05, z 1) would say z, and the
+#        last of the equal edges would also be z.  by y: B | A (by z: A | B)
+#  c2 {C D}  split box x [.6, 1+eps], y, z [0, 1+eps] -> y.  by y: C | D
+#        (by z it would be D | C)
+#  leaves left to right B A C D.
+HAND_3D = dict(
+    P=[[0.6, 0.0, 0.3], [0.0, 0.45, 0.0], [1.0, 1.0, 0.2], [0.1, 0.4, 1.0]],
+    perm=[3, 1, 0, 2],
+    size=[4, 2, 2, 1, 1, 1, 1],
+    begin=[0, 0, 2, 0, 1, 2, 3],
+    level=[0, 1, 1, 2, 2, 2, 2],
+    sons=[[1, 2], [3, 4], [5, 6]] + [[-1, -1]] * 4)
+
+
+@pytest.mark.parametrize("hand", [HAND_2D, HAND_3D], ids=["2d", "3d"])
+def test_split_axis_rule_by_hand(hand):
+    P = np.asfortranarray(np.array(hand["P"], dtype=np.float64).T)
+    A = oracle.OracleH2(P, P, "EXPONENTIAL", 1.0, 0, 0.8)
+    assert A.q == 1
+    assert A.indices(1).tolist() == hand["perm"]
+    cl = A.clusters(1)
+    for k in ("size", "begin", "level", "sons"):
+        assert cl[k].tolist() == hand[k], k
+    # single-point leaves: the flat-box rule widens each axis by 1e2 eps
+    eps = np.finfo(float).eps
+    for c in range(len(hand["size"])):
+        if hand["sons"][c][0] < 0:
+            p = P[:, hand["perm"][hand["begin"][c]]]
+            assert np.array_equal(cl["bbmin"][c], p - 1e2 * eps)
+            assert np.array_equal(cl["bbmax"][c], p + 1e2 * eps)
+    _check_tree(A, 1, P, A.q)
+
+
 def _check_tree(A, which, P, q):
     n = P.shape[1]
     perm = A.indices(which)

@@ -348,3 +348,21 @@ def test_assemble_without_device_reports_status():
     lib = ctypes.CDLL(LIB)
     assert lib.fmca_h2_assemble(None, 0, None) == 1
     assert lib.fmca_h2_release_assembled(None) == 1
+
+
+@pytest.mark.parametrize("hand", ["HAND_2D", "HAND_3D"])
+def test_planner_split_axis_rule_by_hand(hand):
+    """The product planner (host and, with a GPU, device) reproduces the
+    hand-worked 2-D / 3-D trees of test_oracle.py, which pin the split-box
+    longest-edge rule with first-axis ties (ClusterTreeSplitter.h:60-81)."""
+    import test_oracle
+    h = getattr(test_oracle, hand)
+    P = np.asfortranarray(np.array(h["P"], dtype=np.float64).T)
+    planners = ["host"] + (["device"] if gpu_available() else [])
+    for planner in planners:
+        H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("EXPONENTIAL", 1.0), P, P, 0, 0.8,
+                                   planner=planner)
+        assert H.indices(1).tolist() == h["perm"], planner
+        cl = H.clusters(1)
+        for k in ("size", "begin", "level", "sons"):
+            assert np.asarray(cl[k]).tolist() == h[k], (planner, k)
