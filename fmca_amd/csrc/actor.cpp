@@ -437,12 +437,21 @@ KernelParams<T> kernel_params(const fmca_h2_actor *a, const VariantGroups &g) {
   return k;
 }
 
+// A workspace region the caller provides (the device scratch of
+// fmca_h2_action_device_noalloc): xs, mom and tlhs are carved from it.
+struct Region {
+  void *base;
+  size_t bytes;
+};
+
+constexpr size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
 // Enqueue one action on `stream`.  transpose swaps the roles of the trees.
 template <typename T>
 fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r_rows,
                     const DevTreeReals<T> &r_cols, const double *d_rhs, int64_t ld_rhs,
                     int64_t nrhs, double *d_out, int64_t ld_out, bool natural, bool transpose,
-                    hipStream_t stream, const VariantGroups &groups) {
+                    hipStream_t stream, const VariantGroups &groups, const Region *fixed) {
   const Tree &tr_rows = a->rows();
   const Tree &tr_cols = a->cols();
   const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
@@ -462,12 +471,30 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
   HostLevels l_tgt{t_tgt.depth, t_tgt.level_ptr.data()};
   HostLevels l_src{t_src.depth, t_src.level_ptr.data()};
   const int q = a->ip.q;
-  HIP_TRY(ws.xs.alloc((size_t)t_src.npts * nrhs * sizeof(T)));
-  HIP_TRY(ws.mom.alloc((size_t)t_src.ncl * q * nrhs * sizeof(T)));
-  HIP_TRY(ws.tlhs.alloc((size_t)t_tgt.ncl * q * nrhs * sizeof(T)));
-  ActionBuffers<T> buf{ws.xs.template as<T>(), ws.mom.template as<T>(), ws.tlhs.template as<T>()};
+  const size_t b_xs = (size_t)t_src.npts * nrhs * sizeof(T);
+  const size_t b_mom = (size_t)t_src.ncl * q * nrhs * sizeof(T);
+  const size_t b_tlhs = (size_t)t_tgt.ncl * q * nrhs * sizeof(T);
+  ActionBuffers<T> buf;
+  if (fixed) {
+    // the caller's region: xs | mom | tlhs, each start aligned to 256 bytes
+    const size_t o_mom = align_up(b_xs), o_tlhs = o_mom + align_up(b_mom);
+    if (o_tlhs + b_tlhs > fixed->bytes)
+      return fail(FMCA_ERR_NOT_SUPPORTED,
+                  "the action needs " + std::to_string(o_tlhs + b_tlhs) +
+                      " bytes of workspace, the device scratch holds " +
+                      std::to_string(fixed->bytes));
+    unsigned char *base = static_cast<unsigned char *>(fixed->base);
+    buf = ActionBuffers<T>{reinterpret_cast<T *>(base), reinterpret_cast<T *>(base + o_mom),
+                           reinterpret_cast<T *>(base + o_tlhs)};
+  } else {
+    HIP_TRY(ws.xs.alloc(b_xs));
+    HIP_TRY(ws.mom.alloc(b_mom));
+    HIP_TRY(ws.tlhs.alloc(b_tlhs));
+    buf = ActionBuffers<T>{ws.xs.template as<T>(), ws.mom.template as<T>(),
+                           ws.tlhs.template as<T>()};
+  }
   PhaseEvents *pe = nullptr;
-  if (a->profiling) {
+  if (a->profiling && !fixed) {
     PhaseEvents ev;
     for (auto &x : ev.ev) {
       if (a->ev_pool.empty()) {
@@ -570,7 +597,7 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
 fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs,
                                   int64_t nrhs, double *d_out, int64_t ld_out, int precision,
                                   bool natural, bool transpose, hipStream_t stream,
-                                  const VariantGroups &groups);
+                                  const VariantGroups &groups, const Region *fixed);
 
 // One action on `stream`, ordered after the actor's previous action when that
 // ran on another stream: the workspaces (permuted rhs, moments, sums) are
@@ -578,14 +605,15 @@ fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t
 fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, int64_t nrhs,
                         double *d_out, int64_t ld_out, int precision, bool natural,
                         bool transpose, hipStream_t stream,
-                        const VariantGroups &groups = VariantGroups::single(KV_VALUE)) {
+                        const VariantGroups &groups = VariantGroups::single(KV_VALUE),
+                        const Region *fixed = nullptr) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   HIP_TRY(hipStreamIsCapturing(stream, &cs));
   const bool captured = cs != hipStreamCaptureStatusNone;
   if (!captured && a->order_valid && a->order_stream != stream)
     HIP_TRY(hipStreamWaitEvent(stream, a->order_ev, 0));
   fmca_status st = enqueue_any_unordered(a, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision,
-                                         natural, transpose, stream, groups);
+                                         natural, transpose, stream, groups, fixed);
   if (st != FMCA_OK || captured) return st;
   HIP_TRY(hipEventRecord(a->order_ev, stream));
   a->order_stream = stream;
@@ -596,7 +624,7 @@ fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, i
 fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs,
                                   int64_t nrhs, double *d_out, int64_t ld_out, int precision,
                                   bool natural, bool transpose, hipStream_t stream,
-                                  const VariantGroups &groups) {
+                                  const VariantGroups &groups, const Region *fixed) {
   if (transpose && !a->shared && !a->have_pat_t) {
     compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
     HIP_TRY(a->d_pat_t.upload(a->pat_t));
@@ -605,11 +633,11 @@ fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t
   if (precision == FMCA_PRECISION_FP64)
     return enqueue<double>(a, a->ws64, a->d_rows_64, a->shared ? a->d_rows_64 : a->d_cols_64,
                            d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
-                           groups);
+                           groups, fixed);
   if (precision == FMCA_PRECISION_FP32)
     return enqueue<float>(a, a->ws32, a->d_rows_32, a->shared ? a->d_rows_32 : a->d_cols_32,
                           d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
-                          groups);
+                          groups, fixed);
   return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
 }
 
@@ -645,6 +673,33 @@ fmca_status host_action(fmca_h2_actor *a, const double *rhs, int64_t ld_rhs, int
   HIP_TRY(hipMemcpy2DAsync(out, ld_out * sizeof(double), a->stage_out.p, a->m * sizeof(double),
                            a->m * sizeof(double), nrhs, hipMemcpyDeviceToHost, a->stream));
   HIP_TRY(hipStreamSynchronize(a->stream));
+  return FMCA_OK;
+}
+
+// Device scratch of fmca_h2_action_device_noalloc: a fixed array in the code
+// object, so that the entry point allocates nothing.  Each device holds its
+// own copy until the process ends; the host learns its address per device
+// with hipGetSymbolAddress (the array's name is not a device address in host
+// code).  Every call on a device shares that device's copy.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mtx;
+void *g_scratch_ptr[kMaxScratchDevices] = {nullptr};
+
+// The device address of `scratch` on `device` (which is current).
+fmca_status scratch_on_device(int device, void **ptr) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mtx);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(FMCA_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
   return FMCA_OK;
 }
 
@@ -866,6 +921,29 @@ fmca_status fmca_h2_action_device(fmca_h2_actor *actor, const double *d_rhs, int
   if (st != FMCA_OK) return st;
   return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, cluster_order == 0,
                      false, (hipStream_t)stream);
+}
+
+fmca_status fmca_h2_action_device_noalloc(fmca_h2_actor *actor, const double *d_rhs,
+                                          int64_t ld_rhs, int64_t nrhs, double *d_out,
+                                          int64_t ld_out, int precision, int cluster_order,
+                                          void *stream) {
+  if (!actor || nrhs < 0 || (nrhs && (!d_rhs || !d_out)))
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer or negative nrhs");
+  if (ld_rhs < actor->n || ld_out < actor->m)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (nrhs == 0) return FMCA_OK;
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  if (!actor->uploaded)
+    return fail(FMCA_ERR_INVALID_ARGUMENT,
+                "fmca_h2_action_device_noalloc: the plan is not on the device yet "
+                "(call fmca_h2_upload or one allocating action first)");
+  HIP_TRY(hipSetDevice(actor->device));
+  void *base = nullptr;
+  fmca_status st = scratch_on_device(actor->device, &base);
+  if (st != FMCA_OK) return st;
+  const Region region{base, kScratchBytes};
+  return enqueue_any(actor, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision, cluster_order == 0,
+                     false, (hipStream_t)stream, VariantGroups::single(KV_VALUE), &region);
 }
 
 fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *d_rhs,

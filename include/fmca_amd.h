@@ -173,6 +173,23 @@ fmca_status fmca_h2_action_device(fmca_h2_actor *actor, const double *d_rhs, int
                                   int64_t nrhs, double *d_out, int64_t ld_out, int precision,
                                   int cluster_order, void *stream);
 
+/* fmca_h2_action_device that allocates nothing.  No counterpart in FMCA: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * permuted rhs, moments and far-field sums live in a fixed 64 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of
+ * the actor's workspaces: n * nrhs * b + (row + column clusters) * q * nrhs * b
+ * bytes (b = 8 FP64, 4 FP32; each part rounded up to 256 B) must fit, else
+ * FMCA_ERR_NOT_SUPPORTED.  The plan must already be on the device
+ * (fmca_h2_upload or an earlier action), else FMCA_ERR_INVALID_ARGUMENT.  No
+ * profiling events are recorded.  Every call on one GPU shares that GPU's
+ * array, whatever the actor: calls that overlap on one GPU (other streams,
+ * other threads, other actors) race, and the caller must order them. */
+fmca_status fmca_h2_action_device_noalloc(fmca_h2_actor *actor, const double *d_rhs,
+                                          int64_t ld_rhs, int64_t nrhs, double *d_out,
+                                          int64_t ld_out, int precision, int cluster_order,
+                                          void *stream);
+
 /* Transposed action out = K(Peval, P)^T * rhs: rhs m x nrhs, out n x nrhs,
  * device memory (the adjoint needed by the differentiation form; FMCA has no
  * counterpart). */
