@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "H2 matvec Mpoint*RHS/s, n=1e6 3D, FP32+refine vs FP64 at 1e-8 rel err"
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 (vector and matrix)
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 PEAK_HBM_GBS = 8000.0
 
 
@@ -52,6 +53,9 @@ def parse():
     p.add_argument("--no-side", action="store_true",
                    help="skip the differentiation side results (n = 2e5 2-D and n = 8e6 3-D)")
     p.add_argument("--ridge", type=float, default=100.0)
+    p.add_argument("--solve-precision", default="fp32_bf16", choices=["fp32", "fp32_bf16"],
+                   help="matvecs of the refined solve: FP32 entries with FP32 (fp32) or "
+                        "BF16 three-term (fp32_bf16) matrix-core contractions")
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
                             "total-degree", "plan", "assembled"],
@@ -448,7 +452,7 @@ def side_large(args):
     return res
 
 
-def refined_solves(H, x, ridge, steps, warmup):
+def refined_solves(H, x, ridge, steps, warmup, precision):
     """The headline step: one CG solve of (K + ridge I) a = x for all columns
     to a 1e-8 relative FP64 residual, FP32 matvecs driving the recurrence and
     FP64 reliable updates (fmca_h2_cg_device), from a = 0.  Profiling events
@@ -461,7 +465,7 @@ def refined_solves(H, x, ridge, steps, warmup):
 
     def one():
         a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8,
-                       precision=fmca_amd.FP32, max_iter=2000)
+                       precision=precision, max_iter=2000)
         return a, info
     H.set_profiling(True)
     for _ in range(warmup):
@@ -476,7 +480,7 @@ def refined_solves(H, x, ridge, steps, warmup):
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / steps
     H.set_profiling(False)
-    ph32 = H.phase_times(reset=True, precision=fmca_amd.FP32)
+    ph32 = H.phase_times(reset=True, precision=precision)
     H.phase_times(reset=True)
     return t, a, infos, ph32
 
@@ -541,10 +545,13 @@ def main():
     t32, ph32m = timed_steps(H, x, out32, fmca_amd.FP32, mv_steps, 1, True)
     t64, ph64m = timed_steps(H, x, out64, fmca_amd.FP64, 3, 1, True)
     err = float(torch.linalg.norm(out32 - out64) / torch.linalg.norm(out64))
+    tbf, phbfm = timed_steps(H, x, out32, fmca_amd.FP32_BF16, mv_steps, 1, True)
+    err_bf = float(torch.linalg.norm(out32 - out64) / torch.linalg.norm(out64))
     del out32, out64
     x1 = x[:, :1].contiguous()
     o1 = torch.empty((n, 1), dtype=torch.float64, device="cuda")
     t32_1, _ = timed_steps(H, x1, o1, fmca_amd.FP32, mv_steps, 1, False)
+    solve_prec = fmca_amd.FP32_BF16 if args.solve_precision == "fp32_bf16" else fmca_amd.FP32
     t64_1, _ = timed_steps(H, x1, o1, fmca_amd.FP64, 3, 1, False)
     del o1, x1
     # the C-ABI host-buffer boundary (fmca_h2_action): H2D + action + D2H
@@ -554,7 +561,8 @@ def main():
     t_host = time.perf_counter() - t0
 
     # the headline: refined solves (timed steps), then plain FP64 CG once
-    t_ref, a_ref, infos, ph_solve = refined_solves(H, x, args.ridge, args.steps, args.warmup)
+    t_ref, a_ref, infos, ph_solve = refined_solves(H, x, args.ridge, args.steps, args.warmup,
+                                                   solve_prec)
     t_64, a_64, info64 = fp64_solve(H, x, args.ridge)
     sol_diff = float(torch.linalg.norm(a_ref - a_64) / torch.linalg.norm(a_64))
     del a_ref, a_64
@@ -581,22 +589,30 @@ def main():
     dom = max(("leaf_output", "coupling"), key=lambda k: avg(ph_solve, k))
     t_dom = avg(ph_solve, dom) / 1e3
     nt = mfma_tiles(nrhs)
+    bf = args.solve_precision == "fp32_bf16"
     if dom == "leaf_output":
         flops = 2.0 * E_near * nrhs + 2.0 * q * n * nrhs
+        bf16_flops = 3 * 2.0 * E_near * nrhs  # the three BF16 products of the near field
         entries = E_near
-        kname = f"k_leaf_output_mfma<float, 3, 1, {nt}, false>"
+        kname = (f"k_leaf_output_bf16<3, 1, {nt}>" if bf
+                 else f"k_leaf_output_mfma<float, 3, 1, {nt}, false>")
     else:
         flops = 2.0 * E_far * nrhs
+        bf16_flops = 3 * 2.0 * E_far * nrhs
         entries = E_far
-        kname = f"k_coupling_mfma<float, 3, 1, {nt}, false>"
+        kname = (f"k_coupling_bf16<3, 1, {nt}>" if bf
+                 else f"k_coupling_mfma<float, 3, 1, {nt}, false>")
     achieved = flops / t_dom / 1e12
     roofline = {
+        # the products are FP32-accurate (FP32 entries; BF16 three-term split or
+        # FP32 matrix cores): the roofline is the FP32 dense matrix peak
         "bound": "mfma",
         "kernel": kname,
         "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
+        "bf16_matrix_core_frac": (bf16_flops / t_dom / 1e12 / PEAK_BF16_TFLOPS) if bf else None,
         "traffic": traffic_from_profiles(kname),
         "traffic_source": ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                            "runs of this workload's FP32 matvec, scaled by the counters' reading "
@@ -646,7 +662,11 @@ def main():
                      "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph32m.items()}},
             "fp64": {"ms": t64 * 1e3, "mpts": n * nrhs / t64 / 1e6,
                      "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph64m.items()}},
+            "fp32_bf16": {"ms": tbf * 1e3, "mpts": n * nrhs / tbf / 1e6,
+                          "phases_ms": {k: v[0] / max(v[1], 1) for k, v in phbfm.items()},
+                          "vs_fp64_rel_err": err_bf},
             "fp32_speedup": t64 / t32,
+            "fp32_bf16_speedup": t64 / tbf,
             "fp32_vs_fp64_rel_err": err,
             "rhs1": {"fp32_mpts": n / t32_1 / 1e6, "fp64_mpts": n / t64_1 / 1e6,
                      "fp32_ms": t32_1 * 1e3, "fp64_ms": t64_1 * 1e3},

@@ -36,6 +36,7 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken instal
 
 FP64 = _C.FP64
 FP32 = _C.FP32
+FP32_BF16 = _C.FP32_BF16
 ROWS = _C.ROWS
 COLS = _C.COLS
 
@@ -359,5 +360,5 @@ def _check_out(out, rows, nrhs):
         raise ValueError("out must be column-major")
 
 
-__all__ = ["CovarianceKernel", "H2Matrix", "H2kernelActor", "FP64", "FP32", "ROWS", "COLS",
+__all__ = ["CovarianceKernel", "H2Matrix", "H2kernelActor", "FP64", "FP32", "FP32_BF16", "ROWS", "COLS",
            "KERNEL_TYPES", "INTERPOLATORS", "MultipoleFunctionEvaluator"]

@@ -35,8 +35,8 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-SOURCES_HIP = ["kernels.hip", "kernels_mfma.hip", "blas1.hip", "tree_device.hip", "assembled.hip",
-               "pattern_device.hip"]
+SOURCES_HIP = ["kernels.hip", "kernels_mfma.hip", "kernels_bf16.hip", "kernels_grad.hip",
+               "blas1.hip", "tree_device.hip", "assembled.hip", "pattern_device.hip"]
 SOURCES_CPP = ["plan.cpp", "actor.cpp"]
 
 

@@ -192,6 +192,7 @@ struct fmca_h2_actor {
   Workspace<float> ws32;
   DevBuf stage_in, stage_out;
   DevBuf cg_ws, cg_scal;  // CG vectors and per-column scalars
+  DevBuf grad_x, grad_part;  // gradient: [alpha | G | alpha] rhs, partial sums of dL/dl
   // assembled blocks (fmca_h2_assemble) of precision st_prec (-1: none)
   DevBuf st_far, st_near, st_near_off, st_v_off, st_far_tgt, st_near_leaf, st_far_order,
       st_near_order;
@@ -219,8 +220,8 @@ struct fmca_h2_actor {
   // records events but creates none once the pool is warm
   std::vector<std::pair<int, PhaseEvents>> pending;
   std::vector<hipEvent_t> ev_pool;
-  double phase_ms[2][PH_COUNT] = {{0}};
-  int64_t phase_count[2][PH_COUNT] = {{0}};
+  double phase_ms[3][PH_COUNT] = {{0}};
+  int64_t phase_count[3][PH_COUNT] = {{0}};
 
   const Tree &rows() const { return rows_tree; }
   const Tree &cols() const { return shared ? rows_tree : cols_tree; }
@@ -451,7 +452,8 @@ template <typename T>
 fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r_rows,
                     const DevTreeReals<T> &r_cols, const double *d_rhs, int64_t ld_rhs,
                     int64_t nrhs, double *d_out, int64_t ld_out, bool natural, bool transpose,
-                    hipStream_t stream, const VariantGroups &groups, const Region *fixed) {
+                    hipStream_t stream, const VariantGroups &groups, const Region *fixed,
+                    bool split_bf16 = false, const GradJob *grad = nullptr) {
   const Tree &tr_rows = a->rows();
   const Tree &tr_cols = a->cols();
   const DevTreeInts &i_cols = a->shared ? a->d_rows_i : a->d_cols_i;
@@ -504,7 +506,10 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
         a->ev_pool.pop_back();
       }
     }
-    a->pending.emplace_back(sizeof(T) == 8 ? FMCA_PRECISION_FP64 : FMCA_PRECISION_FP32, ev);
+    a->pending.emplace_back(sizeof(T) == 8   ? FMCA_PRECISION_FP64
+                            : split_bf16 ? FMCA_PRECISION_FP32_BF16
+                                         : FMCA_PRECISION_FP32,
+                            ev);
     pe = &a->pending.back().second;
   }
   // the assembled blocks serve plain forward actions of their precision
@@ -521,10 +526,12 @@ fmca_status enqueue(fmca_h2_actor *a, Workspace<T> &ws, const DevTreeReals<T> &r
     st.Rq = a->st_Rq;
     st.Kq = a->st_Kq;
   }
+  KernelParams<T> kp = kernel_params<T>(a, groups);
+  kp.split_bf16 = split_bf16 ? 1 : 0;
   hipError_t e = launch_action<T>(v_rows, v_cols, l_tgt, l_src, dpat->view(*pat),
-                                  interp_params<T>(a), kernel_params<T>(a, groups), d_rhs, ld_rhs,
+                                  interp_params<T>(a), kp, d_rhs, ld_rhs,
                                   (int)nrhs, d_out, ld_out, natural, buf, stream, pe,
-                                  use_stored ? &st : nullptr);
+                                  use_stored ? &st : nullptr, grad);
   return from_hip(e, "launch_action");
 }
 
@@ -597,7 +604,8 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
 fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs,
                                   int64_t nrhs, double *d_out, int64_t ld_out, int precision,
                                   bool natural, bool transpose, hipStream_t stream,
-                                  const VariantGroups &groups, const Region *fixed);
+                                  const VariantGroups &groups, const Region *fixed,
+                                  const GradJob *grad);
 
 // One action on `stream`, ordered after the actor's previous action when that
 // ran on another stream: the workspaces (permuted rhs, moments, sums) are
@@ -606,14 +614,14 @@ fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, i
                         double *d_out, int64_t ld_out, int precision, bool natural,
                         bool transpose, hipStream_t stream,
                         const VariantGroups &groups = VariantGroups::single(KV_VALUE),
-                        const Region *fixed = nullptr) {
+                        const Region *fixed = nullptr, const GradJob *grad = nullptr) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   HIP_TRY(hipStreamIsCapturing(stream, &cs));
   const bool captured = cs != hipStreamCaptureStatusNone;
   if (!captured && a->order_valid && a->order_stream != stream)
     HIP_TRY(hipStreamWaitEvent(stream, a->order_ev, 0));
   fmca_status st = enqueue_any_unordered(a, d_rhs, ld_rhs, nrhs, d_out, ld_out, precision,
-                                         natural, transpose, stream, groups, fixed);
+                                         natural, transpose, stream, groups, fixed, grad);
   if (st != FMCA_OK || captured) return st;
   HIP_TRY(hipEventRecord(a->order_ev, stream));
   a->order_stream = stream;
@@ -624,7 +632,8 @@ fmca_status enqueue_any(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs, i
 fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t ld_rhs,
                                   int64_t nrhs, double *d_out, int64_t ld_out, int precision,
                                   bool natural, bool transpose, hipStream_t stream,
-                                  const VariantGroups &groups, const Region *fixed) {
+                                  const VariantGroups &groups, const Region *fixed,
+                                  const GradJob *grad) {
   if (transpose && !a->shared && !a->have_pat_t) {
     compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
     HIP_TRY(a->d_pat_t.upload(a->pat_t));
@@ -633,11 +642,11 @@ fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t
   if (precision == FMCA_PRECISION_FP64)
     return enqueue<double>(a, a->ws64, a->d_rows_64, a->shared ? a->d_rows_64 : a->d_cols_64,
                            d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
-                           groups, fixed);
-  if (precision == FMCA_PRECISION_FP32)
+                           groups, fixed, false, grad);
+  if (precision == FMCA_PRECISION_FP32 || precision == FMCA_PRECISION_FP32_BF16)
     return enqueue<float>(a, a->ws32, a->d_rows_32, a->shared ? a->d_rows_32 : a->d_cols_32,
                           d_rhs, ld_rhs, nrhs, d_out, ld_out, natural, transpose, stream,
-                          groups, fixed);
+                          groups, fixed, precision == FMCA_PRECISION_FP32_BF16, grad);
   return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
 }
 
@@ -1051,7 +1060,8 @@ fmca_status fmca_h2_action_variants_device(fmca_h2_actor *actor, const int *vari
   const int64_t rows_out = transpose ? actor->n : actor->m;
   if (ld_rhs < rows_in || ld_out < rows_out)
     return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
-  if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+  if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32 &&
+      precision != FMCA_PRECISION_FP32_BF16)
     return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
   std::lock_guard<std::mutex> lock(actor->mtx);
   fmca_status st = ensure_device(actor);
@@ -1073,6 +1083,69 @@ fmca_status fmca_h2_action_variants_device(fmca_h2_actor *actor, const int *vari
     g0 = g1;
   }
   return FMCA_OK;
+}
+
+fmca_status fmca_h2_gradient_device(fmca_h2_actor *a, const double *d_alpha, int64_t ld_alpha,
+                                   const double *d_g, int64_t ld_g, int64_t nrhs, int precision,
+                                   double *d_dalpha, int64_t ld_dalpha, double *d_dpoints,
+                                   int64_t ld_points, double *d_dl, void *stream_ptr) {
+  if (!a || !d_alpha || !d_g || !d_dalpha || !d_dpoints || !d_dl)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (nrhs < 1 || nrhs > 16)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the gradient form takes 1 to 16 right-hand sides");
+  if (!a->shared)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the gradient form needs one point set (targets = sources)");
+  if (a->ip.scheme == IS_TOTAL_DEGREE)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the gradient form needs the tensor-product interpolator");
+  if (ld_alpha < a->n || ld_g < a->n || ld_dalpha < a->n || ld_points < a->n)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (precision == FMCA_PRECISION_FP32_BF16) precision = FMCA_PRECISION_FP32;
+  if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+  const int r = (int)nrhs, R = r <= 8 ? 8 : 16, ncol = 2 * R + 16;
+  VariantGroups g;
+  g.n = ncol / 16;
+  for (int i = 0; i < g.n; ++i) g.v[i] = 16 * i < 2 * R ? KV_VALUE : KV_DL;
+  const int vm = (1 << KV_VALUE) | (1 << KV_DL);
+  const bool coupling_ok =
+      precision == FMCA_PRECISION_FP64
+          ? mfma_supported<double>(false, a->dim, a->ip.P1, a->ip.q, ncol, a->kt, vm, 0)
+          : mfma_supported<float>(false, a->dim, a->ip.P1, a->ip.q, ncol, a->kt, vm, 0);
+  if (!coupling_ok)
+    return fail(FMCA_ERR_NOT_SUPPORTED,
+                "the coupling of the gradient form does not fit the matrix cores");
+  std::lock_guard<std::mutex> lock(a->mtx);
+  fmca_status st = ensure_device(a);
+  if (st != FMCA_OK) return st;
+  hipStream_t sm = (hipStream_t)stream_ptr;
+  const int64_t n = a->n;
+  HIP_TRY(a->grad_x.alloc((size_t)n * ncol * sizeof(double)));
+  const int nparts = (int)a->pat.near_order.size();
+  HIP_TRY(a->grad_part.alloc((size_t)(nparts + grad_far_dot_blocks()) * sizeof(double)));
+  double *X = a->grad_x.as<double>();
+  // rhs [alpha | G | alpha] with zero padding, natural order
+  HIP_TRY(launch_grad_zero(X, n * ncol, sm));
+  const size_t w = (size_t)n * sizeof(double);
+  HIP_TRY(hipMemcpy2DAsync(X, w, d_alpha, ld_alpha * sizeof(double), w, r,
+                           hipMemcpyDeviceToDevice, sm));
+  HIP_TRY(hipMemcpy2DAsync(X + (size_t)R * n, w, d_g, ld_g * sizeof(double), w, r,
+                           hipMemcpyDeviceToDevice, sm));
+  HIP_TRY(hipMemcpy2DAsync(X + (size_t)2 * R * n, w, d_alpha, ld_alpha * sizeof(double), w, r,
+                           hipMemcpyDeviceToDevice, sm));
+  for (int k = 0; k < r; ++k) HIP_TRY(launch_grad_zero(d_dalpha + k * ld_dalpha, n, sm));
+  for (int d = 0; d < a->dim; ++d) HIP_TRY(launch_grad_zero(d_dpoints + d * ld_points, n, sm));
+  GradJob job;
+  job.r = r;
+  job.R = R;
+  job.ncol = ncol;
+  job.dalpha = d_dalpha;
+  job.ld_dalpha = ld_dalpha;
+  job.dpoints = d_dpoints;
+  job.ld_points = ld_points;
+  job.dl_part = a->grad_part.as<double>();
+  job.nnear_parts = nparts;
+  job.dl_out = d_dl;
+  return enqueue_any(a, X, n, ncol, nullptr, n, precision, true, false, sm, g, nullptr, &job);
 }
 
 fmca_status fmca_h2_indices(const fmca_h2_actor *actor, int which, uint32_t *out) {
@@ -1199,7 +1272,8 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
   if (a->m != a->n) return fail(FMCA_ERR_INVALID_ARGUMENT, "CG needs a square kernel matrix");
   if (ld_b < a->n || ld_x < a->n) return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
   if (nrhs > 65535) return fail(FMCA_ERR_NOT_SUPPORTED, "more than 65535 right-hand sides");
-  if (opts->precision != FMCA_PRECISION_FP64 && opts->precision != FMCA_PRECISION_FP32)
+  if (opts->precision != FMCA_PRECISION_FP64 && opts->precision != FMCA_PRECISION_FP32 &&
+      opts->precision != FMCA_PRECISION_FP32_BF16)
     return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
   if (!(opts->tol > 0) || !(opts->ridge >= 0))
     return fail(FMCA_ERR_INVALID_ARGUMENT, "tol must be > 0 and ridge >= 0");
@@ -1387,7 +1461,7 @@ fmca_status fmca_h2_set_profiling(fmca_h2_actor *actor, int enable) {
 fmca_status fmca_h2_phase_times_precision(fmca_h2_actor *actor, int precision, double ms[6],
                                           int64_t count[6], int reset) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
-  if (precision != -1 && precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+  if (precision < -1 || precision > FMCA_PRECISION_FP32_BF16)
     return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
   std::lock_guard<std::mutex> lock(actor->mtx);
   for (auto &pp : actor->pending) {
@@ -1405,7 +1479,7 @@ fmca_status fmca_h2_phase_times_precision(fmca_h2_actor *actor, int precision, d
   for (int k = 0; k < PH_COUNT; ++k) {
     double t = 0;
     int64_t c = 0;
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < 3; ++p) {
       if (precision != -1 && p != precision) continue;
       t += actor->phase_ms[p][k];
       c += actor->phase_count[p][k];

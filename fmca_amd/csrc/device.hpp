@@ -94,6 +94,7 @@ struct KernelParams {
   T c2 = 1;     // c * c
   int vmask = 1;
   int tv[kMaxVariantGroups] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int split_bf16 = 0;  // FP32 form: kernel-value contractions on BF16 matrix cores
 };
 
 enum KernelVariant : int { KV_VALUE = 0, KV_DL = 1, KV_PHI = 2 };
@@ -166,7 +167,36 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
                          hipStream_t stream, const PhaseEvents *events = nullptr,
-                         const StoredBlocks<T> *stored = nullptr);
+                         const StoredBlocks<T> *stored = nullptr,
+                         const struct GradJob *grad = nullptr);
+
+// Gradient of <G, K~ alpha> for one shared tree (kernels_grad.hip): the
+// action on [alpha | G | alpha] (columns [0, R), [R, 2R), [2R, 2R + 16), the
+// last group with dk/dl) stops after the downward pass, and these kernels
+// take its moments and far sums.  Outputs are FP64, natural order, added
+// atomically into arrays the caller has zeroed.
+struct GradJob {
+  int r = 0;            // right-hand sides (columns of alpha and G), r <= R
+  int R = 8;            // 8 or 16
+  int ncol = 0;         // 2 R + 16
+  double *dalpha = nullptr;   // n x r, column stride ld_dalpha
+  int64_t ld_dalpha = 0;
+  double *dpoints = nullptr;  // dim x n, row stride ld_points
+  int64_t ld_points = 0;
+  double *dl_part = nullptr;  // nnear_parts + grad_far_dot_blocks() partial sums
+  int nnear_parts = 0;        // = the leaf-output grid size
+  double *dl_out = nullptr;   // one FP64 value
+};
+int grad_far_dot_blocks();
+template <typename T>
+hipError_t launch_grad_far_dot(const GradJob &job, int64_t nmom, const T *mom, const T *tlhs,
+                               hipStream_t stream);
+template <typename T>
+hipError_t launch_grad_leaves(const DevTreeView<T> &tree, const DevPatternView &pat,
+                              const InterpParams<T> &ip, const KernelParams<T> &kp,
+                              const GradJob &job, const T *xs, const T *tlhs,
+                              hipStream_t stream);
+hipError_t launch_grad_zero(double *p, int64_t n, hipStream_t stream);
 
 // Writes every block of the assembled matrix (assembled.hip).  far_tgt: far
 // block -> target cluster; near_leaf: near list entry -> leaf index.
@@ -195,6 +225,15 @@ constexpr int kMfmaMinRhs = 8;
 // 64-column passes, also in FP64).
 template <typename T>
 bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask, int td);
+
+// BF16 three-term forms of the FP32 leaf output / coupling (kernels_bf16.hip):
+// kernel values only, >= kMfmaMinRhs rhs, where the LDS staging fits.
+bool bf16_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask);
+hipError_t launch_bf16(bool output, const DevTreeView<float> &rows,
+                       const DevTreeView<float> &cols, const DevPatternView &pat,
+                       const InterpParams<float> &ip, const KernelParams<float> &kp, int nrhs,
+                       const float *xs, const float *mom, float *tlhs, double *out, int64_t ldo,
+                       bool natural, hipStream_t stream);
 
 template <typename T>
 hipError_t launch_mfma(bool output, const DevTreeView<T> &rows, const DevTreeView<T> &cols,

@@ -294,6 +294,43 @@ __device__ __forceinline__ void lagrange_1d(T u, const InterpParams<T> &ip, T *o
   }
 }
 
+// Derivatives l_a'(u), a < P1, of the barycentric Lagrange factors (the
+// values stay lagrange_1d's).  Away from the nodes l_a' = l_a (S2 / S1 -
+// 1 / (u - xi_a)) with S1 = sum_j w_j / (u - xi_j), S2 = sum_j w_j / (u - xi_j)^2
+// (the derivative of the barycentric quotient).  Within eps of node m, where
+// lagrange_1d takes the factors as delta_am, they are the entries of the
+// barycentric differentiation matrix: l_a'(xi_m) = (w_a / w_m) / (xi_m - xi_a)
+// for a != m and l_m'(xi_m) = -sum_{a != m} l_a'(xi_m).
+template <typename T>
+__device__ __forceinline__ void lagrange_1d_deriv(T u, const InterpParams<T> &ip, T *der,
+                                                  int stride) {
+  int hit = -1;
+  T s1 = 0, s2 = 0;
+  for (int j = 0; j < ip.P1; ++j) {
+    const T d = u - ip.xi[j];
+    if (fabs(d) <= Math<T>::eps()) hit = j;
+    const T wd = ip.w[j] / d;
+    s1 += wd;
+    s2 += wd / d;
+  }
+  if (hit < 0) {
+    const T rho = s2 / s1;
+    for (int a = 0; a < ip.P1; ++a) {
+      const T d = u - ip.xi[a];
+      der[a * stride] = ip.w[a] / d / s1 * (rho - T(1) / d);
+    }
+  } else {
+    T sum = 0;
+    for (int a = 0; a < ip.P1; ++a) {
+      if (a == hit) continue;
+      const T da = (ip.w[a] / ip.w[hit]) / (ip.xi[hit] - ip.xi[a]);
+      der[a * stride] = da;
+      sum += da;
+    }
+    der[hit * stride] = -sum;
+  }
+}
+
 // L2-normalised shifted Legendre polynomials sqrt(2a+1) P_a(2u-1), a < P1,
 // by the three-term recurrence (evalPolynomials.h:19-40), written with a
 // stride.

@@ -751,6 +751,11 @@ hipError_t dispatch_nr(bool output, const DevTreeView<T> &rows, const DevTreeVie
                        const DevPatternView &pat, const InterpParams<T> &ip,
                        const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
                        int64_t ldo, bool natural, hipStream_t stream) {
+  if constexpr (sizeof(T) == 4) {
+    if (kp.split_bf16 && bf16_supported(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask))
+      return launch_bf16(output, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.mom, buf.tlhs, out,
+                         ldo, natural, stream);
+  }
   if (mfma_supported<T>(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask, ip.td))
     return launch_mfma<T>(output, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.mom, buf.tlhs, out,
                           ldo, natural, stream);
@@ -776,7 +781,7 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                     const InterpParams<T> &ip, const KernelParams<T> &kp, const double *rhs,
                     int64_t ld_rhs, int nrhs, double *out, int64_t ld_out, bool natural,
                     ActionBuffers<T> buf, hipStream_t stream, const PhaseEvents *ev,
-                    const StoredBlocks<T> *stored) {
+                    const StoredBlocks<T> *stored, const GradJob *grad) {
   const int q = ip.q, P1 = ip.P1;
   hipError_t err;
   auto mark = [&](int k) {
@@ -897,6 +902,12 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                                  stream)) != hipSuccess)
       return err;
   }
+  // gradient job: the far part of dL/dl from the coupling sums of the dk/dl group
+  if (grad) {
+    if ((err = launch_grad_far_dot<T>(*grad, (int64_t)cols.ncl * q, buf.mom, buf.tlhs,
+                                      stream)) != hipSuccess)
+      return err;
+  }
   if (ip.td) basis_change(rows.ncl, 0, buf.tlhs);
   // 5. downward transfers, top level first
   mark(PH_DOWNWARD_TRANSFER);
@@ -932,6 +943,14 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 6. leaf output: far field through V^T plus the near field
   mark(PH_LEAF_OUTPUT);
+  if (grad) {
+    KernelParams<T> kg = kp;  // value, dk/dl and phi of every pair
+    kg.kt = kp.kt % KT_COUNT;
+    kg.vmask = (1 << KV_VALUE) | (1 << KV_DL) | (1 << KV_PHI);
+    err = launch_grad_leaves<T>(rows, pat, ip, kg, *grad, buf.xs, buf.tlhs, stream);
+    mark(PH_COUNT);
+    return err;
+  }
   if (stored) {
     err = launch_stored<T>(true, rows, cols, pat, *stored, q, nrhs, buf.xs, buf.mom, buf.tlhs,
                            out, ld_out, natural, stream);
@@ -982,17 +1001,17 @@ hipError_t launch_action(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const KernelParams<T> &kp, const double *rhs, int64_t ld_rhs, int nrhs,
                          double *out, int64_t ld_out, bool natural_order, ActionBuffers<T> buf,
                          hipStream_t stream, const PhaseEvents *events,
-                         const StoredBlocks<T> *stored) {
+                         const StoredBlocks<T> *stored, const GradJob *grad) {
   switch (rows.dim) {
     case 1:
       return action_d<T, 1>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream, events, stored);
+                            ld_out, natural_order, buf, stream, events, stored, grad);
     case 2:
       return action_d<T, 2>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream, events, stored);
+                            ld_out, natural_order, buf, stream, events, stored, grad);
     case 3:
       return action_d<T, 3>(rows, cols, lrows, lcols, pat, ip, kp, rhs, ld_rhs, nrhs, out,
-                            ld_out, natural_order, buf, stream, events, stored);
+                            ld_out, natural_order, buf, stream, events, stored, grad);
   }
   return hipErrorInvalidValue;
 }
@@ -1003,13 +1022,13 @@ template hipError_t launch_action<double>(const DevTreeView<double> &, const Dev
                                           const KernelParams<double> &, const double *, int64_t,
                                           int, double *, int64_t, bool, ActionBuffers<double>,
                                           hipStream_t, const PhaseEvents *,
-                                          const StoredBlocks<double> *);
+                                          const StoredBlocks<double> *, const GradJob *);
 template hipError_t launch_action<float>(const DevTreeView<float> &, const DevTreeView<float> &,
                                          const HostLevels &, const HostLevels &,
                                          const DevPatternView &, const InterpParams<float> &,
                                          const KernelParams<float> &, const double *, int64_t, int,
                                          double *, int64_t, bool, ActionBuffers<float>,
                                          hipStream_t, const PhaseEvents *,
-                                         const StoredBlocks<float> *);
+                                         const StoredBlocks<float> *, const GradJob *);
 
 }  // namespace fmca_amd

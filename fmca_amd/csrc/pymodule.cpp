@@ -133,6 +133,7 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
   m.doc() = "MI355X-native H2-matrix kernel action (C ABI bindings)";
   m.attr("FP64") = (int)FMCA_PRECISION_FP64;
   m.attr("FP32") = (int)FMCA_PRECISION_FP32;
+  m.attr("FP32_BF16") = (int)FMCA_PRECISION_FP32_BF16;
   m.attr("ROWS") = (int)FMCA_ROWS;
   m.attr("COLS") = (int)FMCA_COLS;
   m.attr("KERNEL_VALUE") = (int)FMCA_KERNEL_VALUE;
@@ -275,6 +276,18 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
              py::array_t<double> out({(py::ssize_t)6, (py::ssize_t)1});
              check(fmca_h2_statistics(a.h, out.mutable_data()));
              return out;
+           })
+      .def("gradient_device",
+           [](Actor &a, uintptr_t alpha, int64_t ld_alpha, uintptr_t g, int64_t ld_g,
+              int64_t nrhs, int precision, uintptr_t dalpha, int64_t ld_dalpha,
+              uintptr_t dpoints, int64_t ld_points, uintptr_t dl, uintptr_t stream) {
+             py::gil_scoped_release nogil;
+             check(fmca_h2_gradient_device(
+                 a.h, reinterpret_cast<const double *>(alpha), ld_alpha,
+                 reinterpret_cast<const double *>(g), ld_g, nrhs, precision,
+                 reinterpret_cast<double *>(dalpha), ld_dalpha,
+                 reinterpret_cast<double *>(dpoints), ld_points,
+                 reinterpret_cast<double *>(dl), reinterpret_cast<void *>(stream)));
            })
       .def("set_profiling", [](Actor &a, bool on) { check(fmca_h2_set_profiling(a.h, on ? 1 : 0)); })
       .def(

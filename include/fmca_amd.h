@@ -36,7 +36,14 @@ typedef enum {
 /* Arithmetic of the coupling and near-field kernels. */
 enum {
   FMCA_PRECISION_FP64 = 0, /* plain form: every phase in FP64 */
-  FMCA_PRECISION_FP32 = 1  /* precision form: FP32 kernel entries and contractions */
+  FMCA_PRECISION_FP32 = 1, /* precision form: FP32 kernel entries and contractions */
+  /* precision form with the contractions of the kernel entries on BF16 matrix
+   * cores: FP32 entries and right operands, each split into two BF16 terms
+   * (hi = bf16(v), lo = bf16(v - hi)), products hi.hi + lo.hi + hi.lo with
+   * FP32 accumulation -- the error class of FP32 entries at 3/16 of the FP32
+   * matrix-core cycles.  Used where the FP32 form would use the matrix cores
+   * for kernel values (>= 8 rhs); every other phase is the FP32 form's. */
+  FMCA_PRECISION_FP32_BF16 = 2
 };
 
 enum { FMCA_ROWS = 0, FMCA_COLS = 1 }; /* row tree = Peval (targets), column tree = P */
@@ -275,6 +282,29 @@ fmca_status fmca_h2_action_variants_device(fmca_h2_actor *actor, const int *vari
                                            int ngroups, int transpose, const double *d_rhs,
                                            int64_t ld_rhs, int64_t nrhs, double *d_out,
                                            int64_t ld_out, int precision, void *stream);
+
+/* Gradient of L = <G, K~(P, P; l) alpha> for one point set (the backward of
+ * the differentiable matvec y = K~ alpha with dL/dy = G; no FMCA
+ * counterpart).  K~ is the H2 matrix of the actor (targets = sources, the
+ * tensor-product interpolator), with the tree and boxes held fixed:
+ *   d_dalpha  (n x nrhs, ld_dalpha) = K~^T G = K~ G
+ *   d_dpoints (dim x n, row stride ld_points) = dL/dP: near blocks through
+ *             phi(r) = k'(r)/r and the pair weights G_p.alpha_j + alpha_p.G_j,
+ *             far blocks through the gradient of the interpolation basis
+ *             dotted with the far-field sums of alpha and G (the derivative of
+ *             the interpolated far field)
+ *   d_dl      (one device double) = <G, dK~/dl alpha>
+ * alpha, G: n x nrhs device FP64 (ld_alpha, ld_g), natural order, nrhs <= 16.
+ * Enqueued on `stream` (one action on [alpha | G | alpha] with the last
+ * group under dk/dl, then the gradient kernels); outputs are overwritten.
+ * FMCA_ERR_NOT_SUPPORTED for two point sets, the total-degree interpolator,
+ * more than 16 rhs, or a coupling that does not fit the matrix cores -- the
+ * derivative-kernel actions above cover those. */
+fmca_status fmca_h2_gradient_device(fmca_h2_actor *actor, const double *d_alpha,
+                                   int64_t ld_alpha, const double *d_g, int64_t ld_g,
+                                   int64_t nrhs, int precision, double *d_dalpha,
+                                   int64_t ld_dalpha, double *d_dpoints, int64_t ld_points,
+                                   double *d_dl, void *stream);
 
 /* Permutation of a tree (cluster order position -> point index), as
  * pyH2Matrix::rindices / cindices (py/FMCA.cpp:224-229) and

@@ -159,3 +159,38 @@ def test_action_replays_in_a_hip_graph(precision):
         ref = H.action_device(x, precision=prec)
         torch.cuda.synchronize()
         assert torch.equal(y, ref) or (torch.linalg.norm(y - ref) / torch.linalg.norm(ref)).item() <= 1e-15
+
+
+BF16_TOL = 2e-5  # FP32 entries (~4e-6 here) plus the dropped 2^-16 terms of the split
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=[c[0] for c in FP32_CASES])
+def test_fp32_bf16_action_close_to_fp64(case):
+    """The FP32 form with BF16 three-term matrix-core contractions (and its
+    FP32 fallbacks below 8 rhs) against the FP64 action."""
+    import fmca_amd
+    H, A, x, _, _ = _setup(case)
+    y64 = H.action(x)
+    yb = H.action(x, precision=fmca_amd.FP32_BF16)
+    y32 = H.action(x, precision=fmca_amd.FP32)
+    err, err32 = rel_err(yb, y64), rel_err(y32, y64)
+    print(f"  fp32-bf16 vs fp64 rel err {err:.3e} (fp32 {err32:.3e})", flush=True)
+    assert err <= BF16_TOL
+
+
+@pytest.mark.parametrize("dim,kernel,nrhs", [(3, "GAUSSIAN", 32), (2, "MATERN32", 16),
+                                             (3, "EXPONENTIAL", 64)])
+def test_fp32_bf16_on_clustered_points_with_split_targets(dim, kernel, nrhs):
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print(f"[case] bf16 clustered d={dim} {kernel} nrhs={nrhs}", flush=True)
+    n = 30000
+    P = W.gaussian_mixture(n, dim, 16, 0.02, 43)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel(kernel, 0.1), P, P, 3, 0.8)
+    x = torch.from_numpy(np.asfortranarray(np.random.default_rng(44).standard_normal((n, nrhs)))).cuda()
+    y64 = H.action_device(x, precision=fmca_amd.FP64).cpu().numpy()
+    yb = H.action_device(x, precision=fmca_amd.FP32_BF16).cpu().numpy()
+    err = rel_err(yb, y64)
+    print(f"  fp32-bf16 vs fp64 rel err {err:.3e}", flush=True)
+    assert err <= BF16_TOL

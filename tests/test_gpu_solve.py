@@ -101,3 +101,23 @@ def test_cg_reports_indefinite_system():
         x, info = H.cg(b, ridge=0.0, tol=1e-8, precision=prec, max_iter=500)
         assert info["indefinite"] and not info["converged"], info
         assert info["iterations"] < 500
+
+
+def test_cg_with_bf16_matvecs_reaches_fp64_solution():
+    """CG with reliable updates whose recurrence runs on the BF16 three-term
+    matvecs reaches the FP64 residual tolerance and the FP64 CG solution."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print("[case] cg fp32-bf16 n=20000 32 rhs", flush=True)
+    n, nrhs = 20000, 32
+    P = W.uniform(n, 3, 3)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 3, 0.8)
+    b = torch.from_numpy(W.normal_rhs(n, nrhs, 4)).cuda()
+    x64, i64 = H.cg(b, x0=torch.zeros_like(b), ridge=20.0, tol=1e-10, precision=fmca_amd.FP64)
+    xb, ib = H.cg(b, x0=torch.zeros_like(b), ridge=20.0, tol=1e-10, precision=fmca_amd.FP32_BF16)
+    print(f"  iterations fp64 {i64['iterations']} bf16 {ib['iterations']} "
+          f"(fp64 matvecs {ib['matvecs_fp64']}), residual {ib['rel_residual']:.2e}", flush=True)
+    assert ib["converged"] and ib["rel_residual"] <= 1e-10 and ib["matvecs_fp32"] > 0
+    assert _resid(H, xb, b, 20.0) <= 1e-10
+    assert (torch.linalg.norm(xb - x64) / torch.linalg.norm(x64)).item() <= 1e-8
