@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--n", type=int, default=1_000_000)
     p.add_argument("--nrhs", type=int, default=32)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-solve", action="store_true",
+                   help="time the single matvecs only and print them (for profiler runs)")
     p.add_argument("--no-side", action="store_true",
                    help="skip the differentiation side results (n = 2e5 2-D and n = 8e6 3-D)")
     p.add_argument("--ridge", type=float, default=100.0)
@@ -213,12 +215,17 @@ def side_small_dense(args):
 def side_clustered_cg(args):
     """n = 1e6 points from a mixture of 16 isotropic Gaussians (sigma 0.02,
     centres uniform in [0,1]^3), GAUSSIAN l = 0.1, deg 4, eta 0.8, 64 rhs
-    y = sin(2 pi x1) cos(2 pi x2) + 0.01 N(0,1): matvec times per phase, FP32
-    vs FP64 difference, and CG on (K + 1e-4 I) a = y (bounded iterations)."""
+    y = sin(2 pi x1) cos(2 pi x2) + 0.01 N(0,1): matvec times per phase in the
+    three precisions, their differences from FP64, CG on (K + 1e-4 I) a = y
+    (FP64, to the first p.(K~ + lambda I)p <= 0: the H2 matrix plus this ridge
+    is indefinite) and CG to 1e-8 at a ridge where it is definite
+    (--ridge, default 10 here): FP32 and FP32-BF16 matvecs with FP64
+    reliable updates against plain FP64 CG."""
     import torch
     import fmca_amd
     from fmca_amd import workloads as W
     n, nrhs = 1_000_000, 64
+    ridge = args.ridge if args.ridge != 100.0 else 10.0
     P = W.gaussian_mixture(n, 3, 16, 0.02, 0)
     t0 = time.perf_counter()
     H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("GAUSSIAN", 0.1), P, P, 4, 0.8)
@@ -228,20 +235,38 @@ def side_clustered_cg(args):
     res = {"workload": "clustered-cg: n=1e6 16-Gaussian mixture (sigma 0.02), GAUSSIAN l=0.1, "
                        "deg 4, eta 0.8, 64 rhs", "plan_s": plan, "info": H.info()}
     ys = {}
-    for name, prec in (("fp32", fmca_amd.FP32), ("fp64", fmca_amd.FP64)):
+    for name, prec in (("fp32", fmca_amd.FP32), ("fp32_bf16", fmca_amd.FP32_BF16),
+                       ("fp64", fmca_amd.FP64)):
         t, ph = timed_steps(H, y, out, prec, 2, 1, True)
         ys[name] = out.clone()
         res[name] = {"ms": t * 1e3, "mpts": n * nrhs / t / 1e6,
                      "phases_ms": {k: v[0] / max(v[1], 1) for k, v in ph.items()}}
-    res["fp32_vs_fp64_rel_err"] = float(torch.linalg.norm(ys["fp32"] - ys["fp64"])
-                                        / torch.linalg.norm(ys["fp64"]))
+    for name in ("fp32", "fp32_bf16"):
+        res[name]["vs_fp64_rel_err"] = float(torch.linalg.norm(ys[name] - ys["fp64"])
+                                             / torch.linalg.norm(ys["fp64"]))
     del ys, out
     try:
-        a, info = H.cg(y, x0=torch.zeros_like(y), ridge=1e-4, tol=1e-8,
-                       precision=fmca_amd.FP32, max_iter=args.steps * 5)
-    except RuntimeError as e:
+        _, info = H.cg(y, x0=torch.zeros_like(y), ridge=1e-4, tol=1e-8,
+                       precision=fmca_amd.FP64, max_iter=500)
+    except (RuntimeError, ValueError) as e:
         info = {"error": str(e)}
-    res["cg_fp32_refined_ridge_1e-4"] = info
+    res["cg_fp64_ridge_1e-4"] = info
+    sol = {}
+    for name, prec in (("fp32_bf16", fmca_amd.FP32_BF16), ("fp64", fmca_amd.FP64)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            a, info = H.cg(y, x0=torch.zeros_like(y), ridge=ridge, tol=1e-8, precision=prec,
+                           max_iter=3000)
+            sol[name] = a
+        except (RuntimeError, ValueError) as e:
+            info = {"error": str(e)}
+        torch.cuda.synchronize()
+        info["wall_s"] = time.perf_counter() - t0
+        res[f"cg_{name}_ridge_{ridge:g}"] = info
+    if len(sol) == 2:
+        res["solution_rel_diff"] = float(torch.linalg.norm(sol["fp32_bf16"] - sol["fp64"])
+                                         / torch.linalg.norm(sol["fp64"]))
     return res
 
 
@@ -560,6 +585,18 @@ def main():
     H.action(xh, precision=fmca_amd.FP32)
     t_host = time.perf_counter() - t0
 
+    if args.no_solve:
+        if args.calibrate_copy_mib > 0:
+            nel = args.calibrate_copy_mib * (1 << 20) // 8
+            src = torch.ones(nel, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            torch.neg(src, out=dst)
+            torch.cuda.synchronize()
+            del src, dst
+        print(json.dumps({"workload": "single matvecs (profiler run)", "n": n, "nrhs": nrhs,
+                          "fp32_ms": t32 * 1e3, "fp32_bf16_ms": tbf * 1e3, "fp64_ms": t64 * 1e3,
+                          "fp32_rhs1_ms": t32_1 * 1e3, "fp64_rhs1_ms": t64_1 * 1e3}), flush=True)
+        return
     # the headline: refined solves (timed steps), then plain FP64 CG once
     t_ref, a_ref, infos, ph_solve = refined_solves(H, x, args.ridge, args.steps, args.warmup,
                                                    solve_prec)

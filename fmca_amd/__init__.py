@@ -148,6 +148,16 @@ class H2kernelActor:
         self.n = self._a.n
         self.dim = self._a.dim
 
+    def project(self, M, precision: int = FP64):
+        """H2MatrixProjector(h2mat).project(M) (H2MatrixProjector.h:21-75):
+        the dense m x n matrix M (natural order) projected onto this plan's H2
+        format -- M's near blocks, and per far block the Frobenius-orthogonal
+        projection onto the cluster bases.  Later forward actions of that
+        precision apply the projected matrix (release_assembled() returns to
+        the kernel).  Returns FMCA's dual-basis check on the row and column
+        trees (FMCA asserts each < 5e-10)."""
+        return self._a.project(np.asfortranarray(M, dtype=np.float64), int(precision))
+
     def assemble(self, precision: int = FP64) -> int:
         """H2Matrix::computeH2Matrix: store every coupling matrix, dense block
         and leaf basis in device memory (returns the bytes held); the plain

@@ -1,18 +1,31 @@
 """Differentiable H2 kernel matvec: y = K(targets, points; l) alpha.
 
-The backward pass returns
+One point set (targets = points, one tree, tensor-product interpolation, at
+most 16 columns) with point gradients requested: the backward is
+fmca_h2_gradient_device -- one action on
+[alpha | G | alpha] (the last group with dk/dl) that stops after the downward
+pass, then gradient kernels that give
+  d alpha   = K~ G
+  d l       = <G, dK~/dl alpha>
+  d points  = the derivative of the H2 forward itself: near blocks through
+              phi(r) = k'(r)/r and the pair weights G_p.alpha_j + alpha_p.G_j,
+              far blocks through the gradient of the interpolation basis at
+              the point dotted with the far-field sums of alpha and G
+with the tree and boxes held fixed.
+
+Otherwise (no point gradients, two point sets, total degree, more columns)
+the backward applies
+H2 actions with derivative kernels, which the H2 operator's linearity in the
+kernel function allows:
   d alpha   = K^T G                                (transposed H2 action)
   d l       = <G, (dK/dl) alpha>                    (H2 action with dk/dl)
   d points  = sum_r alpha_r (s (Phi^T g_r) - Phi^T (g_r * t))   per coordinate
   d targets = sum_r g_r   (t (Phi a_r)   - Phi (a_r * s))
-with Phi the H2 action of phi(r) = k'(r)/r on the same tree, bases and blocks
-(for targets = points both terms add up and Phi^T = Phi).  The H2 operator is
-linear in the kernel function, so each of these is an action with a
-derivative kernel.  They run fused: one action whose 16-column groups carry
-different kernel variants (fmca_h2_action_variants_device), so that every
-kernel entry is evaluated once for K, dK/dl and Phi.  No moments are stored
-between forward and backward -- the backward action recomputes them from the
-points and its right-hand sides.
+with Phi the H2 action of phi on the same tree, bases and blocks, fused into
+one action whose 16-column groups carry different kernel variants
+(fmca_h2_action_variants_device).  In both forms no moments are stored
+between forward and backward: the backward recomputes them from the points
+and its right-hand sides.
 
 FMCA has no differentiation code; these forms are checked against centred
 finite differences of the H2 forward and against dense FP64 autograd.
@@ -138,6 +151,34 @@ def _variants(actor: H2kernelActor, blocks, transpose: bool, precision: int):
     return [(p[0] if len(p) == 1 else torch.cat(p, dim=0)).t() for p in parts]
 
 
+def _colmajor(X: torch.Tensor) -> torch.Tensor:
+    X = X.to(torch.float64)
+    if X.shape[1] == 1:
+        return X.contiguous()
+    return X if (X.stride(0) == 1 and X.stride(1) == X.shape[0]) else X.t().contiguous().t()
+
+
+def _gradient(actor: H2kernelActor, A: torch.Tensor, G: torch.Tensor, precision: int):
+    """(K~ G, dL/dP, dL/dl) of L = <G, K~ A> for one shared point set, from the
+    gradient kernels (fmca_h2_gradient_device); None where that form does not
+    apply (two point sets, total degree, > 16 columns, no matrix-core fit)."""
+    n, r = A.shape
+    A, G = _colmajor(A), _colmajor(G)
+    dev = A.device
+    dalpha = torch.empty((r, n), dtype=torch.float64, device=dev).t()
+    dpoints = torch.empty((actor.dim, n), dtype=torch.float64, device=dev)
+    dl = torch.empty(1, dtype=torch.float64, device=dev)
+    try:
+        actor._a.gradient_device(A.data_ptr(), n, G.data_ptr(), n, r, precision,
+                                 dalpha.data_ptr(), n, dpoints.data_ptr(), n, dl.data_ptr(),
+                                 torch.cuda.current_stream(dev).cuda_stream)
+    except ValueError as e:
+        if str(e).startswith("not supported"):
+            return None
+        raise
+    return dalpha, dpoints, dl
+
+
 class _H2Matvec(torch.autograd.Function):
     @staticmethod
     def forward(ctx, alpha, lengthscale, points, targets, spec):
@@ -169,6 +210,18 @@ class _H2Matvec(torch.autograd.Function):
         need_t = ctx.needs_input_grad[3] or (ctx.symmetric and need_s)
         dev = A.device
         r = A.shape[1]
+        if ctx.symmetric and need_s and r <= 16:
+            # one point set with point gradients: the gradient kernels give all
+            # three at once (without them the fused derivative actions below
+            # need fewer columns)
+            out = _gradient(actor, A, G, prec)
+            if out is not None:
+                dal, dpo, dl = out
+                d_alpha = (dal[:, 0] if ctx.vector else dal) if need_a else None
+                d_l = dl.reshape(lengthscale.shape).to(lengthscale.device,
+                                                       lengthscale.dtype) if need_l else None
+                d_points = dpo.to(points.device, points.dtype) if need_s else None
+                return d_alpha, d_l, d_points, None, None
         S = points.detach().to(dev, torch.float64) if (need_s or need_t) else None  # (d, n)
         T = targets.detach().to(dev, torch.float64) if (need_s or need_t) else None  # (d, m)
         d = S.shape[0] if S is not None else 0

@@ -37,7 +37,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"
 
 SOURCES_HIP = ["kernels.hip", "kernels_mfma.hip", "kernels_bf16.hip", "kernels_grad.hip",
                "blas1.hip", "tree_device.hip", "assembled.hip", "pattern_device.hip"]
-SOURCES_CPP = ["plan.cpp", "actor.cpp"]
+SOURCES_CPP = ["plan.cpp", "actor.cpp", "projector.cpp"]
 
 
 def _run(cmd, verbose):

@@ -17,6 +17,7 @@
 #include "../../include/fmca_amd.h"
 #include "device.hpp"
 #include "plan.hpp"
+#include "projector.hpp"
 
 using namespace fmca_amd;
 
@@ -198,6 +199,7 @@ struct fmca_h2_actor {
       st_near_order;
   int st_prec = -1, st_Rq = 0, st_Kq = 0;
   int64_t st_bytes = 0;
+  std::vector<int64_t> st_near_off_host;  // element offsets of the stored near blocks
   void release_stored() {
     for (DevBuf *b : {&st_far, &st_near, &st_near_off, &st_v_off, &st_far_tgt, &st_near_leaf,
                       &st_far_order, &st_near_order})
@@ -577,6 +579,7 @@ fmca_status assemble(fmca_h2_actor *a, const DevTreeReals<T> &r_rows,
   HIP_TRY(a->st_far.alloc((size_t)nfar * Rq * Kq * sizeof(T)));
   HIP_TRY(a->st_near.alloc((size_t)off * sizeof(T)));
   HIP_TRY(a->st_near_off.upload(near_off));
+  a->st_near_off_host = near_off;
   HIP_TRY(a->st_v_off.upload(v_off));
   HIP_TRY(a->st_far_tgt.upload(far_tgt));
   HIP_TRY(a->st_near_leaf.upload(near_leaf));
@@ -1182,6 +1185,108 @@ fmca_status fmca_h2_info(const fmca_h2_actor *actor, double out[13]) {
   out[10] = actor->rows().depth;
   out[11] = actor->cols().depth;
   out[12] = actor->shared ? 1 : 0;
+  return FMCA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Position of element (i, j) of a stored R x K block in its matrix-core tile
+// layout (the inverse of the device's tile_pos; layout in device.hpp).
+template <typename T>
+int64_t tile_index(int i, int j, int K) {
+  constexpr int G = kStoredGroup<T>;
+  const int ng = K / (4 * G);
+  const int lane = (i & 15) + 16 * (j & 3);
+  const int ks = j >> 2;
+  const int64_t sg = (int64_t)(i >> 4) * ng + ks / G;
+  return (sg * 64 + lane) * G + ks % G;
+}
+
+// Writes the projected blocks over the assembled ones of precision T.
+template <typename T>
+fmca_status store_projected(fmca_h2_actor *a, const ProjectedMatrix &pm) {
+  const int q = a->ip.q, Rq = a->st_Rq, Kq = a->st_Kq;
+  const size_t nfar = a->pat.far_src.size();
+  std::vector<T> far((size_t)nfar * Rq * Kq, T(0));
+  parallel_ranges((int64_t)nfar, [&](int64_t lo, int64_t hi) {
+    for (int64_t b = lo; b < hi; ++b) {
+      T *dst = far.data() + (size_t)b * Rq * Kq;
+      const double *S = pm.far.data() + (size_t)b * q * q;
+      for (int i = 0; i < q; ++i)
+        for (int j = 0; j < q; ++j) dst[tile_index<T>(i, j, Kq)] = (T)S[(size_t)i * q + j];
+    }
+  });
+  if (nfar)
+    HIP_TRY(hipMemcpy(a->st_far.p, far.data(), far.size() * sizeof(T), hipMemcpyHostToDevice));
+  // near blocks: patch them into a host copy of the stored near region (which
+  // also holds the leaf bases V_t^T)
+  std::vector<T> near(a->st_near.bytes / sizeof(T));
+  if (!near.empty())
+    HIP_TRY(hipMemcpy(near.data(), a->st_near.p, near.size() * sizeof(T), hipMemcpyDeviceToHost));
+  const Tree &tr = a->rows(), &tc = a->cols();
+  const int nl = (int)tr.leaves.size();
+  parallel_ranges(nl, [&](int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; ++k) {
+      const int t = tr.leaves[k];
+      const int nt = (int)tr.size[t];
+      for (int e = a->pat.near_ptr[k]; e < a->pat.near_ptr[k + 1]; ++e) {
+        const int ns = (int)tc.size[a->pat.near_src[e]];
+        const int Kt = stored_kpad<T>(ns);
+        T *dst = near.data() + a->st_near_off_host[e];
+        const double *src = pm.near.data() + pm.near_off[e];
+        for (int j = 0; j < ns; ++j)
+          for (int i = 0; i < nt; ++i) dst[tile_index<T>(i, j, Kt)] = (T)src[(size_t)j * nt + i];
+      }
+    }
+  });
+  if (!near.empty())
+    HIP_TRY(hipMemcpy(a->st_near.p, near.data(), near.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fmca_status fmca_h2_project(fmca_h2_actor *a, const double *M, int64_t ld, int precision,
+                            double dual_check[2]) {
+  if (!a || !M) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (ld < a->m) return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
+  if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (a->ip.scheme != IS_TENSOR)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the projector takes the tensor-product interpolator");
+  if (a->ip.q > kStoredMaxRows / 2)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "projected blocks need q <= 128");
+  std::lock_guard<std::mutex> lock(a->mtx);
+  ProjectedMatrix pm;
+  std::string err;
+  if (!project_matrix(a->rows(), a->cols(), a->shared, a->ip, a->trans_rows,
+                      a->shared ? a->trans_rows : a->trans_cols, a->pts_rows,
+                      a->shared ? a->pts_rows : a->pts_cols, a->pat, M, ld, pm, &err))
+    return fail(FMCA_ERR_NOT_SUPPORTED, err);
+  if (dual_check) {
+    dual_check[0] = pm.dual_check_rows;
+    dual_check[1] = pm.dual_check_cols;
+  }
+  fmca_status st = ensure_device(a);
+  if (st != FMCA_OK) return st;
+  // the stored-block layout and the leaf bases, then the projected blocks over
+  // the generated ones
+  a->release_stored();
+  st = precision == FMCA_PRECISION_FP64
+           ? assemble<double>(a, a->d_rows_64, a->shared ? a->d_rows_64 : a->d_cols_64)
+           : assemble<float>(a, a->d_rows_32, a->shared ? a->d_rows_32 : a->d_cols_32);
+  if (st == FMCA_OK)
+    st = precision == FMCA_PRECISION_FP64 ? store_projected<double>(a, pm)
+                                          : store_projected<float>(a, pm);
+  if (st != FMCA_OK) {
+    a->release_stored();
+    return st;
+  }
+  a->st_prec = precision;
   return FMCA_OK;
 }
 

@@ -81,8 +81,8 @@ __device__ __forceinline__ T shfl_xor(T v, int m) {
 //   (K = 2R), whose accumulator layout (row crow(lk, v), column lr) is also
 //   the A-operand layout of the K-step v of the d-alpha product, source
 //   j = crow(lk, v);
-//   per pair: k, dk/dl, phi from one evaluation; phi W into the point sums,
-//   dk/dl W into the lengthscale sum;
+//   per pair: k, dk/dl, phi from one evaluation; phi W (x_p - x_j) into the
+//   point sums, dk/dl W into the lengthscale sum;
 //   d-alpha: A = k, B = G_j on the matrix cores.
 template <typename T, int D, int KT, int R>
 __global__ __launch_bounds__(256) void k_grad_near(
@@ -125,19 +125,18 @@ __global__ __launch_bounds__(256) void k_grad_near(
         }
       }
       acc_t da[kRT];
-      T s0[kRT], sd[kRT][D];
+      T sd[kRT][D];  // sum_j phi W (x_p - x_j) of this lane's pairs
 #pragma unroll
       for (int rt = 0; rt < kRT; ++rt) {
         da[rt] = acc_t{0, 0, 0, 0};
-        s0[rt] = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) sd[rt][d] = 0;
       }
       int nb = n0, j0 = 0;
       while (nb < n1) {
         const int run = pat.near_run[nb];
-        const int s = pat.near_src[nb], sl = pat.near_src[nb + run - 1];
-        const int sb = tree.begin[s], ss = tree.begin[sl] + tree.size[sl] - sb;
+        const int s0 = pat.near_src[nb], s1 = pat.near_src[nb + run - 1];
+        const int sb = tree.begin[s0], ss = tree.begin[s1] + tree.size[s1] - sb;
         const int nj = min(kGChunk, ss - j0);
         __syncthreads();
         for (int e = threadIdx.x; e < D * kGChunk; e += blockDim.x) {
@@ -165,41 +164,35 @@ __global__ __launch_bounds__(256) void k_grad_near(
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const int j = jb + M::crow(lk, v);
-              T r2 = 0, y[D];
+              T r2 = 0, dd[D];
 #pragma unroll
               for (int d = 0; d < D; ++d) {
-                y[d] = Ys[d][j];
-                const T dd = x[rt][d] - y[d];
-                r2 = fmat(dd, dd, r2);
+                dd[d] = x[rt][d] - Ys[d][j];
+                r2 = fmat(dd[d], dd[d], r2);
               }
               T ev[3];
               kernel_r2_all<KT, T>(r2, kp, ev);
               const T wv = w[v];
               dls = fmat(ev[KV_DL], wv, dls);
               const T u = ev[KV_PHI] * wv;
-              s0[rt] += u;
 #pragma unroll
-              for (int d = 0; d < D; ++d) sd[rt][d] = fmat(u, y[d], sd[rt][d]);
+              for (int d = 0; d < D; ++d) sd[rt][d] = fmat(u, dd[d], sd[rt][d]);
               const T b = lr < R ? Xs[j * XSP + R + lr] : T(0);
               da[rt] = M::mma(ev[KV_VALUE], b, da[rt]);
             }
           }
         }
       }
-      // point sums: add the four lane groups that share a row, then
-      // x_p sum phi W - sum phi W x_j
+      // point sums: add the four lane groups that share a row
 #pragma unroll
       for (int rt = 0; rt < kRT; ++rt) {
-        T a = s0[rt];
-        a += shfl_xor(a, 16);
-        a += shfl_xor(a, 32);
         T g[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           T b = sd[rt][d];
           b += shfl_xor(b, 16);
           b += shfl_xor(b, 32);
-          g[d] = x[rt][d] * a - b;
+          g[d] = b;
         }
         const int i = pbase + (wave + kWaves * rt) * 16 + lr;
         if (tile_on[rt] && lk == 0 && i < sz) {

@@ -289,6 +289,18 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
                  reinterpret_cast<double *>(dpoints), ld_points,
                  reinterpret_cast<double *>(dl), reinterpret_cast<void *>(stream)));
            })
+      .def("project",
+           [](Actor &a, const py::array_t<double, py::array::f_style | py::array::forcecast> &M,
+              int precision) {
+             if (M.ndim() != 2 || M.shape(0) != a.m || M.shape(1) != a.n)
+               throw py::value_error("project: M must be m x n");
+             double chk[2] = {0, 0};
+             {
+               py::gil_scoped_release nogil;
+               check(fmca_h2_project(a.h, M.data(), a.m, precision, chk));
+             }
+             return py::make_tuple(chk[0], chk[1]);
+           })
       .def("set_profiling", [](Actor &a, bool on) { check(fmca_h2_set_profiling(a.h, on ? 1 : 0)); })
       .def(
           "phase_times",

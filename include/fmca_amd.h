@@ -138,6 +138,28 @@ fmca_status fmca_h2_assemble(fmca_h2_actor *actor, int precision, int64_t *bytes
 /* Frees the assembled blocks; later actions generate entries again. */
 fmca_status fmca_h2_release_assembled(fmca_h2_actor *actor);
 
+/* Replaces H2MatrixProjector<Derived>(h2mat).project(mat)
+ * (FMCA/src/H2Matrix/H2MatrixProjector.h:21-75): projects the dense matrix M
+ * (m x n, column-major host memory with leading dimension ld, natural order:
+ * row i belongs to target point i of Peval, column j to source point j of P)
+ * onto the actor's H2 format.  Near blocks are M's blocks; every far block
+ * (t, s) gets the coupling matrix
+ *   S_ts = (V_t V_t^T)^-1 (V_t M_ts V_s^T) (V_s V_s^T)^-1,
+ * the Frobenius-orthogonal projection onto the cluster bases (FMCA computes
+ * the inverse Gram matrices through its dual cluster bases,
+ * compute_dual_cluster_bases_impl.h:25-57; here by Cholesky factors, FP64,
+ * on the host).  The projected blocks are kept in device memory in the given
+ * precision as fmca_h2_assemble keeps its blocks, and the forward actions of
+ * that precision stream them until fmca_h2_release_assembled,
+ * fmca_h2_assemble or fmca_h2_set_kernel with other parameters.
+ * dual_check (if not null): FMCA's check of the dual bases
+ * (compute_dual_cluster_bases_impl.h:66-86) on the row and the column tree,
+ * max over clusters of ||V^T - V^T (V V^T)^-1 V V^T||_F / ||V^T||_F (FMCA
+ * asserts < 5e-10).  FMCA_ERR_NOT_SUPPORTED for the total-degree
+ * interpolator, q > 128 or a singular basis Gram matrix. */
+fmca_status fmca_h2_project(fmca_h2_actor *actor, const double *M, int64_t ld, int precision,
+                            double dual_check[2]);
+
 /* Interpolation data, as the interpolator's idcs(), Xi() and invV() return
  * them (TotalDegreeInterpolator.h:50-55, TensorProductInterpolator.h:83-85):
  * the interpolator id, q, the q multi-indices in FMCA's order (q x dim,

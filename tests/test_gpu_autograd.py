@@ -287,3 +287,76 @@ def test_backward_uses_its_own_forward_lengthscale():
     assert torch.allclose(a1, a0, rtol=0, atol=1e-13 * a0.abs().max().item())
     assert abs(l1.item() - l0.item()) <= 1e-12 * abs(l0.item())
     assert torch.allclose(p1, p0, rtol=0, atol=1e-12 * p0.abs().max().item())
+
+
+def _interior_points(H, P, count, seed):
+    """Points strictly inside their leaf's box on every axis: a small move of
+    such a point changes no box, no split and no block, so the H2 forward is a
+    smooth function of it and finite differences see its exact derivative."""
+    cl = H.clusters(0)
+    perm = H.indices(0)
+    cand = []
+    for c in np.nonzero(cl["sons"][:, 0] < 0)[0]:
+        b, s = cl["begin"][c], cl["size"][c]
+        for j in perm[b:b + s]:
+            p = P[:, j]
+            if np.all(p > cl["bbmin"][c]) and np.all(p < cl["bbmax"][c]):
+                cand.append(j)
+    rng = np.random.default_rng(seed)
+    return rng.choice(np.array(cand), count, replace=False)
+
+
+@pytest.mark.parametrize("d,ktype,l,deg,nrhs,prec", [
+    (2, "MATERN32", 0.3, 5, 8, "FP64"), (3, "EXPONENTIAL", 0.5, 3, 3, "FP64"),
+    (2, "GAUSSIAN", 0.2, 4, 16, "FP64"), (1, "MATERN52", 0.2, 6, 1, "FP64"),
+    (3, "MATERN32", 0.5, 3, 12, "FP32")])
+def test_gradient_kernels_are_the_derivative_of_the_h2_forward(d, ktype, l, deg, nrhs, prec):
+    """The shared-tree gradient (fmca_h2_gradient_device, used by the backward)
+    against (a) the oracle's K~ G for dL/dalpha, (b) a central difference in
+    l of L = <G, K~(l) alpha> for dL/dl (the blocks do not depend on l),
+    (c) central differences of L in the coordinates of points inside their
+    leaf boxes (h = 1e-5).  FP64 to 1e-6 of the largest gradient entry; the
+    FP32 form against the FP64 gradient to 1e-4."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    from fmca_amd.autograd import _gradient
+    print(f"[case] gradient kernels d={d} {ktype} deg={deg} nrhs={nrhs} {prec}", flush=True)
+    n = 4000
+    P = W.uniform(n, d, 21)
+    K = fmca_amd.CovarianceKernel(ktype, l)
+    H = fmca_amd.H2kernelActor(K, P, P, deg, 0.8)
+    A = torch.from_numpy(W.normal_rhs(n, nrhs, 22)).cuda()
+    G = torch.from_numpy(W.normal_rhs(n, nrhs, 23)).cuda()
+    out64 = _gradient(H, A, G, fmca_amd.FP64)
+    assert out64 is not None
+    dal, dpo, dl = (t.cpu().numpy() for t in out64)
+    An, Gn = A.cpu().numpy(), G.cpu().numpy()
+    if prec == "FP32":
+        dal32, dpo32, dl32 = (t.cpu().numpy() for t in _gradient(H, A, G, fmca_amd.FP32))
+        assert rel_err(dal32, dal) <= 1e-4 and rel_err(dpo32, dpo) <= 1e-4
+        assert abs(dl32[0] - dl[0]) <= 1e-4 * abs(dl[0])
+        return
+    ref = oracle.OracleH2(P, P, ktype, l, deg, 0.8)
+    assert rel_err(dal, ref.action(Gn)) <= 1e-12
+
+    def L_of(Pm, lm):
+        Hm = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel(ktype, lm), Pm, Pm, deg, 0.8)
+        return float(np.sum(Gn * Hm.action(An))), Hm
+
+    h = 1e-5 * l
+    fd_l = (L_of(P, l + h)[0] - L_of(P, l - h)[0]) / (2 * h)
+    assert abs(fd_l - dl[0]) <= 1e-7 * abs(dl[0])
+    scale = np.abs(dpo).max()
+    perm0 = H.indices(0)
+    for j in _interior_points(H, P, 6, 24):
+        for k in range(d):
+            hp = 1e-5
+            Pp, Pm = P.copy(order="F"), P.copy(order="F")
+            Pp[k, j] += hp
+            Pm[k, j] -= hp
+            Lp, Hp = L_of(Pp, l)
+            Lm, Hm = L_of(Pm, l)
+            assert np.array_equal(Hp.indices(0), perm0) and np.array_equal(Hm.indices(0), perm0)
+            fd = (Lp - Lm) / (2 * hp)
+            assert abs(fd - dpo[k, j]) <= 1e-6 * scale, (j, k, fd, dpo[k, j])

@@ -121,3 +121,39 @@ def test_cg_with_bf16_matvecs_reaches_fp64_solution():
     assert ib["converged"] and ib["rel_residual"] <= 1e-10 and ib["matvecs_fp32"] > 0
     assert _resid(H, xb, b, 20.0) <= 1e-10
     assert (torch.linalg.norm(xb - x64) / torch.linalg.norm(x64)).item() <= 1e-8
+
+
+def test_clustered_configuration_needs_a_ridge_above_the_h2_error():
+    """The clustered configuration (16-Gaussian mixture, sigma 0.02, GAUSSIAN
+    l = 0.1, smooth rhs + 1% noise, 64 rhs) at n = 3e4, degree 4, eta 0.8.
+    At the configuration's ridge 1e-4 the H2 matrix plus ridge is indefinite --
+    its approximation error in the spectral norm exceeds the ridge -- and CG
+    reports that instead of iterating (p.(K~ + lambda I)p <= 0).  At ridge 0.1
+    (the smallest of 1e-3, 1e-2, 1e-1 at which FP64 CG converges there) both
+    precision forms with FP64 reliable updates reach the 1e-8 FP64 residual
+    and FP64 CG's solution."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print("[case] clustered cg ridges", flush=True)
+    n, nrhs = 30000, 64
+    P = W.gaussian_mixture(n, 3, 16, 0.02, 0)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("GAUSSIAN", 0.1), P, P, 4, 0.8)
+    y = torch.from_numpy(W.smooth_rhs(P, nrhs)).cuda()
+    _, info = H.cg(y, x0=torch.zeros_like(y), ridge=1e-4, tol=1e-8, precision=fmca_amd.FP64,
+                   max_iter=1000)
+    print(" ridge 1e-4 fp64", info, flush=True)
+    assert info["indefinite"] and not info["converged"]
+    x64, i64 = H.cg(y, x0=torch.zeros_like(y), ridge=0.1, tol=1e-8, precision=fmca_amd.FP64,
+                    max_iter=3000)
+    assert i64["converged"] and i64["rel_residual"] <= 1e-8
+    for prec in (fmca_amd.FP32, fmca_amd.FP32_BF16):
+        x, info = H.cg(y, x0=torch.zeros_like(y), ridge=0.1, tol=1e-8, precision=prec,
+                       max_iter=3000)
+        print(f" ridge 0.1 precision {prec}", info, flush=True)
+        assert info["converged"] and _resid(H, x, y, 0.1) <= 1e-8
+        # both residuals <= 1e-8 bound the difference by about cond * 2e-8
+        # (cond ~ lambda_max / 0.1 ~ 2e4 here)
+        diff = (torch.linalg.norm(x - x64) / torch.linalg.norm(x64)).item()
+        print(f"  solution difference {diff:.2e}", flush=True)
+        assert diff <= 1e-4

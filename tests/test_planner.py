@@ -366,3 +366,33 @@ def test_planner_split_axis_rule_by_hand(hand):
         cl = H.clusters(1)
         for k in ("size", "begin", "level", "sons"):
             assert np.asarray(cl[k]).tolist() == h[k], (planner, k)
+
+
+@pytest.mark.parametrize("d,deg,n,m", [(2, 3, 3000, None), (3, 2, 2500, 2000), (1, 6, 2000, None)])
+def test_projector_dual_bases_meet_fmcas_bar(d, deg, n, m):
+    """fmca_h2_project computes the inverse Gram matrices of the cluster bases
+    on the host before it needs the device; FMCA's check of its dual cluster
+    bases (compute_dual_cluster_bases_impl.h:66-86) must stay < 5e-10 on both
+    trees.  Without a device the call then reports FMCA_ERR_NO_DEVICE."""
+    lib = ctypes.CDLL(LIB)
+    lib.fmca_last_error.restype = ctypes.c_char_p
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("ktype", ctypes.c_char_p), ("l", ctypes.c_double), ("c", ctypes.c_double),
+                    ("nu", ctypes.c_double)]
+    P = uniform_points(n, d, seed=71)
+    Pe = P if m is None else uniform_points(m, d, seed=72)
+    mm = Pe.shape[1]
+    M = np.asfortranarray(np.random.default_rng(73).standard_normal((mm, n)))
+    desc = Desc(b"EXPONENTIAL", 0.5, 1.0, 1.0)
+    h = ctypes.c_void_p()
+    st = lib.fmca_h2_create_ex(ctypes.byref(desc), d, ctypes.c_int64(mm),
+                               Pe.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(n),
+                               P.ctypes.data_as(ctypes.c_void_p), deg, ctypes.c_double(0.8), 0, 0,
+                               ctypes.byref(h))
+    assert st == 0
+    chk = (ctypes.c_double * 2)(-1.0, -1.0)
+    st = lib.fmca_h2_project(h, M.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(mm), 0, chk)
+    assert st in (0, 4), lib.fmca_last_error()  # ok, or no device after the host part
+    assert 0 <= chk[0] < 5e-10 and 0 <= chk[1] < 5e-10, (chk[0], chk[1])
+    lib.fmca_h2_destroy(h)
