@@ -58,6 +58,11 @@ def parse():
     p.add_argument("--solve-precision", default="fp32_bf16", choices=["fp32", "fp32_bf16"],
                    help="matvecs of the refined solve: FP32 entries with FP32 (fp32) or "
                         "BF16 three-term (fp32_bf16) matrix-core contractions")
+    p.add_argument("--solve-method", default="gcr", choices=["gcr", "cg"],
+                   help="method of the refined solve (FP64 residual replacement in both)")
+    p.add_argument("--refine-factor", type=float, default=None,
+                   help="residual drop between FP64 replacements (default 0.01 for gcr, "
+                        "0.1 for cg)")
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
                             "total-degree", "plan", "assembled"],
@@ -477,20 +482,21 @@ def side_large(args):
     return res
 
 
-def refined_solves(H, x, ridge, steps, warmup, precision):
-    """The headline step: one CG solve of (K + ridge I) a = x for all columns
-    to a 1e-8 relative FP64 residual, FP32 matvecs driving the recurrence and
-    FP64 reliable updates (fmca_h2_cg_device), from a = 0.  Profiling events
-    (pooled, created before the timed region) stay on through the timed
-    solves, so that the FP32 leaf-output launch time of the roofline is
-    measured on the launches of the timed region itself."""
+def refined_solves(H, x, ridge, steps, warmup, precision, method, factor):
+    """The headline step: one solve of (K + ridge I) a = x for all columns
+    to a 1e-8 relative FP64 residual (fmca_h2_cg_device), FP32-class matvecs
+    driving the iteration and FP64 residual replacement, from a = 0.
+    Profiling events (pooled, created before the timed region) stay on
+    through the timed solves, so that the leaf-output launch time of the
+    roofline is measured on the launches of the timed region itself."""
     import torch
     import fmca_amd
     infos = []
 
     def one():
         a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8,
-                       precision=precision, max_iter=2000)
+                       precision=precision, refine_factor=factor, method=method,
+                       max_iter=2000)
         return a, info
     H.set_profiling(True)
     for _ in range(warmup):
@@ -510,16 +516,16 @@ def refined_solves(H, x, ridge, steps, warmup, precision):
     return t, a, infos, ph32
 
 
-def fp64_solve(H, x, ridge):
-    """Plain FP64 CG on the same system, from a = 0, timed once."""
+def once_solve(H, x, ridge, precision, method="cg", factor=0.1):
+    """One solve of the same system from a = 0, timed once (FMCA's plain FP64
+    CG by default)."""
     import torch
-    import fmca_amd
     H.cg(x[:, :1].contiguous(), x0=torch.zeros_like(x[:, :1].contiguous()), ridge=ridge,
-         tol=1e-8, precision=fmca_amd.FP64, max_iter=3)  # sizes nothing new, warms the path
+         tol=1e-8, precision=precision, method=method, max_iter=3)  # warms the path
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8, precision=fmca_amd.FP64,
-                   max_iter=2000)
+    a, info = H.cg(x, x0=torch.zeros_like(x), ridge=ridge, tol=1e-8, precision=precision,
+                   refine_factor=factor, method=method, max_iter=2000)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, a, info
 
@@ -597,12 +603,23 @@ def main():
                           "fp32_ms": t32 * 1e3, "fp32_bf16_ms": tbf * 1e3, "fp64_ms": t64 * 1e3,
                           "fp32_rhs1_ms": t32_1 * 1e3, "fp64_rhs1_ms": t64_1 * 1e3}), flush=True)
         return
-    # the headline: refined solves (timed steps), then plain FP64 CG once
+    # the headline: refined solves (timed steps); then, once each, FMCA's plain
+    # FP64 CG (whose iteration count the value credits), the same method in
+    # FP64, and the other method with the same matvecs
+    factor = args.refine_factor or (0.01 if args.solve_method == "gcr" else 0.1)
     t_ref, a_ref, infos, ph_solve = refined_solves(H, x, args.ridge, args.steps, args.warmup,
-                                                   solve_prec)
-    t_64, a_64, info64 = fp64_solve(H, x, args.ridge)
+                                                   solve_prec, args.solve_method, factor)
+    t_64, a_64, info64 = once_solve(H, x, args.ridge, fmca_amd.FP64)
     sol_diff = float(torch.linalg.norm(a_ref - a_64) / torch.linalg.norm(a_64))
-    del a_ref, a_64
+    del a_ref
+    other = "cg" if args.solve_method == "gcr" else "gcr"
+    t_64m, a_64m, info64m = (once_solve(H, x, args.ridge, fmca_amd.FP64, "gcr", 0.1)
+                             if args.solve_method == "gcr" else (t_64, None, info64))
+    del a_64m
+    t_oth, a_oth, info_oth = once_solve(H, x, args.ridge, solve_prec, other,
+                                        0.1 if other == "cg" else 0.01)
+    oth_diff = float(torch.linalg.norm(a_oth - a_64) / torch.linalg.norm(a_64))
+    del a_oth, a_64
     iters64 = info64["iterations"]
     value = n * nrhs * iters64 / t_ref / 1e6
     value_fp64 = n * nrhs * iters64 / t_64 / 1e6
@@ -674,11 +691,13 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic: uniform points in [0,1]^3 and N(0,1) rhs, numpy default_rng(0)",
         "config": {
-            "workload": (f"CG solve of (K + {args.ridge:g} I) a = x to a 1e-8 relative FP64 "
+            "workload": (f"solve of (K + {args.ridge:g} I) a = x to a 1e-8 relative FP64 "
                          f"residual, n={n} uniform 3D points, MATERN32 l=0.5, tensor Chebyshev "
-                         f"deg 4 (q={q}), eta=0.8, {nrhs} rhs N(0,1); FP32 matvecs + FP64 "
-                         f"reliable updates; value = n * nrhs * (iterations of plain FP64 CG "
-                         f"to the same tolerance) / seconds per refined solve"),
+                         f"deg 4 (q={q}), eta=0.8, {nrhs} rhs N(0,1); "
+                         f"{args.solve_method.upper()} on {args.solve_precision.upper()} "
+                         f"matvecs + FP64 residual replacement (factor {factor:g}); value = "
+                         f"n * nrhs * (iterations of FMCA's plain FP64 CG to the same "
+                         f"tolerance) / seconds per refined solve"),
             "n": n, "nrhs": nrhs, "kernel": "MATERN32", "lengthscale": 0.5, "degree": 4,
             "eta": 0.8, "ridge": args.ridge, "tol": 1e-8,
             "far_blocks": int(info["far_blocks"]),
@@ -687,12 +706,18 @@ def main():
         },
         "roofline": roofline,
         "solve": {
-            "fp32_refined": {**infos[-1], "seconds_mean": t_ref,
+            "fp32_refined": {**infos[-1], "method": args.solve_method,
+                             "precision": args.solve_precision, "refine_factor": factor,
+                             "seconds_mean": t_ref,
                              "iterations_all_steps": [i["iterations"] for i in infos],
                              "rel_residual_max_all_steps": max(i["rel_residual"] for i in infos)},
-            "fp64": {**info64, "wall_s": t_64, "value": value_fp64},
+            "fp64": {**info64, "method": "cg", "wall_s": t_64, "value": value_fp64},
+            "fp64_same_method": {**info64m, "method": args.solve_method, "wall_s": t_64m},
+            "refined_other_method": {**info_oth, "method": other, "wall_s": t_oth,
+                                     "solution_rel_diff": oth_diff},
             "solution_rel_diff": sol_diff,
             "speedup_fp32_refined_vs_fp64": t_64 / t_ref,
+            "speedup_fp32_refined_vs_fp64_same_method": t_64m / t_ref,
         },
         "matvec": {
             "fp32": {"ms": t32 * 1e3, "mpts": n * nrhs / t32 / 1e6,

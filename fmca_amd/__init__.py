@@ -225,18 +225,32 @@ class H2kernelActor:
     # -- solver ----------------------------------------------------------------
     def cg(self, b, x0=None, ridge: float = 0.0, tol: float = 1e-8, precision: int = FP32,
            refine_factor: float = 0.1, max_iter: int = 100000, max_outer: int = 1000,
-           stream=None):
-        """Solve (K + ridge I) x = b by CG (FMCA's pCG, PCG.h:15-44, identity
-        preconditioner) for every column of b.
+           method: str = "cg", max_basis: int = 0, stream=None):
+        """Solve (K + ridge I) x = b for every column of b, to
+        ||b - (K + ridge I) x|| <= tol ||b|| (FMCA's pCG, PCG.h:15-44, identity
+        preconditioner).
 
-        precision=FP64: plain CG.  precision=FP32: CG with reliable updates --
-        FP32 matvecs drive the recurrence, FP64 vectors and inner products, and
-        the iterated residual is replaced by the FP64 residual each time it has
-        dropped by refine_factor (all columns together; the search direction
-        is re-orthogonalised against the new residual).  b, x0: float64 CUDA tensors (n,) or
-        (n, nrhs); x0 defaults to b as in pCG.  Returns (x, info) with
-        info['rel_residual'] measured with the FP64 action.
+        method="cg": FMCA's CG.  precision=FP64: plain CG.  precision=FP32 or
+        FP32_BF16: reliable updates -- the lower-precision matvecs drive the
+        recurrence, FP64 vectors and inner products, and the iterated residual
+        is replaced by the FP64 residual each time it has dropped by
+        refine_factor (all columns together; the search direction is
+        re-orthogonalised against the new residual).
+
+        method="gcr": GCR over a window of up to max_basis stored pairs
+        (p, (K + ridge I) p) (0: as many as fit in half the free device memory,
+        at most 256), the residual minimised over the window, with the same
+        replacement rule; a replaced residual is projected on the window.  The
+        pairs stay consistent under lower-precision matvecs, so FP32 forms
+        converge in about as many iterations as FP64.
+
+        b, x0: float64 CUDA tensors (n,) or (n, nrhs); x0 defaults to b as in
+        pCG.  Returns (x, info) with info['rel_residual'] measured with the
+        FP64 action.
         """
+        methods = {"cg": 0, "gcr": 1}
+        if method not in methods:
+            raise ValueError(f"method must be one of {sorted(methods)}")
         import torch
         bm = _as_device_matrix(b, self.n)
         nrhs = bm.shape[1]
@@ -246,7 +260,8 @@ class H2kernelActor:
         info = self._a.cg_device(bm.data_ptr(), bm.stride(1) if nrhs > 1 else self.n, nrhs,
                                  x.data_ptr(), x.stride(1) if nrhs > 1 else self.n, float(ridge),
                                  float(tol), int(precision), float(refine_factor), int(max_iter),
-                                 int(max_outer), s.cuda_stream)
+                                 int(max_outer), method=methods[method],
+                                 max_basis=int(max_basis), stream=s.cuda_stream)
         return (x if b.dim() == 2 else x[:, 0]), info
 
     # -- plan queries ---------------------------------------------------------

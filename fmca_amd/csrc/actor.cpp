@@ -193,6 +193,10 @@ struct fmca_h2_actor {
   Workspace<float> ws32;
   DevBuf stage_in, stage_out;
   DevBuf cg_ws, cg_scal;  // CG vectors and per-column scalars
+  // GCR basis: chunks of kBasisChunk slot pairs, slot pointer tables (held
+  // during one solve)
+  std::vector<std::unique_ptr<DevBuf>> basis;
+  DevBuf basis_tab;
   DevBuf grad_x, grad_part;  // gradient: [alpha | G | alpha] rhs, partial sums of dL/dl
   // assembled blocks (fmca_h2_assemble) of precision st_prec (-1: none)
   DevBuf st_far, st_near, st_near_off, st_v_off, st_far_tgt, st_near_leaf, st_far_order,
@@ -1369,6 +1373,243 @@ fmca_status fmca_h2_blocks(const fmca_h2_actor *actor, int32_t *row, int32_t *co
   return FMCA_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+constexpr int kBasisChunk = 8;  // slot pairs per basis allocation
+
+// GCR with FP64 residual replacement, the method FMCA_SOLVER_GCR of
+// fmca_h2_cg_device (no FMCA counterpart).  GCR (Eisenstat, Elman & Schultz
+// 1983): each iteration multiplies the residual, q0 = (K + ridge I) r,
+// orthogonalises q0 against the stored images (one classical Gram-Schmidt
+// pass) and applies the same combination to p0 = r, so a stored pair keeps
+// q_j = (K + ridge I) p_j as that matvec computed it, whatever its rounding;
+// the residual then drops by its projection on the new image, which
+// minimises ||r|| over the window.  Lower-precision matvecs change the
+// operator from one iteration to the next by their rounding; the pairs stay
+// consistent, so the iteration count follows the FP64 one.  Replacement as in
+// CG (every active column dropped by refine_factor, or one reached tol in
+// FP64); a column whose FP64 residual meets tol stops, the others have the
+// replaced residual projected on the window at once.  Pairs are stored as S
+// (double for FP64 matvecs, float otherwise) in a circular window of at most
+// m pairs; the oldest is dropped when it is full.
+template <typename S>
+fmca_status gcr_solve(fmca_h2_actor *a, const double *d_b, int64_t ld_b, int nr, double *d_x,
+                      int64_t ld_x, const fmca_cg_options *opts, fmca_cg_result *res,
+                      hipStream_t sm, std::chrono::steady_clock::time_point t0) {
+  const int64_t n = a->n;
+  const double ridge = opts->ridge;
+  const bool cl = a->shared;
+  const uint32_t *perm = a->d_rows_i.perm.as<uint32_t>();
+  const size_t vec = (size_t)n * nr;
+  const int prec = opts->precision;
+  const int max_iter = opts->max_iter > 0 ? opts->max_iter : 100000;
+  const int max_updates = opts->max_outer > 0 ? opts->max_outer : 1000;
+  const double delta = (opts->refine_factor > 0 && opts->refine_factor < 1) ? opts->refine_factor
+                                                                              : 0.1;
+  // window size
+  const size_t pair_bytes = 2 * vec * sizeof(S);
+  int m = opts->max_basis;
+  if (m < 0 || m > kBasisMax)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "max_basis must be in [0, 256]");
+  if (m == 0) {
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    size_t budget = free_b / 2;
+    const int64_t cap = g_dev_limit.load();
+    if (cap > 0) {
+      const int64_t left = cap - g_dev_live.load() - (int64_t)(4 * vec * sizeof(double));
+      budget = std::min<size_t>(budget, left > 0 ? (size_t)left / 2 : 0);
+    }
+    m = (int)std::min<size_t>((size_t)kBasisMax, budget / pair_bytes);
+    m = std::min(m, max_iter + 1);
+  }
+  if (m < 2) return fail(FMCA_ERR_OUT_OF_MEMORY, "GCR needs room for two basis pairs");
+  HIP_TRY(a->cg_ws.alloc(4 * vec * sizeof(double)));
+  // scalars: rr, qqrq, coef, qq table (doubles), then partial sums, then active flags
+  const size_t nd = (size_t)nr * (1 + 2 + 2 * (size_t)kBasisMax);
+  const size_t scratch_d =
+      std::max(basis_scratch(nr, kBasisMax), blas1_dot_scratch(nr)) / sizeof(double);
+  HIP_TRY(a->cg_scal.alloc((nd + scratch_d) * sizeof(double) + (size_t)nr * sizeof(int)));
+  double *w = a->cg_ws.as<double>();
+  double *B = w, *X = w + vec, *R = w + 2 * vec, *Y = w + 3 * vec;
+  double *sc = a->cg_scal.as<double>();
+  double *d_rr = sc, *d_qqrq = sc + nr, *d_coef = sc + 3 * nr,
+         *d_qq = sc + 3 * nr + (size_t)nr * kBasisMax, *d_scratch = sc + nd;
+  int *d_active = reinterpret_cast<int *>(sc + nd + scratch_d);
+  auto chk = [&](hipError_t e, const char *what) { return from_hip(e, what); };
+#define GCR_TRY(expr)                                 \
+  do {                                                \
+    fmca_status _s = chk((expr), #expr);              \
+    if (_s != FMCA_OK) return _s;                     \
+  } while (0)
+  auto matvec = [&](const double *in, double *out, int p) -> fmca_status {
+    if (p == FMCA_PRECISION_FP64) res->matvecs_fp64++; else res->matvecs_fp32++;
+    return enqueue_any(a, in, n, nr, out, n, p, !cl, false, sm);
+  };
+  std::vector<double> rn2(nr), bn2(nr), target(nr), rmax(nr);
+  std::vector<int> active(nr);
+  auto read_rr = [&]() -> fmca_status {
+    GCR_TRY(hipMemcpyAsync(rn2.data(), d_rr, nr * sizeof(double), hipMemcpyDeviceToHost, sm));
+    GCR_TRY(hipStreamSynchronize(sm));
+    return FMCA_OK;
+  };
+  auto push_active = [&]() -> hipError_t {
+    return hipMemcpyAsync(d_active, active.data(), nr * sizeof(int), hipMemcpyHostToDevice, sm);
+  };
+  // basis slots: chunks allocated as the window grows, released at the end
+  std::vector<const S *> qtab(m, nullptr), ptab(m, nullptr);
+  a->basis.clear();
+  HIP_TRY(a->basis_tab.alloc(2 * (size_t)m * sizeof(void *)));
+  const S *const *d_qtab = reinterpret_cast<const S *const *>(a->basis_tab.p);
+  const S *const *d_ptab = d_qtab + m;
+  int allocated = 0;
+  auto ensure_slot = [&](int slot) -> fmca_status {
+    if (slot < allocated) return FMCA_OK;
+    auto buf = std::make_unique<DevBuf>();
+    const int k = std::min(kBasisChunk, m - allocated);
+    hipError_t e = buf->alloc((size_t)k * pair_bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(FMCA_ERR_OUT_OF_MEMORY, "GCR basis allocation failed (lower max_basis)");
+    }
+    S *base = buf->as<S>();
+    for (int j = 0; j < k; ++j) {
+      ptab[allocated + j] = base + (size_t)j * vec;
+      qtab[allocated + j] = base + (size_t)(k + j) * vec;
+    }
+    a->basis.push_back(std::move(buf));
+    allocated += k;
+    GCR_TRY(hipMemcpyAsync(a->basis_tab.p, qtab.data(), m * sizeof(void *),
+                           hipMemcpyHostToDevice, sm));
+    GCR_TRY(hipMemcpyAsync(reinterpret_cast<char *>(a->basis_tab.p) + m * sizeof(void *),
+                           ptab.data(), m * sizeof(void *), hipMemcpyHostToDevice, sm));
+    // the host tables change when the next chunk arrives
+    GCR_TRY(hipStreamSynchronize(sm));
+    return FMCA_OK;
+  };
+  fmca_status st;
+  if (cl) {
+    GCR_TRY(blas1_permute(n, nr, perm, true, d_b, ld_b, B, n, sm));
+    GCR_TRY(blas1_permute(n, nr, perm, true, d_x, ld_x, X, n, sm));
+  } else {
+    GCR_TRY(hipMemcpy2DAsync(B, n * sizeof(double), d_b, ld_b * sizeof(double),
+                             n * sizeof(double), nr, hipMemcpyDeviceToDevice, sm));
+    GCR_TRY(hipMemcpy2DAsync(X, n * sizeof(double), d_x, ld_x * sizeof(double),
+                             n * sizeof(double), nr, hipMemcpyDeviceToDevice, sm));
+  }
+  GCR_TRY(blas1_dot(n, nr, B, n, B, n, d_rr, d_scratch, sm));
+  if ((st = read_rr()) != FMCA_OK) return st;
+  bn2 = rn2;
+  for (int j = 0; j < nr; ++j)
+    if (bn2[j] == 0) bn2[j] = 1;  // zero rhs: absolute residual
+  // r = b - (K + ridge I) x, FP64; exactly b when x = 0
+  GCR_TRY(blas1_dot(n, nr, X, n, X, n, d_rr, d_scratch, sm));
+  if ((st = read_rr()) != FMCA_OK) return st;
+  if (std::all_of(rn2.begin(), rn2.end(), [](double v) { return v == 0; })) {
+    GCR_TRY(hipMemcpyAsync(R, B, vec * sizeof(double), hipMemcpyDeviceToDevice, sm));
+  } else {
+    if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+    GCR_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+  }
+  GCR_TRY(blas1_dot(n, nr, R, n, R, n, d_rr, d_scratch, sm));
+  if ((st = read_rr()) != FMCA_OK) return st;
+  bool any = false;
+  double worst = 0;
+  for (int j = 0; j < nr; ++j) {
+    rmax[j] = std::sqrt(rn2[j]);
+    target[j] = opts->tol * std::sqrt(bn2[j]);
+    active[j] = rmax[j] > target[j];
+    any = any || active[j];
+    worst = std::max(worst, rmax[j] / std::sqrt(bn2[j]));
+  }
+  res->rel_residual = worst;
+  GCR_TRY(push_active());
+  int first = 0, count = 0;
+  while (any && res->iterations < max_iter && res->outer < max_updates) {
+    // q0 = (K + ridge I) r
+    if ((st = matvec(R, Y, prec)) != FMCA_OK) return st;
+    GCR_TRY(blas1_add_ridge(n, nr, ridge, R, n, Y, n, sm));
+    if (count == m) {  // full: drop the oldest pair, whose slot takes the new one
+      first = (first + 1) % m;
+      --count;
+    }
+    const int slot = (first + count) % m;
+    if ((st = ensure_slot(slot)) != FMCA_OK) return st;
+    GCR_TRY(basis_dots<S>(n, nr, Y, d_qtab, first, count, m, d_qq, d_active, d_coef, d_scratch,
+                          sm));
+    S *qk = const_cast<S *>(qtab[slot]), *pk = const_cast<S *>(ptab[slot]);
+    GCR_TRY(basis_new_pair<S>(n, nr, Y, R, d_coef, d_qtab, d_ptab, first, count, m, qk, pk,
+                              d_active, d_qqrq, d_scratch, sm));
+    GCR_TRY(basis_update<S>(n, nr, pk, qk, d_qqrq, d_qq + (size_t)slot * nr, X, R, d_active,
+                            d_rr, d_scratch, sm));
+    ++count;
+    if ((st = read_rr()) != FMCA_OK) return st;
+    res->iterations++;
+    bool any_done = false, all_ready = true;
+    for (int j = 0; j < nr; ++j) {
+      if (!active[j]) continue;
+      const double rn = std::sqrt(rn2[j]);
+      if (!std::isfinite(rn)) return fail(FMCA_ERR_NONFINITE, "GCR residual is not finite");
+      const bool done = rn <= target[j];
+      any_done = any_done || done;
+      all_ready = all_ready && (done || rn < delta * rmax[j]);
+    }
+    const bool update = prec == FMCA_PRECISION_FP64 ? any_done : all_ready;
+    if (!update) continue;
+    // FP64 residual; columns that meet tol stop, the others are projected
+    if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+    GCR_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+    GCR_TRY(blas1_dot(n, nr, R, n, R, n, d_rr, d_scratch, sm));
+    if ((st = read_rr()) != FMCA_OK) return st;
+    res->outer++;
+    worst = 0;
+    any = false;
+    for (int j = 0; j < nr; ++j) {
+      const double rn = std::sqrt(rn2[j]);
+      worst = std::max(worst, rn / std::sqrt(bn2[j]));
+      if (active[j] && rn <= target[j]) active[j] = 0;
+      any = any || active[j];
+    }
+    res->rel_residual = worst;
+    GCR_TRY(push_active());
+    if (!any) break;
+    GCR_TRY(basis_dots<S>(n, nr, R, d_qtab, first, count, m, d_qq, d_active, d_coef, d_scratch,
+                          sm));
+    GCR_TRY(basis_project<S>(n, nr, d_coef, d_qtab, d_ptab, first, count, m, X, R, d_active,
+                             d_rr, d_scratch, sm));
+    if ((st = read_rr()) != FMCA_OK) return st;
+    for (int j = 0; j < nr; ++j)
+      if (active[j]) rmax[j] = std::sqrt(rn2[j]);
+  }
+  res->converged = !any;
+  if (any) {  // stopped by a cap: report the FP64 residual of the returned x
+    if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+    GCR_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+    GCR_TRY(blas1_dot(n, nr, R, n, R, n, d_rr, d_scratch, sm));
+    if ((st = read_rr()) != FMCA_OK) return st;
+    worst = 0;
+    for (int j = 0; j < nr; ++j) worst = std::max(worst, std::sqrt(rn2[j] / bn2[j]));
+    res->rel_residual = worst;
+  }
+  if (cl)
+    GCR_TRY(blas1_permute(n, nr, perm, false, X, n, d_x, ld_x, sm));
+  else
+    GCR_TRY(hipMemcpy2DAsync(d_x, ld_x * sizeof(double), X, n * sizeof(double),
+                             n * sizeof(double), nr, hipMemcpyDeviceToDevice, sm));
+  GCR_TRY(hipStreamSynchronize(sm));
+#undef GCR_TRY
+  a->basis.clear();
+  a->basis_tab.release();
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res->converged ? FMCA_OK : fail(FMCA_ERR_NOT_CONVERGED, "GCR did not reach tol");
+}
+
+}  // namespace
+
+extern "C" {
+
 fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b, int64_t nrhs,
                               double *d_x, int64_t ld_x, const fmca_cg_options *opts,
                               fmca_cg_result *res, void *stream_ptr) {
@@ -1387,6 +1628,14 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
   if (st != FMCA_OK) return st;
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t sm = (hipStream_t)stream_ptr;
+  if (opts->method != FMCA_SOLVER_CG && opts->method != FMCA_SOLVER_GCR)
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown solver method");
+  if (opts->method == FMCA_SOLVER_GCR) {
+    *res = fmca_cg_result{};
+    return opts->precision == FMCA_PRECISION_FP64
+               ? gcr_solve<double>(a, d_b, ld_b, (int)nrhs, d_x, ld_x, opts, res, sm, t0)
+               : gcr_solve<float>(a, d_b, ld_b, (int)nrhs, d_x, ld_x, opts, res, sm, t0);
+  }
   const int64_t n = a->n;
   const int nr = (int)nrhs;
   const double ridge = opts->ridge;
@@ -1442,9 +1691,14 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
   if ((st = dot(B, B, bn2.data())) != FMCA_OK) return st;
   for (int j = 0; j < nr; ++j)
     if (bn2[j] == 0) bn2[j] = 1;  // zero rhs: absolute residual
-  // r = b - (K + ridge I) x, FP64
-  if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
-  CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+  // r = b - (K + ridge I) x, FP64; exactly b when x = 0
+  if ((st = dot(X, X, rn2.data())) != FMCA_OK) return st;
+  if (std::all_of(rn2.begin(), rn2.end(), [](double v) { return v == 0; })) {
+    CG_TRY(hipMemcpyAsync(R, B, vec * sizeof(double), hipMemcpyDeviceToDevice, sm));
+  } else {
+    if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+    CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+  }
   // CG with reliable updates (Sleijpen & van der Vorst 1996; the form of
   // Clark et al. 2010): the recurrence runs with matvecs of the chosen
   // precision (FP64 vectors, inner products and updates throughout).  Once
@@ -1545,6 +1799,14 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *a, const double *d_b, int64_t ld_b,
     CG_TRY(blas1_xpby(n, nr, d_beta, R, n, Pv, n, sm));
   }
   res->converged = !any;
+  if (any) {  // stopped by a cap: report the FP64 residual of the returned x
+    if ((st = matvec(X, Y, FMCA_PRECISION_FP64)) != FMCA_OK) return st;
+    CG_TRY(blas1_residual(n, nr, ridge, B, n, Y, n, X, n, R, n, sm));
+    if ((st = dot(R, R, rn2.data())) != FMCA_OK) return st;
+    worst = 0;
+    for (int j = 0; j < nr; ++j) worst = std::max(worst, std::sqrt(rn2[j] / bn2[j]));
+    res->rel_residual = worst;
+  }
   if (cl)
     CG_TRY(blas1_permute(n, nr, perm, false, X, n, d_x, ld_x, sm));
   else

@@ -118,6 +118,202 @@ __global__ __launch_bounds__(kThreads) void k_permute(int64_t n, const uint32_t 
   }
 }
 
+// ---- Krylov basis of the GCR solve ------------------------------------------
+// Pairs (p_j, q_j = (K + ridge I) p_j) live in the slots of a circular window:
+// window position j is slot (first + j) % cap, and a slot holds n x nrhs values
+// of type S (column-major, leading dimension n) at ptab[slot] / qtab[slot].
+// Columns with active[col] == 0 are left untouched and report zero sums.
+
+constexpr int kBasisGroup = 16;  // basis vectors per pass of a dot block
+
+// Sum of s[k] over the block, k < N; thread k receives sum k.  Fixed order.
+template <int N>
+__device__ __forceinline__ double block_sums(double (&s)[N], double *lds /* [kThreads/64][N] */) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double v = s[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    s[k] = v;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) lds[wave * N + k] = s[k];
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x < N)
+    for (int w = 0; w < kThreads / 64; ++w) t += lds[w * N + threadIdx.x];
+  __syncthreads();
+  return t;
+}
+
+// partial[(col * count + j) * kDotBlocks + block] = sum over the block's rows of
+// v * q_j (v: n x nrhs FP64)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_basis_dots(int64_t n, const double *__restrict__ v,
+                                                         const S *const *__restrict__ qtab,
+                                                         int first, int count, int cap,
+                                                         const int *__restrict__ active,
+                                                         double *__restrict__ partial) {
+  const int col = blockIdx.y;
+  const bool on = active[col] != 0;
+  const double *vc = v + (size_t)col * n;
+  __shared__ double lds[(kThreads / 64) * kBasisGroup];
+  for (int g0 = 0; g0 < count; g0 += kBasisGroup) {
+    const int ng = min(kBasisGroup, count - g0);
+    const S *q[kBasisGroup];
+#pragma unroll
+    for (int k = 0; k < kBasisGroup; ++k)
+      q[k] = qtab[(first + g0 + min(k, ng - 1)) % cap] + (size_t)col * n;
+    double s[kBasisGroup];
+#pragma unroll
+    for (int k = 0; k < kBasisGroup; ++k) s[k] = 0;
+    if (on)
+      for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+           i += (int64_t)kDotBlocks * kThreads) {
+        const double vi = vc[i];
+#pragma unroll
+        for (int k = 0; k < kBasisGroup; ++k)
+          if (k < ng) s[k] = __builtin_fma(vi, (double)q[k][i], s[k]);
+      }
+    const double t = block_sums(s, lds);
+    if (threadIdx.x < ng)
+      partial[((size_t)col * count + g0 + threadIdx.x) * kDotBlocks + blockIdx.x] = t;
+  }
+}
+
+// coef[col * count + j] = (v . q_j) / (q_j . q_j), zero for inactive columns
+// and empty pairs; qq[slot * nrhs + col] holds q_slot . q_slot
+__global__ __launch_bounds__(kDotBlocks) void k_basis_coef(const double *__restrict__ partial,
+                                                           int count, const double *__restrict__ qq,
+                                                           int first, int cap, int nrhs,
+                                                           const int *__restrict__ active,
+                                                           double *__restrict__ coef) {
+  const int j = blockIdx.x, col = blockIdx.y;
+  __shared__ double red[kDotBlocks];
+  red[threadIdx.x] = partial[((size_t)col * count + j) * kDotBlocks + threadIdx.x];
+  __syncthreads();
+  for (int w = kDotBlocks / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double d = qq[(size_t)((first + j) % cap) * nrhs + col];
+    coef[(size_t)col * count + j] = active[col] && d > 0 ? red[0] / d : 0.0;
+  }
+}
+
+// The new pair from q0 = (K + ridge I) r (in y) and p0 = r, orthogonalised
+// against the window: q = q0 - sum_j c_j q_j, p = r - sum_j c_j p_j, stored in
+// slot (qk, pk); partial sums of q.q and r.q (of the stored q) per block.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_basis_new_pair(
+    int64_t n, const double *__restrict__ y, const double *__restrict__ r,
+    const double *__restrict__ coef, const S *const *__restrict__ qtab,
+    const S *const *__restrict__ ptab, int first, int count, int cap, S *__restrict__ qk,
+    S *__restrict__ pk, const int *__restrict__ active, double *__restrict__ partial) {
+  const int col = blockIdx.y;
+  __shared__ double c[kBasisMax];
+  __shared__ const S *qs[kBasisMax];
+  __shared__ const S *ps[kBasisMax];
+  __shared__ double lds[(kThreads / 64) * 2];
+  const bool on = active[col] != 0;
+  for (int j = threadIdx.x; j < count; j += kThreads) {
+    const int sl = (first + j) % cap;
+    c[j] = coef[(size_t)col * count + j];
+    qs[j] = qtab[sl] + (size_t)col * n;
+    ps[j] = ptab[sl] + (size_t)col * n;
+  }
+  __syncthreads();
+  double s[2] = {0, 0};
+  if (on) {
+    const double *yc = y + (size_t)col * n, *rc = r + (size_t)col * n;
+    S *qc = qk + (size_t)col * n, *pc = pk + (size_t)col * n;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)kDotBlocks * kThreads) {
+      const double ri = rc[i];
+      double q = yc[i], p = ri;
+      for (int j = 0; j < count; ++j) {
+        q = __builtin_fma(-c[j], (double)qs[j][i], q);
+        p = __builtin_fma(-c[j], (double)ps[j][i], p);
+      }
+      const S qs_ = (S)q;
+      qc[i] = qs_;
+      pc[i] = (S)p;
+      const double qd = (double)qs_;
+      s[0] = __builtin_fma(qd, qd, s[0]);
+      s[1] = __builtin_fma(ri, qd, s[1]);
+    }
+  }
+  const double t = block_sums(s, lds);
+  if (threadIdx.x < 2) partial[((size_t)col * 2 + threadIdx.x) * kDotBlocks + blockIdx.x] = t;
+}
+
+// x += a p, r -= a q with a = (r.q) / (q.q) from qqrq[2 col + {0, 1}];
+// records q.q of the new slot; partial sums of the new r.r per block
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_basis_update(
+    int64_t n, const S *__restrict__ pk, const S *__restrict__ qk,
+    const double *__restrict__ qqrq, double *__restrict__ qq_slot, double *__restrict__ x,
+    double *__restrict__ r, const int *__restrict__ active, double *__restrict__ partial) {
+  const int col = blockIdx.y;
+  __shared__ double lds[kThreads / 64];
+  double s[1] = {0};
+  if (active[col]) {
+    const double qq = qqrq[2 * col], rq = qqrq[2 * col + 1];
+    const double a = qq > 0 ? rq / qq : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) qq_slot[col] = qq;
+    const S *pc = pk + (size_t)col * n, *qc = qk + (size_t)col * n;
+    double *xc = x + (size_t)col * n, *rc = r + (size_t)col * n;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)kDotBlocks * kThreads) {
+      xc[i] = __builtin_fma(a, (double)pc[i], xc[i]);
+      const double rn = __builtin_fma(-a, (double)qc[i], rc[i]);
+      rc[i] = rn;
+      s[0] = __builtin_fma(rn, rn, s[0]);
+    }
+  }
+  const double t = block_sums(s, lds);
+  if (threadIdx.x == 0) partial[(size_t)col * kDotBlocks + blockIdx.x] = t;
+}
+
+// x += sum_j c_j p_j, r -= sum_j c_j q_j; partial sums of the new r.r per block
+template <typename S>
+__global__ __launch_bounds__(kThreads) void k_basis_project(
+    int64_t n, const double *__restrict__ coef, const S *const *__restrict__ qtab,
+    const S *const *__restrict__ ptab, int first, int count, int cap, double *__restrict__ x,
+    double *__restrict__ r, const int *__restrict__ active, double *__restrict__ partial) {
+  const int col = blockIdx.y;
+  __shared__ double c[kBasisMax];
+  __shared__ const S *qs[kBasisMax];
+  __shared__ const S *ps[kBasisMax];
+  __shared__ double lds[kThreads / 64];
+  for (int j = threadIdx.x; j < count; j += kThreads) {
+    const int sl = (first + j) % cap;
+    c[j] = coef[(size_t)col * count + j];
+    qs[j] = qtab[sl] + (size_t)col * n;
+    ps[j] = ptab[sl] + (size_t)col * n;
+  }
+  __syncthreads();
+  double s[1] = {0};
+  if (active[col]) {
+    double *xc = x + (size_t)col * n, *rc = r + (size_t)col * n;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)kDotBlocks * kThreads) {
+      double xi = xc[i], ri = rc[i];
+      for (int j = 0; j < count; ++j) {
+        xi = __builtin_fma(c[j], (double)ps[j][i], xi);
+        ri = __builtin_fma(-c[j], (double)qs[j][i], ri);
+      }
+      xc[i] = xi;
+      rc[i] = ri;
+      s[0] = __builtin_fma(ri, ri, s[0]);
+    }
+  }
+  const double t = block_sums(s, lds);
+  if (threadIdx.x == 0) partial[(size_t)col * kDotBlocks + blockIdx.x] = t;
+}
+
 dim3 grid_for(int64_t n, int nrhs) {
   const int64_t g = (n + kThreads - 1) / kThreads;
   return dim3((unsigned)(g < 1024 ? (g > 0 ? g : 1) : 1024), (unsigned)nrhs);
@@ -168,6 +364,74 @@ hipError_t blas1_fill(int64_t n, int nrhs, double v, double *x, int64_t ldx, hip
   hipLaunchKernelGGL(k_fill, grid_for(n, nrhs), dim3(kThreads), 0, stream, n, v, x, ldx);
   return hipGetLastError();
 }
+
+size_t basis_scratch(int nrhs, int count) {
+  return (size_t)nrhs * (size_t)(count > 2 ? count : 2) * kDotBlocks * sizeof(double);
+}
+
+template <typename S>
+hipError_t basis_dots(int64_t n, int nrhs, const double *v, const S *const *qtab, int first,
+                      int count, int cap, const double *qq, const int *active, double *coef,
+                      double *scratch, hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  if (count > kBasisMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_basis_dots<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n, v,
+                     qtab, first, count, cap, active, scratch);
+  hipLaunchKernelGGL(k_basis_coef, dim3(count, nrhs), dim3(kDotBlocks), 0, stream, scratch, count,
+                     qq, first, cap, nrhs, active, coef);
+  return hipGetLastError();
+}
+
+template <typename S>
+hipError_t basis_new_pair(int64_t n, int nrhs, const double *y, const double *r,
+                          const double *coef, const S *const *qtab, const S *const *ptab,
+                          int first, int count, int cap, S *qk, S *pk, const int *active,
+                          double *qqrq, double *scratch, hipStream_t stream) {
+  if (count < 0 || count > kBasisMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_basis_new_pair<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n, y,
+                     r, coef, qtab, ptab, first, count, cap, qk, pk, active, scratch);
+  hipLaunchKernelGGL(k_dot_final, dim3(2 * nrhs), dim3(kDotBlocks), 0, stream, scratch, qqrq);
+  return hipGetLastError();
+}
+
+template <typename S>
+hipError_t basis_update(int64_t n, int nrhs, const S *pk, const S *qk, const double *qqrq,
+                        double *qq_slot, double *x, double *r, const int *active, double *rr,
+                        double *scratch, hipStream_t stream) {
+  hipLaunchKernelGGL(k_basis_update<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n, pk,
+                     qk, qqrq, qq_slot, x, r, active, scratch);
+  hipLaunchKernelGGL(k_dot_final, dim3(nrhs), dim3(kDotBlocks), 0, stream, scratch, rr);
+  return hipGetLastError();
+}
+
+template <typename S>
+hipError_t basis_project(int64_t n, int nrhs, const double *coef, const S *const *qtab,
+                         const S *const *ptab, int first, int count, int cap, double *x, double *r,
+                         const int *active, double *rr, double *scratch, hipStream_t stream) {
+  if (count < 0 || count > kBasisMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_basis_project<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n,
+                     coef, qtab, ptab, first, count, cap, x, r, active, scratch);
+  hipLaunchKernelGGL(k_dot_final, dim3(nrhs), dim3(kDotBlocks), 0, stream, scratch, rr);
+  return hipGetLastError();
+}
+
+#define FMCA_BASIS_INST(S)                                                                     \
+  template hipError_t basis_dots<S>(int64_t, int, const double *, const S *const *, int, int, \
+                                    int, const double *, const int *, double *, double *,     \
+                                    hipStream_t);                                             \
+  template hipError_t basis_new_pair<S>(int64_t, int, const double *, const double *,         \
+                                        const double *, const S *const *, const S *const *,   \
+                                        int, int, int, S *, S *, const int *, double *,       \
+                                        double *, hipStream_t);                               \
+  template hipError_t basis_update<S>(int64_t, int, const S *, const S *, const double *,     \
+                                      double *, double *, double *, const int *, double *,    \
+                                      double *, hipStream_t);                                 \
+  template hipError_t basis_project<S>(int64_t, int, const double *, const S *const *,        \
+                                       const S *const *, int, int, int, double *, double *,   \
+                                       const int *, double *, double *, hipStream_t);
+FMCA_BASIS_INST(double)
+FMCA_BASIS_INST(float)
+#undef FMCA_BASIS_INST
 
 hipError_t blas1_permute(int64_t n, int nrhs, const uint32_t *perm, bool gather, const double *in,
                          int64_t ldi, double *out, int64_t ldo, hipStream_t stream) {

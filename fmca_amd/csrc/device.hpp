@@ -259,6 +259,36 @@ hipError_t blas1_fill(int64_t n, int nrhs, double v, double *x, int64_t ldx, hip
 hipError_t blas1_permute(int64_t n, int nrhs, const uint32_t *perm, bool gather, const double *in,
                          int64_t ldi, double *out, int64_t ldo, hipStream_t stream);
 
+// Krylov basis of the GCR solve (blas1.hip): a circular window of `count`
+// pairs (p_j, q_j = (K + ridge I) p_j), window position j in slot
+// (first + j) % cap, slots of n x nrhs values of type S at ptab / qtab (device
+// arrays of slot pointers); qq[slot * nrhs + col] = q_slot . q_slot.
+// Columns with active[col] == 0 are left as they are.  Deterministic sums.
+constexpr int kBasisMax = 256;
+size_t basis_scratch(int nrhs, int count);
+// coef[col * count + j] = (v . q_j) / (q_j . q_j)
+template <typename S>
+hipError_t basis_dots(int64_t n, int nrhs, const double *v, const S *const *qtab, int first,
+                      int count, int cap, const double *qq, const int *active, double *coef,
+                      double *scratch, hipStream_t stream);
+// q = y - sum_j coef_j q_j, p = r - sum_j coef_j p_j into (qk, pk);
+// qqrq[2 col] = q.q, qqrq[2 col + 1] = r.q
+template <typename S>
+hipError_t basis_new_pair(int64_t n, int nrhs, const double *y, const double *r,
+                          const double *coef, const S *const *qtab, const S *const *ptab,
+                          int first, int count, int cap, S *qk, S *pk, const int *active,
+                          double *qqrq, double *scratch, hipStream_t stream);
+// x += a p_k, r -= a q_k with a = r.q / q.q; qq_slot[col] = q.q; rr[col] = r.r
+template <typename S>
+hipError_t basis_update(int64_t n, int nrhs, const S *pk, const S *qk, const double *qqrq,
+                        double *qq_slot, double *x, double *r, const int *active, double *rr,
+                        double *scratch, hipStream_t stream);
+// x += sum_j coef_j p_j, r -= sum_j coef_j q_j; rr[col] = r.r
+template <typename S>
+hipError_t basis_project(int64_t n, int nrhs, const double *coef, const S *const *qtab,
+                         const S *const *ptab, int first, int count, int cap, double *x, double *r,
+                         const int *active, double *rr, double *scratch, hipStream_t stream);
+
 // Evaluates the device exp (fn 0 / 2) or sqrt (fn 1 / 3) used by the kernel
 // loops, in FP64 (fn 0, 1) or FP32 (fn 2, 3), on n device values.
 hipError_t launch_math_probe(int fn, const double *x, double *y, int64_t n, hipStream_t stream);

@@ -208,8 +208,9 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
           "cg_device",
           [](Actor &a, uintptr_t b, int64_t ld_b, int64_t nrhs, uintptr_t x, int64_t ld_x,
              double ridge, double tol, int precision, double refine_factor, int max_iter,
-             int max_outer, uintptr_t stream) {
-            fmca_cg_options o{ridge, tol, precision, refine_factor, max_iter, max_outer};
+             int max_outer, int method, int max_basis, uintptr_t stream) {
+            fmca_cg_options o{ridge,    tol,       precision, refine_factor,
+                              max_iter, max_outer, method,    max_basis};
             fmca_cg_result r{};
             fmca_status st;
             {
@@ -232,7 +233,8 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
           },
           py::arg("b"), py::arg("ld_b"), py::arg("nrhs"), py::arg("x"), py::arg("ld_x"),
           py::arg("ridge"), py::arg("tol"), py::arg("precision"), py::arg("refine_factor"),
-          py::arg("max_iter"), py::arg("max_outer"), py::arg("stream") = 0)
+          py::arg("max_iter"), py::arg("max_outer"), py::arg("method") = 0,
+          py::arg("max_basis") = 0, py::arg("stream") = 0)
       .def("set_kernel",
            [](Actor &a, const std::string &ktype, double l, double c, double nu) {
              fmca_kernel_desc k{ktype.c_str(), l, c, nu};

@@ -226,26 +226,42 @@ fmca_status fmca_h2_action_transpose_device(fmca_h2_actor *actor, const double *
                                             int64_t ld_rhs, int64_t nrhs, double *d_out,
                                             int64_t ld_out, int precision, void *stream);
 
-/* Options of the CG solve of (K + ridge I) x = b, anchored on FMCA's pCG
- * (FMCA/src/util/PCG.h:15-44: identity preconditioner here, stop at
+/* Options of the iterative solve of (K + ridge I) x = b, anchored on FMCA's
+ * pCG (FMCA/src/util/PCG.h:15-44: identity preconditioner here, stop at
  * ||b - (K + ridge I) x|| / ||b|| <= tol, tol = 1e-8 in FMCA).  Vectors, inner
- * products and x updates are FP64 in both forms; precision selects the matvec
- * of the CG recurrence:
- *   FMCA_PRECISION_FP64  plain CG;
- *   FMCA_PRECISION_FP32  CG with reliable updates: FP32 matvecs drive the
- *                        recurrence; once every active column's iterated
- *                        residual has dropped by refine_factor since its last
- *                        replacement (or reached tol), the true residual is
- *                        recomputed with the FP64 action and replaces it, and
- *                        the search direction is re-orthogonalised against it.
+ * products and x updates are FP64 in every form; precision selects the matvec
+ * that drives the iteration:
+ *   FMCA_PRECISION_FP64       FP64 matvecs throughout;
+ *   FMCA_PRECISION_FP32(_BF16) lower-precision matvecs with reliable updates:
+ *                        once every active column's iterated residual has
+ *                        dropped by refine_factor since its last replacement
+ *                        (or reached tol), the true residual is recomputed
+ *                        with the FP64 action and replaces it.
+ * method:
+ *   FMCA_SOLVER_CG   FMCA's CG; on replacement the search direction is
+ *                    re-orthogonalised against the new residual;
+ *   FMCA_SOLVER_GCR  GCR (Eisenstat, Elman & Schultz 1983; no FMCA
+ *                    counterpart): the residual's image is orthogonalised
+ *                    against a window of up to max_basis stored pairs
+ *                    (p_j, (K + ridge I) p_j), and the residual minimised over
+ *                    them.  Each stored pair is consistent as its matvec
+ *                    computed it, so lower-precision matvecs do not delay
+ *                    convergence as they do CG's; a replaced residual is
+ *                    projected on the window.  Holds 2 max_basis vectors
+ *                    of n x nrhs (FP64 for FP64 matvecs, FP32 otherwise)
+ *                    during the solve.
  * Convergence is always decided on the FP64 residual. */
+enum { FMCA_SOLVER_CG = 0, FMCA_SOLVER_GCR = 1 };
 typedef struct {
   double ridge;
   double tol;
   int precision;
   double refine_factor; /* 0 < f < 1, default 0.1 */
-  int max_iter;         /* cap on CG iterations (matvecs of `precision`) */
+  int max_iter;         /* cap on iterations (matvecs of `precision`) */
   int max_outer;        /* cap on FP64 residual replacements */
+  int method;           /* FMCA_SOLVER_CG (0) or FMCA_SOLVER_GCR */
+  int max_basis;        /* GCR window, at most 256 pairs; 0: as many as fit in
+                           half the free device memory */
 } fmca_cg_options;
 
 typedef struct {
@@ -255,16 +271,17 @@ typedef struct {
   int matvecs_fp64;     /* FP64 actions */
   int matvecs_fp32;     /* FP32 actions */
   double rel_residual;  /* max over columns of ||b - (K + ridge I) x|| / ||b||, FP64 action,
-                           at the last replacement */
+                           at the last replacement, or of the returned x when a cap
+                           stopped the solve */
   double seconds;       /* wall time of the solve */
   int indefinite;       /* 1 if a column met p.(K + ridge I)p <= 0 (the H2 matrix plus ridge
                            is not positive definite); the solve then stops */
 } fmca_cg_result;
 
-/* CG solve of (K(P, P) + ridge I) x = b for nrhs independent columns
- * (requires m == n).  d_b: n x nrhs, d_x: n x nrhs initial guess on entry
- * (FMCA's pCG starts from x = b), solution on exit; device memory, natural
- * order.  Synchronises `stream`. */
+/* Solve of (K(P, P) + ridge I) x = b for nrhs independent columns (requires
+ * m == n) by opts->method.  d_b: n x nrhs, d_x: n x nrhs initial guess on
+ * entry (FMCA's pCG starts from x = b), solution on exit; device memory,
+ * natural order.  Synchronises `stream`. */
 fmca_status fmca_h2_cg_device(fmca_h2_actor *actor, const double *d_b, int64_t ld_b,
                               int64_t nrhs, double *d_x, int64_t ld_x,
                               const fmca_cg_options *opts, fmca_cg_result *result, void *stream);

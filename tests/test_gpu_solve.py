@@ -157,3 +157,97 @@ def test_clustered_configuration_needs_a_ridge_above_the_h2_error():
         diff = (torch.linalg.norm(x - x64) / torch.linalg.norm(x64)).item()
         print(f"  solution difference {diff:.2e}", flush=True)
         assert diff <= 1e-4
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32", "FP32_BF16"])
+def test_gcr_reaches_the_fp64_cg_solution(precision):
+    """GCR with each matvec precision reaches the 1e-10 FP64 residual, checked
+    again through the FP64 action, and FP64 CG's solution.  With FP32-class
+    matvecs it needs about as many iterations as with FP64 ones: every stored
+    pair (p, (K + ridge I) p) is consistent as its own matvec computed it,
+    whereas CG's recurrence loses conjugacy under the changing rounding."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print(f"[case] gcr n=20000 8 rhs {precision}", flush=True)
+    n, nrhs, ridge = 20000, 8, 20.0
+    P = W.uniform(n, 3, 3)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.5), P, P, 3, 0.8)
+    b = torch.from_numpy(W.normal_rhs(n, nrhs, 4)).cuda()
+    z = torch.zeros_like(b)
+    x64, i64 = H.cg(b, x0=z, ridge=ridge, tol=1e-10, precision=fmca_amd.FP64)
+    _, g64 = H.cg(b, x0=z, ridge=ridge, tol=1e-10, precision=fmca_amd.FP64, method="gcr")
+    x, info = H.cg(b, x0=z, ridge=ridge, tol=1e-10, precision=getattr(fmca_amd, precision),
+                   method="gcr")
+    print(f"  cg fp64 {i64['iterations']} it, gcr fp64 {g64['iterations']} it, "
+          f"gcr {precision} {info}", flush=True)
+    assert info["converged"] and info["rel_residual"] <= 1e-10
+    assert _resid(H, x, b, ridge) <= 1e-10
+    assert (torch.linalg.norm(x - x64) / torch.linalg.norm(x64)).item() <= 1e-8
+    if precision != "FP64":
+        assert info["matvecs_fp32"] == info["iterations"] > 0
+        assert info["iterations"] <= 1.25 * g64["iterations"] + 2
+
+
+def test_gcr_window_smaller_than_the_iteration_count():
+    """With room for 3 and 8 pairs the window wraps (the oldest pair is
+    dropped) and the solve still reaches tol; an oversized window is refused."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    print("[case] gcr small windows", flush=True)
+    n, ridge = 8000, 1e-1
+    P = W.uniform(n, 2, 3)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.3), P, P, 5, 0.8)
+    b = torch.from_numpy(W.normal_rhs(n, 3, 4)).cuda()
+    x64, i64 = H.cg(b, ridge=ridge, tol=1e-10, precision=fmca_amd.FP64)
+    for m in (3, 8):
+        x, info = H.cg(b, ridge=ridge, tol=1e-10, precision=fmca_amd.FP32, method="gcr",
+                       max_basis=m, max_iter=5000)
+        print(f"  window {m}: {info}", flush=True)
+        assert info["converged"] and info["iterations"] > m
+        assert _resid(H, x, b, ridge) <= 1e-10
+        assert rel_err(x.cpu().numpy(), x64.cpu().numpy()) < 1e-8
+    with pytest.raises(ValueError):
+        H.cg(b, ridge=ridge, precision=fmca_amd.FP32, method="gcr", max_basis=257)
+    with pytest.raises(ValueError):
+        H.cg(b, ridge=ridge, precision=fmca_amd.FP32, method="minres")
+
+
+def test_gcr_single_column_and_zero_rhs():
+    """A zero column stays zero and stops at once; a 1-D rhs returns 1-D."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    n = 4000
+    P = W.uniform(n, 3, 7)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN52", 0.3), P, P, 3, 0.8)
+    b = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    b[:, 1] = torch.from_numpy(W.normal_rhs(n, 1, 8)[:, 0]).cuda()
+    x, info = H.cg(b, x0=torch.zeros_like(b), ridge=1.0, tol=1e-9,
+                   precision=fmca_amd.FP32_BF16, method="gcr")
+    assert info["converged"] and torch.all(x[:, 0] == 0)
+    assert _resid(H, x[:, 1:].contiguous(), b[:, 1:].contiguous(), 1.0) <= 1e-9
+    x1, info1 = H.cg(b[:, 1].contiguous(), ridge=1.0, tol=1e-9, precision=fmca_amd.FP64,
+                     method="gcr")
+    assert info1["converged"] and x1.dim() == 1
+    assert _resid(H, x1.reshape(-1, 1), b[:, 1:].contiguous(), 1.0) <= 1e-9
+
+
+@pytest.mark.parametrize("method", ["cg", "gcr"])
+def test_solve_stopped_by_a_cap_reports_the_residual_of_its_x(method):
+    """A solve cut off by max_iter is not converged and reports the FP64
+    residual of the x it returns, not the last replaced one."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    n, ridge = 6000, 1e-2
+    P = W.uniform(n, 3, 9)
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel("MATERN32", 0.4), P, P, 3, 0.8)
+    b = torch.from_numpy(W.normal_rhs(n, 2, 10)).cuda()
+    x, info = H.cg(b, x0=torch.zeros_like(b), ridge=ridge, tol=1e-12,
+                   precision=fmca_amd.FP32, method=method, max_iter=4)
+    print(f"  {method}: {info}", flush=True)
+    assert not info["converged"] and info["iterations"] == 4
+    r = _resid(H, x, b, ridge)
+    assert r < 1 and abs(info["rel_residual"] - r) <= 1e-6 * r
