@@ -61,7 +61,7 @@ def parse():
     p.add_argument("--solve-method", default="gcr", choices=["gcr", "cg"],
                    help="method of the refined solve (FP64 residual replacement in both)")
     p.add_argument("--refine-factor", type=float, default=None,
-                   help="residual drop between FP64 replacements (default 0.01 for gcr, "
+                   help="residual drop between FP64 replacements (default 1e-3 for gcr, "
                         "0.1 for cg)")
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
@@ -606,7 +606,7 @@ def main():
     # the headline: refined solves (timed steps); then, once each, FMCA's plain
     # FP64 CG (whose iteration count the value credits), the same method in
     # FP64, and the other method with the same matvecs
-    factor = args.refine_factor or (0.01 if args.solve_method == "gcr" else 0.1)
+    factor = args.refine_factor or (1e-3 if args.solve_method == "gcr" else 0.1)
     t_ref, a_ref, infos, ph_solve = refined_solves(H, x, args.ridge, args.steps, args.warmup,
                                                    solve_prec, args.solve_method, factor)
     t_64, a_64, info64 = once_solve(H, x, args.ridge, fmca_amd.FP64)
@@ -617,7 +617,7 @@ def main():
                              if args.solve_method == "gcr" else (t_64, None, info64))
     del a_64m
     t_oth, a_oth, info_oth = once_solve(H, x, args.ridge, solve_prec, other,
-                                        0.1 if other == "cg" else 0.01)
+                                        0.1 if other == "cg" else 1e-3)
     oth_diff = float(torch.linalg.norm(a_oth - a_64) / torch.linalg.norm(a_64))
     del a_oth, a_64
     iters64 = info64["iterations"]

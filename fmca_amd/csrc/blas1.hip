@@ -147,9 +147,45 @@ __device__ __forceinline__ double block_sums(double (&s)[N], double *lds /* [kTh
   return t;
 }
 
+// Rows of block b: a contiguous range, [b * chunk, min(n, (b + 1) * chunk)),
+// chunk a multiple of 4 * kThreads; V = 4 reads four consecutive rows per
+// thread (n % 4 == 0, so every column starts 16-byte aligned), V = 1 one.
+template <int V>
+__device__ __forceinline__ void block_rows(int64_t n, int64_t &lo, int64_t &hi) {
+  const int64_t per = (n + kDotBlocks - 1) / kDotBlocks;
+  const int64_t chunk = (per + 4 * kThreads - 1) / (4 * kThreads) * (4 * kThreads);
+  lo = blockIdx.x * chunk;
+  hi = lo + chunk < n ? lo + chunk : n;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void load_v(const T *__restrict__ p, int64_t i, double (&out)[V]) {
+  if constexpr (V == 4) {
+    struct alignas(4 * sizeof(T)) T4 { T v[4]; };
+    const T4 t = *reinterpret_cast<const T4 *>(p + i);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[u] = (double)t.v[u];
+  } else {
+    out[0] = (double)p[i];
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_v(T *__restrict__ p, int64_t i, const T (&in)[V]) {
+  if constexpr (V == 4) {
+    struct alignas(4 * sizeof(T)) T4 { T v[4]; };
+    T4 t;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t.v[u] = in[u];
+    *reinterpret_cast<T4 *>(p + i) = t;
+  } else {
+    p[i] = in[0];
+  }
+}
+
 // partial[(col * count + j) * kDotBlocks + block] = sum over the block's rows of
 // v * q_j (v: n x nrhs FP64)
-template <typename S>
+template <typename S, int V>
 __global__ __launch_bounds__(kThreads) void k_basis_dots(int64_t n, const double *__restrict__ v,
                                                          const S *const *__restrict__ qtab,
                                                          int first, int count, int cap,
@@ -158,6 +194,8 @@ __global__ __launch_bounds__(kThreads) void k_basis_dots(int64_t n, const double
   const int col = blockIdx.y;
   const bool on = active[col] != 0;
   const double *vc = v + (size_t)col * n;
+  int64_t lo, hi;
+  block_rows<V>(n, lo, hi);
   __shared__ double lds[(kThreads / 64) * kBasisGroup];
   for (int g0 = 0; g0 < count; g0 += kBasisGroup) {
     const int ng = min(kBasisGroup, count - g0);
@@ -169,12 +207,17 @@ __global__ __launch_bounds__(kThreads) void k_basis_dots(int64_t n, const double
 #pragma unroll
     for (int k = 0; k < kBasisGroup; ++k) s[k] = 0;
     if (on)
-      for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-           i += (int64_t)kDotBlocks * kThreads) {
-        const double vi = vc[i];
+      for (int64_t i = lo + (int64_t)threadIdx.x * V; i < hi; i += (int64_t)kThreads * V) {
+        double vi[V];
+        load_v<double, V>(vc, i, vi);
 #pragma unroll
         for (int k = 0; k < kBasisGroup; ++k)
-          if (k < ng) s[k] = __builtin_fma(vi, (double)q[k][i], s[k]);
+          if (k < ng) {
+            double qi[V];
+            load_v<S, V>(q[k], i, qi);
+#pragma unroll
+            for (int u = 0; u < V; ++u) s[k] = __builtin_fma(vi[u], qi[u], s[k]);
+          }
       }
     const double t = block_sums(s, lds);
     if (threadIdx.x < ng)
@@ -206,7 +249,7 @@ __global__ __launch_bounds__(kDotBlocks) void k_basis_coef(const double *__restr
 // The new pair from q0 = (K + ridge I) r (in y) and p0 = r, orthogonalised
 // against the window: q = q0 - sum_j c_j q_j, p = r - sum_j c_j p_j, stored in
 // slot (qk, pk); partial sums of q.q and r.q (of the stored q) per block.
-template <typename S>
+template <typename S, int V>
 __global__ __launch_bounds__(kThreads) void k_basis_new_pair(
     int64_t n, const double *__restrict__ y, const double *__restrict__ r,
     const double *__restrict__ coef, const S *const *__restrict__ qtab,
@@ -225,24 +268,40 @@ __global__ __launch_bounds__(kThreads) void k_basis_new_pair(
     ps[j] = ptab[sl] + (size_t)col * n;
   }
   __syncthreads();
+  int64_t lo, hi;
+  block_rows<V>(n, lo, hi);
   double s[2] = {0, 0};
   if (on) {
     const double *yc = y + (size_t)col * n, *rc = r + (size_t)col * n;
     S *qc = qk + (size_t)col * n, *pc = pk + (size_t)col * n;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)kDotBlocks * kThreads) {
-      const double ri = rc[i];
-      double q = yc[i], p = ri;
+    for (int64_t i = lo + (int64_t)threadIdx.x * V; i < hi; i += (int64_t)kThreads * V) {
+      double q[V], p[V];
+      load_v<double, V>(yc, i, q);
+      load_v<double, V>(rc, i, p);
+      double ri[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) ri[u] = p[u];
       for (int j = 0; j < count; ++j) {
-        q = __builtin_fma(-c[j], (double)qs[j][i], q);
-        p = __builtin_fma(-c[j], (double)ps[j][i], p);
+        double qj[V], pj[V];
+        load_v<S, V>(qs[j], i, qj);
+        load_v<S, V>(ps[j], i, pj);
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          q[u] = __builtin_fma(-c[j], qj[u], q[u]);
+          p[u] = __builtin_fma(-c[j], pj[u], p[u]);
+        }
       }
-      const S qs_ = (S)q;
-      qc[i] = qs_;
-      pc[i] = (S)p;
-      const double qd = (double)qs_;
-      s[0] = __builtin_fma(qd, qd, s[0]);
-      s[1] = __builtin_fma(ri, qd, s[1]);
+      S qo[V], po[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        qo[u] = (S)q[u];
+        po[u] = (S)p[u];
+        const double qd = (double)qo[u];
+        s[0] = __builtin_fma(qd, qd, s[0]);
+        s[1] = __builtin_fma(ri[u], qd, s[1]);
+      }
+      store_v<S, V>(qc, i, qo);
+      store_v<S, V>(pc, i, po);
     }
   }
   const double t = block_sums(s, lds);
@@ -251,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void k_basis_new_pair(
 
 // x += a p, r -= a q with a = (r.q) / (q.q) from qqrq[2 col + {0, 1}];
 // records q.q of the new slot; partial sums of the new r.r per block
-template <typename S>
+template <typename S, int V>
 __global__ __launch_bounds__(kThreads) void k_basis_update(
     int64_t n, const S *__restrict__ pk, const S *__restrict__ qk,
     const double *__restrict__ qqrq, double *__restrict__ qq_slot, double *__restrict__ x,
@@ -259,18 +318,28 @@ __global__ __launch_bounds__(kThreads) void k_basis_update(
   const int col = blockIdx.y;
   __shared__ double lds[kThreads / 64];
   double s[1] = {0};
+  int64_t lo, hi;
+  block_rows<V>(n, lo, hi);
   if (active[col]) {
     const double qq = qqrq[2 * col], rq = qqrq[2 * col + 1];
     const double a = qq > 0 ? rq / qq : 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) qq_slot[col] = qq;
     const S *pc = pk + (size_t)col * n, *qc = qk + (size_t)col * n;
     double *xc = x + (size_t)col * n, *rc = r + (size_t)col * n;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)kDotBlocks * kThreads) {
-      xc[i] = __builtin_fma(a, (double)pc[i], xc[i]);
-      const double rn = __builtin_fma(-a, (double)qc[i], rc[i]);
-      rc[i] = rn;
-      s[0] = __builtin_fma(rn, rn, s[0]);
+    for (int64_t i = lo + (int64_t)threadIdx.x * V; i < hi; i += (int64_t)kThreads * V) {
+      double pv[V], qv[V], xv[V], rv[V];
+      load_v<S, V>(pc, i, pv);
+      load_v<S, V>(qc, i, qv);
+      load_v<double, V>(xc, i, xv);
+      load_v<double, V>(rc, i, rv);
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        xv[u] = __builtin_fma(a, pv[u], xv[u]);
+        rv[u] = __builtin_fma(-a, qv[u], rv[u]);
+        s[0] = __builtin_fma(rv[u], rv[u], s[0]);
+      }
+      store_v<double, V>(xc, i, xv);
+      store_v<double, V>(rc, i, rv);
     }
   }
   const double t = block_sums(s, lds);
@@ -278,7 +347,7 @@ __global__ __launch_bounds__(kThreads) void k_basis_update(
 }
 
 // x += sum_j c_j p_j, r -= sum_j c_j q_j; partial sums of the new r.r per block
-template <typename S>
+template <typename S, int V>
 __global__ __launch_bounds__(kThreads) void k_basis_project(
     int64_t n, const double *__restrict__ coef, const S *const *__restrict__ qtab,
     const S *const *__restrict__ ptab, int first, int count, int cap, double *__restrict__ x,
@@ -295,19 +364,29 @@ __global__ __launch_bounds__(kThreads) void k_basis_project(
     ps[j] = ptab[sl] + (size_t)col * n;
   }
   __syncthreads();
+  int64_t lo, hi;
+  block_rows<V>(n, lo, hi);
   double s[1] = {0};
   if (active[col]) {
     double *xc = x + (size_t)col * n, *rc = r + (size_t)col * n;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)kDotBlocks * kThreads) {
-      double xi = xc[i], ri = rc[i];
+    for (int64_t i = lo + (int64_t)threadIdx.x * V; i < hi; i += (int64_t)kThreads * V) {
+      double xv[V], rv[V];
+      load_v<double, V>(xc, i, xv);
+      load_v<double, V>(rc, i, rv);
       for (int j = 0; j < count; ++j) {
-        xi = __builtin_fma(c[j], (double)ps[j][i], xi);
-        ri = __builtin_fma(-c[j], (double)qs[j][i], ri);
+        double qj[V], pj[V];
+        load_v<S, V>(qs[j], i, qj);
+        load_v<S, V>(ps[j], i, pj);
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          xv[u] = __builtin_fma(c[j], pj[u], xv[u]);
+          rv[u] = __builtin_fma(-c[j], qj[u], rv[u]);
+        }
       }
-      xc[i] = xi;
-      rc[i] = ri;
-      s[0] = __builtin_fma(ri, ri, s[0]);
+#pragma unroll
+      for (int u = 0; u < V; ++u) s[0] = __builtin_fma(rv[u], rv[u], s[0]);
+      store_v<double, V>(xc, i, xv);
+      store_v<double, V>(rc, i, rv);
     }
   }
   const double t = block_sums(s, lds);
@@ -375,8 +454,12 @@ hipError_t basis_dots(int64_t n, int nrhs, const double *v, const S *const *qtab
                       double *scratch, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
   if (count > kBasisMax) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_basis_dots<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n, v,
-                     qtab, first, count, cap, active, scratch);
+  if (n % 4 == 0)
+    hipLaunchKernelGGL((k_basis_dots<S, 4>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream,
+                       n, v, qtab, first, count, cap, active, scratch);
+  else
+    hipLaunchKernelGGL((k_basis_dots<S, 1>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream,
+                       n, v, qtab, first, count, cap, active, scratch);
   hipLaunchKernelGGL(k_basis_coef, dim3(count, nrhs), dim3(kDotBlocks), 0, stream, scratch, count,
                      qq, first, cap, nrhs, active, coef);
   return hipGetLastError();
@@ -388,8 +471,14 @@ hipError_t basis_new_pair(int64_t n, int nrhs, const double *y, const double *r,
                           int first, int count, int cap, S *qk, S *pk, const int *active,
                           double *qqrq, double *scratch, hipStream_t stream) {
   if (count < 0 || count > kBasisMax) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_basis_new_pair<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n, y,
-                     r, coef, qtab, ptab, first, count, cap, qk, pk, active, scratch);
+  if (n % 4 == 0)
+    hipLaunchKernelGGL((k_basis_new_pair<S, 4>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0,
+                       stream, n, y, r, coef, qtab, ptab, first, count, cap, qk, pk, active,
+                       scratch);
+  else
+    hipLaunchKernelGGL((k_basis_new_pair<S, 1>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0,
+                       stream, n, y, r, coef, qtab, ptab, first, count, cap, qk, pk, active,
+                       scratch);
   hipLaunchKernelGGL(k_dot_final, dim3(2 * nrhs), dim3(kDotBlocks), 0, stream, scratch, qqrq);
   return hipGetLastError();
 }
@@ -398,8 +487,12 @@ template <typename S>
 hipError_t basis_update(int64_t n, int nrhs, const S *pk, const S *qk, const double *qqrq,
                         double *qq_slot, double *x, double *r, const int *active, double *rr,
                         double *scratch, hipStream_t stream) {
-  hipLaunchKernelGGL(k_basis_update<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n, pk,
-                     qk, qqrq, qq_slot, x, r, active, scratch);
+  if (n % 4 == 0)
+    hipLaunchKernelGGL((k_basis_update<S, 4>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream,
+                       n, pk, qk, qqrq, qq_slot, x, r, active, scratch);
+  else
+    hipLaunchKernelGGL((k_basis_update<S, 1>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream,
+                       n, pk, qk, qqrq, qq_slot, x, r, active, scratch);
   hipLaunchKernelGGL(k_dot_final, dim3(nrhs), dim3(kDotBlocks), 0, stream, scratch, rr);
   return hipGetLastError();
 }
@@ -409,8 +502,12 @@ hipError_t basis_project(int64_t n, int nrhs, const double *coef, const S *const
                          const S *const *ptab, int first, int count, int cap, double *x, double *r,
                          const int *active, double *rr, double *scratch, hipStream_t stream) {
   if (count < 0 || count > kBasisMax) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_basis_project<S>, dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream, n,
-                     coef, qtab, ptab, first, count, cap, x, r, active, scratch);
+  if (n % 4 == 0)
+    hipLaunchKernelGGL((k_basis_project<S, 4>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream,
+                       n, coef, qtab, ptab, first, count, cap, x, r, active, scratch);
+  else
+    hipLaunchKernelGGL((k_basis_project<S, 1>), dim3(kDotBlocks, nrhs), dim3(kThreads), 0, stream,
+                       n, coef, qtab, ptab, first, count, cap, x, r, active, scratch);
   hipLaunchKernelGGL(k_dot_final, dim3(nrhs), dim3(kDotBlocks), 0, stream, scratch, rr);
   return hipGetLastError();
 }

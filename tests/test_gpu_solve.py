@@ -237,7 +237,8 @@ def test_gcr_single_column_and_zero_rhs():
 @pytest.mark.parametrize("method", ["cg", "gcr"])
 def test_solve_stopped_by_a_cap_reports_the_residual_of_its_x(method):
     """A solve cut off by max_iter is not converged and reports the FP64
-    residual of the x it returns, not the last replaced one."""
+    residual of the x it returns, not the last replaced one.  (GCR's residual
+    norm never grows; CG's may, as CG minimises the error in the energy norm.)"""
     import torch
     import fmca_amd
     from fmca_amd import workloads as W
@@ -250,4 +251,6 @@ def test_solve_stopped_by_a_cap_reports_the_residual_of_its_x(method):
     print(f"  {method}: {info}", flush=True)
     assert not info["converged"] and info["iterations"] == 4
     r = _resid(H, x, b, ridge)
-    assert r < 1 and abs(info["rel_residual"] - r) <= 1e-6 * r
+    assert abs(info["rel_residual"] - r) <= 1e-6 * r
+    if method == "gcr":
+        assert r < 1
