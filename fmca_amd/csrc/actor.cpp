@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -119,8 +120,11 @@ struct DevTreeReals {
 struct DevPattern {
   DevBuf far_ptr, far_src, far_order, far_task, far_split;
   DevBuf near_ptr, near_src, near_run, near_order, near_task, near_split;
+  int far_top_target = 0, far_top_source = 0;
   DevPatternView view(const Pattern &p) const {
     DevPatternView v;
+    v.far_top_target = far_top_target;
+    v.far_top_source = far_top_source;
     v.far_ptr = far_ptr.as<int32_t>();
     v.far_src = far_src.as<int32_t>();
     v.nfar_order = (int)p.far_order.size();
@@ -141,6 +145,14 @@ struct DevPattern {
   }
   hipError_t upload(const Pattern &p) {
     hipError_t e;
+    // first far target and source (INT_MAX: no far blocks at all)
+    far_top_target = far_top_source = std::numeric_limits<int>::max();
+    for (size_t t = 0; t + 1 < p.far_ptr.size(); ++t)
+      if (p.far_ptr[t + 1] > p.far_ptr[t]) {
+        far_top_target = (int)t;
+        break;
+      }
+    for (int32_t src : p.far_src) far_top_source = std::min(far_top_source, (int)src);
     if ((e = far_ptr.upload(p.far_ptr)) != hipSuccess) return e;
     if ((e = far_src.upload(p.far_src)) != hipSuccess) return e;
     if ((e = far_order.upload(p.far_order)) != hipSuccess) return e;

@@ -52,6 +52,10 @@ struct DevPatternView {
   const int32_t *near_task = nullptr;   // target leaf index, near list range, flags
   int nnear_split = 0;
   const int32_t *near_split = nullptr;  // target leaf indices with several tasks
+  // smallest target / source cluster id of any far block (ids are level
+  // ordered, so the tree passes above their levels carry only zeros); 0: no
+  // information, the passes run over every level
+  int far_top_target = 0, far_top_source = 0;
 };
 
 // Interpolation data of the kernels.  Tensor scheme: the per-axis Chebyshev

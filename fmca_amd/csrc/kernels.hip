@@ -26,6 +26,15 @@ namespace {
 
 using namespace dev;
 
+// x[0, count) = 0 (a fill kernel, not a memset, so that captured graphs
+// replay it like any other launch)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero_range(int64_t count, T *__restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = T(0);
+}
+
 // ---------------------------------------------------------------------------
 // 1. rhs (column-major, natural order) -> xs (cluster order, point-major).
 // A workgroup moves kPermRows natural rows in column chunks of kPermCols
@@ -200,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_upward_transfer(DevTreeView<T> tr, I
   T *O = B + q * NRC;
   T *Tm = O + q * NRC;  // D x P1 x P1
   fill_moment_index<D>(aidx, q, P1);
-  for (int rc = 0; rc < nrhs; rc += NRC) {
+  for (int rc = blockIdx.y * NRC; rc < nrhs; rc += gridDim.y * NRC) {
     const int nr = min(NRC, nrhs - rc);
     for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) O[e] = 0;
     for (int si = 0; si < 2; ++si) {
@@ -378,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_downward_transfer(DevTreeView<T> tr,
   fill_moment_index<D>(aidx, q, P1);
   for (int e = threadIdx.x; e < D * P1 * P1; e += blockDim.x)
     Tm[e] = tr.transfer[(int64_t)c * D * P1 * P1 + e];
-  for (int rc = 0; rc < nrhs; rc += NRC) {
+  for (int rc = blockIdx.y * NRC; rc < nrhs; rc += gridDim.y * NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
     for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
@@ -437,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_upward_transfer_dense(DevTreeView<T>
     E0 = Es;
     E1 = Es + q * q;
   }
-  for (int rc = 0; rc < nrhs; rc += NRC) {
+  for (int rc = blockIdx.y * NRC; rc < nrhs; rc += gridDim.y * NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
     for (int e = threadIdx.x; e < 2 * q * NRC; e += blockDim.x) {
@@ -473,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_downward_transfer_dense(DevTreeView<
     for (int e = threadIdx.x; e < q * q; e += blockDim.x) Es[e] = E[e];
     E = Es;
   }
-  for (int rc = 0; rc < nrhs; rc += NRC) {
+  for (int rc = blockIdx.y * NRC; rc < nrhs; rc += gridDim.y * NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
     for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
@@ -508,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_cluster_basis_change(int q, const T 
     for (int e = threadIdx.x; e < q * q; e += blockDim.x) Vs[e] = invV[e];
     invV = Vs;
   }
-  for (int rc = 0; rc < nrhs; rc += NRC) {
+  for (int rc = blockIdx.y * NRC; rc < nrhs; rc += gridDim.y * NRC) {
     const int nr = min(NRC, nrhs - rc);
     __syncthreads();
     for (int e = threadIdx.x; e < q * NRC; e += blockDim.x) {
@@ -812,24 +821,39 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
     }
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  // 3. upward transfers, deepest inner level first; columns per pass: the
-  // largest of 32, 8, 1 whose staging fits 64 KB of LDS
+  // 3. upward transfers, deepest inner level first, up to the level of the
+  // first far source: the moments above it are never read (zero-filled, so
+  // that every cluster's moments stay finite).  Columns per workgroup: 8 (or
+  // 1) with the column groups in grid y, so that a level's latency is that of
+  // one 8-column group, not of all columns in one workgroup.
   mark(PH_UPWARD_TRANSFER);
+  auto first_level = [](const HostLevels &lv, int id) {
+    int L = 0;
+    while (L <= lv.depth && lv.level_ptr[L + 1] <= id) ++L;
+    return L;  // depth + 1: no cluster from id on
+  };
+  const int top_src = first_level(lcols, pat.far_top_source);
+  const int top_tgt = first_level(lrows, pat.far_top_target);
+  if (top_src > 0) {
+    const int64_t count = (int64_t)lcols.level_ptr[std::min(top_src, lcols.depth + 1)] * q * nrhs;
+    hipLaunchKernelGGL((k_zero_range<T>), dim3((unsigned)std::min<int64_t>(
+                           1024, std::max<int64_t>(1, (count + kBlock - 1) / kBlock))),
+                       dim3(kBlock), 0, stream, count, buf.mom);
+  }
   auto transfer_lds = [&](int arrays, int nrc) {
     return (size_t)((q + 3) & ~3) * sizeof(int) +
            ((size_t)arrays * q * nrc + (size_t)D * P1 * P1) * sizeof(T);
   };
   auto transfer_nrc = [&](int arrays) {
-    if (nrhs > 8 && transfer_lds(arrays, 32) <= kLdsLimit) return 32;
     if (nrhs > 1 && transfer_lds(arrays, 8) <= kLdsLimit) return 8;
     return 1;
   };
   // total degree: dense transfers and basis changes, [arrays][q][nrc] in LDS
   auto dense_nrc = [&](int arrays) {
-    if (nrhs > 8 && (size_t)arrays * q * 32 * sizeof(T) <= kLdsLimit) return 32;
     if (nrhs > 1 && (size_t)arrays * q * 8 * sizeof(T) <= kLdsLimit) return 8;
     return 1;
   };
+  auto groups = [&](int nrc) { return (unsigned)((nrhs + nrc - 1) / nrc); };
   auto dense_lds = [&](int arrays, int nrc) { return (size_t)arrays * q * nrc * sizeof(T); };
   // the q x q matrices (E, invV) go to LDS too where that fits
   auto mats_fit = [&](int arrays, int nrc) {
@@ -843,47 +867,38 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
     const int nu = dense_nrc(2);
     if (dense_lds(2, nu) > kLdsLimit) return hipErrorInvalidConfiguration;
     const int se = mats_fit(2, nu) ? 1 : 0;
-    for (int L = lcols.depth - 1; L >= 0; --L) {
+    for (int L = lcols.depth - 1; L >= top_src; --L) {
       const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
-      if (nu == 32)
-        hipLaunchKernelGGL((k_upward_transfer_dense<T, 32>), dim3(nc), dim3(kBlock),
-                           with_mats(2, 32), stream, cols, q, cb, nrhs, se, buf.mom);
-      else if (nu == 8)
-        hipLaunchKernelGGL((k_upward_transfer_dense<T, 8>), dim3(nc), dim3(kBlock),
+      if (nu == 8)
+        hipLaunchKernelGGL((k_upward_transfer_dense<T, 8>), dim3(nc, groups(8)), dim3(kBlock),
                            with_mats(2, 8), stream, cols, q, cb, nrhs, se, buf.mom);
       else
-        hipLaunchKernelGGL((k_upward_transfer_dense<T, 1>), dim3(nc), dim3(kBlock),
+        hipLaunchKernelGGL((k_upward_transfer_dense<T, 1>), dim3(nc, groups(1)), dim3(kBlock),
                            with_mats(2, 1), stream, cols, q, cb, nrhs, se, buf.mom);
     }
   }
   auto basis_change = [&](int ncl, int transpose, T *x) {
     const int nb = dense_nrc(1);
     const int se = mats_fit(1, nb) ? 1 : 0;
-    if (nb == 32)
-      hipLaunchKernelGGL((k_cluster_basis_change<T, 32>), dim3(ncl), dim3(kBlock),
-                         with_mats(1, 32), stream, q, ip.invV, transpose, nrhs, se, x);
-    else if (nb == 8)
-      hipLaunchKernelGGL((k_cluster_basis_change<T, 8>), dim3(ncl), dim3(kBlock),
+    if (nb == 8)
+      hipLaunchKernelGGL((k_cluster_basis_change<T, 8>), dim3(ncl, groups(8)), dim3(kBlock),
                          with_mats(1, 8), stream, q, ip.invV, transpose, nrhs, se, x);
     else
-      hipLaunchKernelGGL((k_cluster_basis_change<T, 1>), dim3(ncl), dim3(kBlock),
+      hipLaunchKernelGGL((k_cluster_basis_change<T, 1>), dim3(ncl, groups(1)), dim3(kBlock),
                          with_mats(1, 1), stream, q, ip.invV, transpose, nrhs, se, x);
   };
   const int nrc_up = transfer_nrc(3), nrc_dn = transfer_nrc(2);
   if (!ip.td && (transfer_lds(3, nrc_up) > kLdsLimit || transfer_lds(2, nrc_dn) > kLdsLimit))
     return hipErrorInvalidConfiguration;
-  for (int L = lcols.depth - 1; !ip.td && L >= 0; --L) {
+  for (int L = lcols.depth - 1; !ip.td && L >= top_src; --L) {
     const int cb = lcols.level_ptr[L], nc = lcols.level_ptr[L + 1] - cb;
     const size_t lds = transfer_lds(3, nrc_up);
-    if (nrc_up == 32)
-      hipLaunchKernelGGL((k_upward_transfer<T, D, 32>), dim3(nc), dim3(kBlock), lds, stream, cols,
-                         ip, cb, nrhs, buf.mom);
-    else if (nrc_up == 8)
-      hipLaunchKernelGGL((k_upward_transfer<T, D, 8>), dim3(nc), dim3(kBlock), lds, stream, cols,
-                         ip, cb, nrhs, buf.mom);
+    if (nrc_up == 8)
+      hipLaunchKernelGGL((k_upward_transfer<T, D, 8>), dim3(nc, groups(8)), dim3(kBlock), lds,
+                         stream, cols, ip, cb, nrhs, buf.mom);
     else
-      hipLaunchKernelGGL((k_upward_transfer<T, D, 1>), dim3(nc), dim3(kBlock), lds, stream, cols,
-                         ip, cb, nrhs, buf.mom);
+      hipLaunchKernelGGL((k_upward_transfer<T, D, 1>), dim3(nc, groups(1)), dim3(kBlock), lds,
+                         stream, cols, ip, cb, nrhs, buf.mom);
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 4. coupling (every target cluster is written, empty far lists give zeros)
@@ -909,36 +924,32 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
       return err;
   }
   if (ip.td) basis_change(rows.ncl, 0, buf.tlhs);
-  // 5. downward transfers, top level first
+  // 5. downward transfers, top level first, from the level below the first
+  // far target on (every cluster above it holds zeros after the coupling)
   mark(PH_DOWNWARD_TRANSFER);
+  const int first_dn = std::max(1, top_tgt + 1);
   if (ip.td) {
     const int nd = dense_nrc(1);
     const int se = mats_fit(1, nd) ? 1 : 0;
-    for (int L = 1; L <= lrows.depth; ++L) {
+    for (int L = first_dn; L <= lrows.depth; ++L) {
       const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
-      if (nd == 32)
-        hipLaunchKernelGGL((k_downward_transfer_dense<T, 32>), dim3(nc), dim3(kBlock),
-                           with_mats(1, 32), stream, rows, q, cb, nrhs, se, buf.tlhs);
-      else if (nd == 8)
-        hipLaunchKernelGGL((k_downward_transfer_dense<T, 8>), dim3(nc), dim3(kBlock),
+      if (nd == 8)
+        hipLaunchKernelGGL((k_downward_transfer_dense<T, 8>), dim3(nc, groups(8)), dim3(kBlock),
                            with_mats(1, 8), stream, rows, q, cb, nrhs, se, buf.tlhs);
       else
-        hipLaunchKernelGGL((k_downward_transfer_dense<T, 1>), dim3(nc), dim3(kBlock),
+        hipLaunchKernelGGL((k_downward_transfer_dense<T, 1>), dim3(nc, groups(1)), dim3(kBlock),
                            with_mats(1, 1), stream, rows, q, cb, nrhs, se, buf.tlhs);
     }
   }
-  for (int L = 1; !ip.td && L <= lrows.depth; ++L) {
+  for (int L = first_dn; !ip.td && L <= lrows.depth; ++L) {
     const int cb = lrows.level_ptr[L], nc = lrows.level_ptr[L + 1] - cb;
     const size_t lds = transfer_lds(2, nrc_dn);
-    if (nrc_dn == 32)
-      hipLaunchKernelGGL((k_downward_transfer<T, D, 32>), dim3(nc), dim3(kBlock), lds, stream,
-                         rows, ip, cb, nrhs, buf.tlhs);
-    else if (nrc_dn == 8)
-      hipLaunchKernelGGL((k_downward_transfer<T, D, 8>), dim3(nc), dim3(kBlock), lds, stream,
-                         rows, ip, cb, nrhs, buf.tlhs);
+    if (nrc_dn == 8)
+      hipLaunchKernelGGL((k_downward_transfer<T, D, 8>), dim3(nc, groups(8)), dim3(kBlock), lds,
+                         stream, rows, ip, cb, nrhs, buf.tlhs);
     else
-      hipLaunchKernelGGL((k_downward_transfer<T, D, 1>), dim3(nc), dim3(kBlock), lds, stream,
-                         rows, ip, cb, nrhs, buf.tlhs);
+      hipLaunchKernelGGL((k_downward_transfer<T, D, 1>), dim3(nc, groups(1)), dim3(kBlock), lds,
+                         stream, rows, ip, cb, nrhs, buf.tlhs);
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 6. leaf output: far field through V^T plus the near field
