@@ -5,18 +5,19 @@ Workload (the configuration the headline metric is quoted on):
   kernel with lengthscale 0.5, tensor Chebyshev degree 4, eta = 0.8,
   32 right-hand sides x with i.i.d. N(0,1) entries.
 
-One step = one CG solve of (K + 100 I) a = x for the 32 resident columns to a
-1e-8 relative FP64 residual: FP32 matvecs (FP32 kernel entries and
-contractions on the matrix cores) drive the recurrence, the residual is
-replaced by the FP64 action's each time it has dropped by 10x, and vectors,
-inner products and updates are FP64 (the precision form with FP64
-refinement).  value = n * nrhs * (iterations plain FP64 CG needs for the same
-tolerance) / seconds per refined solve: the FP64-equivalent matvec throughput
-of the refined solver.  Plain FP64 CG is timed beside it on the same system
-(its own value, the ratio, the solution difference), and so are single
-matvecs of both precisions (1 and 32 rhs, per phase), the host-buffer
-boundary, the differentiation side results (n = 2e5 2-D and n = 8e6 3-D:
-forward, backward, ratio, memory) and the FP64 CPU oracle on the host cores.
+One step = one solve of (K + 100 I) a = x for the 32 resident columns to a
+1e-8 relative FP64 residual: GCR whose matvecs use FP32 kernel entries with
+BF16 three-term matrix-core contractions, the residual replaced by the FP64
+action's each time it has dropped by 1e-3, and vectors, inner products and
+updates in FP64 (the precision form with FP64 refinement).
+value = n * nrhs * (iterations FMCA's plain FP64 CG needs for the same
+tolerance) / seconds per refined solve: the FP64-equivalent matvec
+throughput of the refined solver.  Beside it, each once: FMCA's FP64 CG on
+the same system, GCR with FP64 matvecs, CG with reliable updates on the same
+BF16 matvecs; single matvecs of the three precision forms (1 and 32 rhs, per
+phase), the host-buffer boundary, the differentiation side results (n = 2e5
+2-D and n = 8e6 3-D: forward, backward, ratio, memory, and a refined solve
+at n = 8e6) and the FP64 CPU oracle on the host cores.
 
 Prints ONE JSON line.  Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
@@ -65,9 +66,13 @@ def parse():
                         "0.1 for cg)")
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
-                            "total-degree", "plan", "assembled"],
+                            "total-degree", "plan", "assembled", "clustered-spectrum"],
                    help="matvec: the headline line (default); the others print a side line "
                         "for the other configurations")
+    p.add_argument("--spectrum-cases", default="4:0.8,5:0.8,6:0.8,7:0.8,8:0.8,9:0.8",
+                   help="clustered-spectrum: degree:eta pairs")
+    p.add_argument("--spectrum-steps", type=int, default=8,
+                   help="clustered-spectrum: block Krylov steps (block size 16)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="also run one elementwise copy of this size (reads and writes "
                         "M MiB) as the reference kernel for HBM traffic counters")
@@ -275,6 +280,109 @@ def side_clustered_cg(args):
     return res
 
 
+def side_clustered_spectrum(args):
+    """Is K~ + 1e-4 I positive definite for the clustered configuration
+    (n = 1e6, 16-Gaussian mixture sigma 0.02, GAUSSIAN l = 0.1), and at which
+    interpolation degree?  K~ is the H2 matrix, K the exact kernel matrix
+    (an eta = 0 actor: every block dense).  For each (degree, eta):
+    block Krylov space of E = K~ - K (block 16, --spectrum-steps blocks,
+    full reorthogonalisation, FP64 actions), then Rayleigh-Ritz of E and of
+    K~ on that space.
+      * min Ritz value of K~ + ridge I over the space: a Rayleigh quotient, so
+        a negative one proves K~ + ridge I indefinite (the vector is explicit);
+      * min Ritz value of E: estimates lambda_min(E); by Weyl,
+        lambda_min(K~ + ridge I) >= ridge + lambda_min(E) since K is PSD;
+      * ||K~||_2 from 12 block power steps on K~, giving cond estimates.
+    One JSON line per case, printed as it finishes."""
+    import torch
+    import fmca_amd
+    from fmca_amd import workloads as W
+    n, b, steps, ridge = args.n, 16, args.spectrum_steps, 1e-4
+    cases = [(int(c.split(":")[0]), float(c.split(":")[1]))
+             for c in args.spectrum_cases.split(",")]
+    P = W.gaussian_mixture(n, 3, 16, 0.02, 0)
+    K = fmca_amd.CovarianceKernel("GAUSSIAN", 0.1)
+    t0 = time.perf_counter()
+    Hd = fmca_amd.H2kernelActor(K, P, P, 4, 0.0)  # the exact K: no admissible blocks
+    dinfo = Hd.info()
+    assert dinfo["far_blocks"] == 0
+    print(json.dumps({"dense_actor": {"plan_s": time.perf_counter() - t0,
+                                      "near_entries": dinfo["near_entries"]}}), flush=True)
+    gen = torch.Generator(device="cuda")
+    out = []
+    for deg, eta in cases:
+        t0 = time.perf_counter()
+        H = fmca_amd.H2kernelActor(K, P, P, deg, eta)
+        info = H.info()
+        gen.manual_seed(1234)
+        Q = torch.linalg.qr(torch.randn(n, b, dtype=torch.float64, device="cuda",
+                                        generator=gen))[0]
+        Qs, KtQ, KQ = [], [], []
+        t_h2 = t_dense = 0.0
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            A1 = H.action_device(Q)
+            torch.cuda.synchronize()
+            tb = time.perf_counter()
+            A2 = Hd.action_device(Q)
+            torch.cuda.synchronize()
+            t_h2 += tb - ta
+            t_dense += time.perf_counter() - tb
+            Qs.append(Q)
+            KtQ.append(A1)
+            KQ.append(A2)
+            Wm = A1 - A2
+            Qa = torch.cat(Qs, dim=1)
+            for _ in range(2):
+                Wm = Wm - Qa @ (Qa.t() @ Wm)
+            Q = torch.linalg.qr(Wm)[0]
+        Qa = torch.cat(Qs, dim=1)
+        Kt = torch.cat(KtQ, dim=1)
+        Ke = torch.cat(KQ, dim=1)
+        Tk = Qa.t() @ Kt
+        Te = Tk - Qa.t() @ Ke
+        Tk, Te = 0.5 * (Tk + Tk.t()), 0.5 * (Te + Te.t())
+        th_k, S_k = torch.linalg.eigh(Tk)
+        th_e, S_e = torch.linalg.eigh(Te)
+        # residual of the lowest Ritz pair of E, and the Rayleigh quotients of
+        # K~ + ridge I and of K at that Ritz vector
+        s = S_e[:, :1]
+        u = Qa @ s
+        r_e = (Kt @ s - Ke @ s) - th_e[0] * u
+        rq_k = float((u.t() @ (Kt @ s)).item()) + ridge
+        rq_exact = float((u.t() @ (Ke @ s)).item())
+        res_e = float(torch.linalg.norm(r_e).item())
+        del Qs, KtQ, KQ, Qa, Kt, Ke, u, r_e
+        # ||K~||_2 by block power steps
+        gen.manual_seed(99)
+        X = torch.linalg.qr(torch.randn(n, b, dtype=torch.float64, device="cuda",
+                                        generator=gen))[0]
+        for _ in range(12):
+            X = torch.linalg.qr(H.action_device(X))[0]
+        lam_max = float(torch.linalg.eigvalsh(X.t() @ H.action_device(X))[-1].item())
+        res = {"degree": deg, "eta": eta, "q": info["q"], "far_blocks": info["far_blocks"],
+               "near_entries": info["near_entries"], "far_entries": info["far_entries"],
+               "krylov_dim": b * steps,
+               "min_ritz_Kh_plus_ridge": float(th_k[0].item()) + ridge,
+               "indefinite_proven": bool(float(th_k[0].item()) + ridge < 0),
+               "min_ritz_E": float(th_e[0].item()), "max_ritz_E": float(th_e[-1].item()),
+               "ritz_residual_E": res_e,
+               "rayleigh_Kh_plus_ridge_at_min_E_vector": rq_k,
+               "rayleigh_K_at_min_E_vector": rq_exact,
+               "weyl_lower_bound_estimate": ridge + float(th_e[0].item()),
+               "lambda_max_Kh": lam_max,
+               "cond_estimate": lam_max / max(float(th_k[0].item()) + ridge, ridge),
+               "h2_action_s": t_h2 / steps, "dense_action_s": t_dense / steps,
+               "case_s": time.perf_counter() - t0}
+        print(json.dumps({"spectrum": res}), flush=True)
+        out.append(res)
+        del H
+        torch.cuda.empty_cache()
+    return {"workload": f"clustered-spectrum: n={n} 16-Gaussian mixture (sigma 0.02), "
+                        f"GAUSSIAN l=0.1, ridge {ridge:g}", "cases": out}
+
+
 def side_total_degree(args):
     """n = 1e6 uniform 3-D, MATERN32 l = 0.5, deg 4, eta 0.8 with
     TotalDegreeInterpolator (q = 35, leaves of 35-69 points), 32 rhs N(0,1):
@@ -458,7 +566,8 @@ def side_large(args):
     torch.cuda.synchronize()
     first = time.perf_counter() - t0
     res = {"workload": "large: n=8e6 uniform 3D, EXPONENTIAL l=0.25, deg 4, eta 0.8, 16 rhs, "
-                       "FP32 form", "first_forward_incl_plan_s": first, "plan": plan,
+                       "FP32 form for forward and backward; FP32-BF16 + FP64-refined solve",
+           "first_forward_incl_plan_s": first, "plan": plan,
            "info": spec._actor.info()}
     tf, tb = [], []
     for _ in range(max(2, args.steps // 2)):
@@ -479,6 +588,29 @@ def side_large(args):
                 "backward_over_forward": b / f,
                 "device_memory": {"library_peak_bytes": fmca_amd.device_memory()[1],
                                   "torch_peak_bytes": int(torch.cuda.max_memory_allocated())}})
+    # FP32 bulk + FP64 refinement: solve (K + 1e-4 n I) a = alpha to a 1e-8
+    # relative FP64 residual, GCR on the BF16-split FP32 matvecs with FP64
+    # residual replacement, on the same actor (its plan and resident points)
+    H = spec._actor
+    b_rhs = alpha.detach()
+    ridge = 1e-4 * n
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a, info = H.cg(b_rhs, x0=torch.zeros_like(b_rhs), ridge=ridge, tol=1e-8,
+                   precision=fmca_amd.FP32_BF16, method="gcr", refine_factor=1e-3,
+                   max_iter=2000)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    r = b_rhs - H.action_device(a) - ridge * a
+    resid = (torch.linalg.norm(r, dim=0) / torch.linalg.norm(b_rhs, dim=0)).max().item()
+    del r
+    res["refined_solve"] = {
+        "system": f"(K + {ridge:g} I) a = alpha, 16 rhs, tol 1e-8, from a = 0",
+        "method": "gcr", "precision": "fp32_bf16", "refine_factor": 1e-3, "seconds": ts,
+        "iterations": info["iterations"], "matvecs_fp64": info["matvecs_fp64"],
+        "rel_residual_fp64_rechecked": resid, "converged": info["converged"],
+        "device_memory": {"library_peak_bytes": fmca_amd.device_memory()[1],
+                          "torch_peak_bytes": int(torch.cuda.max_memory_allocated())}}
     return res
 
 
@@ -542,7 +674,8 @@ def main():
     side = {"small-dense": side_small_dense, "clustered-cg": side_clustered_cg,
             "autograd": side_autograd, "large": side_large,
             "total-degree": side_total_degree, "plan": side_plan,
-            "assembled": side_assembled}.get(args.workload)
+            "assembled": side_assembled,
+            "clustered-spectrum": side_clustered_spectrum}.get(args.workload)
     if side is not None:
         print(json.dumps(side(args)), flush=True)
         return

@@ -38,6 +38,13 @@ CASES = [
     ("tps2d-valu", 4000, None, 2, 4, "TPS2D", 1.0, 0.8, 5),
     ("tps2d-mfma", 4000, 3500, 2, 4, "TPS2D", 1.0, 0.8, 16),
     ("rhs-40", 3000, None, 3, 3, "MATERN32", 0.5, 0.8, 40),
+    # high degrees in 3-D (q = 512, 729, 1000; leaves of up to 2q - 1 points),
+    # eta 1.5 so that n = 3e4 still has far blocks at q = 1000
+    ("q512-deg7-1rhs", 30000, None, 3, 7, "GAUSSIAN", 0.3, 1.5, 1),
+    ("q512-deg7-12rhs", 30000, None, 3, 7, "GAUSSIAN", 0.3, 1.5, 12),
+    ("q729-deg8-3rhs", 30000, None, 3, 8, "MATERN32", 0.4, 1.5, 3),
+    ("q1000-deg9-1rhs", 30000, None, 3, 9, "GAUSSIAN", 0.3, 1.5, 1),
+    ("q1000-deg9-16rhs", 30000, None, 3, 9, "GAUSSIAN", 0.3, 1.5, 16),
     ("q216-16rhs-two-row-passes", 4000, None, 3, 5, "EXPONENTIAL", 0.5, 0.8, 16),
     ("q343-9rhs-valu-coupling", 3000, None, 3, 6, "GAUSSIAN", 0.3, 0.8, 9),
     ("1d-16rhs", 5000, None, 1, 4, "MATERN32", 0.2, 0.8, 16),

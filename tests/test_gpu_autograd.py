@@ -360,3 +360,78 @@ def test_gradient_kernels_are_the_derivative_of_the_h2_forward(d, ktype, l, deg,
             assert np.array_equal(Hp.indices(0), perm0) and np.array_equal(Hm.indices(0), perm0)
             fd = (Lp - Lm) / (2 * hp)
             assert abs(fd - dpo[k, j]) <= 1e-6 * scale, (j, k, fd, dpo[k, j])
+
+
+def test_exponential_3d_deg4_16rhs_forward_gradients_and_refined_solve():
+    """The n = 8e6 configuration's shape (uniform 3-D, EXPONENTIAL l = 0.25,
+    degree 4, eta 0.8, 16 rhs) at n = 4096, where the oracle and dense FP64
+    autograd finish in seconds.
+    Forward: FP64 vs the oracle <= 1e-12, FP32 vs FP64 <= 1e-5.
+    Backward (d alpha, d l; no point gradients, as at n = 8e6, where the
+    exponential's k'(r)/r is singular at coincident points anyway), FP64 and
+    FP32: against dense FP64 autograd (bound tied to the forward's H2 error)
+    and centred differences (h = 1e-4) of the H2 forward; FP32 vs FP64 <= 1e-4.
+    Refinement leg: GCR on BF16-split FP32 matvecs with FP64 residual
+    replacement reaches an FP64 residual <= 1e-8 (rechecked through the FP64
+    action) and FP64 CG's solution, at ridge 1e-4 n as at n = 8e6."""
+    import torch
+    import fmca_amd
+    from fmca_amd.autograd import H2Spec, h2_matvec
+    print("[case] exponential 3-D deg 4, 16 rhs, n = 4096", flush=True)
+    n, nrhs, ktype, l, deg = 4096, 16, "EXPONENTIAL", 0.25, 4
+    P, alpha, y = _problem(n=n, d=3, nrhs=nrhs, seed=40)
+    Pn, an = P.cpu().numpy(), alpha.cpu().numpy()
+    H = fmca_amd.H2kernelActor(fmca_amd.CovarianceKernel(ktype, l), Pn, Pn, deg, 0.8)
+    assert H.info()["far_blocks"] > 0
+    y64 = H.action_device(alpha, precision=fmca_amd.FP64)
+    y32 = H.action_device(alpha, precision=fmca_amd.FP32)
+    ref = oracle.OracleH2(Pn, Pn, ktype, l, deg, 0.8).action(an)
+    assert rel_err(y64.cpu().numpy(), ref) <= 1e-12
+    assert rel_err(y32.cpu().numpy(), ref) <= 1e-5
+    with torch.no_grad():
+        efwd = rel_err(ref, (dense_kernel(ktype, P, P, l) @ alpha).cpu().numpy())
+    assert efwd < 1e-3
+    _, da, dl, _ = _dense_grads(P, alpha, y, ktype, l)
+
+    def h2_grads(prec):
+        spec = H2Spec(ktype, deg, 0.8, precision=prec)
+        ag = alpha.clone().requires_grad_(True)
+        lt = torch.tensor(l, dtype=torch.float64, requires_grad=True)
+        loss = 0.5 * ((h2_matvec(ag, lt, P, spec) - y) ** 2).sum()
+        loss.backward()
+        return ag.grad, lt.grad.item()
+
+    ga, gl = h2_grads(fmca_amd.FP64)
+    ga32, gl32 = h2_grads(fmca_amd.FP32)
+    ea = rel_err(ga.cpu().numpy(), da.cpu().numpy())
+    el = abs(gl - dl) / abs(dl)
+    print(f"  forward H2 err {efwd:.2e}; vs dense: alpha {ea:.2e} l {el:.2e}; "
+          f"fp32 vs fp64: alpha {rel_err(ga32.cpu().numpy(), ga.cpu().numpy()):.2e} "
+          f"l {abs(gl32 - gl) / abs(gl):.2e}", flush=True)
+    assert ea < 20 * efwd and el < 20 * efwd
+    assert rel_err(ga32.cpu().numpy(), ga.cpu().numpy()) <= 1e-4
+    assert abs(gl32 - gl) <= 1e-4 * abs(gl)
+    h = 1e-4
+    spec = H2Spec(ktype, deg, 0.8)
+
+    def loss(av, lv):
+        with torch.no_grad():
+            return 0.5 * ((h2_matvec(av, torch.tensor(lv, dtype=torch.float64), P, spec) - y)
+                          ** 2).sum().item()
+    fl = (loss(alpha, l + h) - loss(alpha, l - h)) / (2 * h)
+    assert abs(fl - gl) <= 1e-6 * abs(gl)
+    v = torch.from_numpy(np.random.default_rng(41).standard_normal(alpha.shape)).cuda()
+    fa = (loss(alpha + h * v, l) - loss(alpha - h * v, l)) / (2 * h)
+    assert abs(fa - (ga * v).sum().item()) <= 1e-8 * abs(fa)
+    # refinement leg
+    ridge = 1e-4 * n
+    z = torch.zeros_like(y)
+    x64, i64 = H.cg(y, x0=z, ridge=ridge, tol=1e-8, precision=fmca_amd.FP64)
+    xr, ir = H.cg(y, x0=z, ridge=ridge, tol=1e-8, precision=fmca_amd.FP32_BF16,
+                  method="gcr", refine_factor=1e-3)
+    print(f"  fp64 cg {i64['iterations']} it; refined gcr {ir}", flush=True)
+    assert i64["converged"] and ir["converged"] and ir["matvecs_fp32"] > 0
+    r = y - H.action_device(xr) - ridge * xr
+    assert (torch.linalg.norm(r, dim=0) / torch.linalg.norm(y, dim=0)).max().item() <= 1e-8
+    # both residuals <= 1e-8 bound the difference by about cond * 2e-8, cond ~ 1e3 here
+    assert (torch.linalg.norm(xr - x64) / torch.linalg.norm(x64)).item() <= 1e-4
