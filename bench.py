@@ -66,7 +66,7 @@ def parse():
                         "0.1 for cg)")
     p.add_argument("--workload", default="matvec",
                    choices=["matvec", "small-dense", "clustered-cg", "autograd", "large",
-                            "total-degree", "plan", "assembled", "clustered-spectrum"],
+                            "total-degree", "plan", "assembled", "clustered-spectrum", "samplets"],
                    help="matvec: the headline line (default); the others print a side line "
                         "for the other configurations")
     p.add_argument("--spectrum-cases", default="4:0.8,5:0.8,6:0.8,7:0.8,8:0.8,9:0.8",
@@ -383,6 +383,54 @@ def side_clustered_spectrum(args):
                         f"GAUSSIAN l=0.1, ridge {ridge:g}", "cases": out}
 
 
+def side_samplets(args):
+    """Samplet compression (FMCA's tests/test_H2SampletCompressor.cpp setting:
+    n = 1e5 uniform 2-D, EXPONENTIAL l = 1, dtilde 2..5, multipole degree
+    2 (dtilde - 1)) with CompareCluster at eta 0.8, and the same at dtilde 3
+    in 3-D: tree build, compression phases, triplets, a-posteriori triplets
+    at 1e-3 and the compression error of FMCA's test (ten columns of K)."""
+    import fmca_amd
+    import oracle
+    import scipy.sparse as sp
+    n = 100_000
+    out = []
+    for dim, dtilde in ((2, 2), (2, 3), (2, 4), (2, 5), (3, 3)):
+        P = np.asfortranarray(np.random.default_rng(11).random((dim, n)))
+        t0 = time.perf_counter()
+        T = fmca_amd.SampletTree(P, dtilde)
+        t_tree = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        comp = fmca_amd.SampletKernelCompressor(T, fmca_amd.CovarianceKernel("EXPONENTIAL", 1.0),
+                                                P, 0.8, 0.0)
+        t_comp = time.perf_counter() - t0
+        st = comp.statistics()
+        r, c, v = comp.triplets()
+        ra, ca, va = comp.aposteriori_triplets_fast(1e-3)
+        perm = T.indices()
+        idx = np.random.default_rng(12).choice(n, 10, replace=False)
+        X = np.zeros((n, 10))
+        X[idx, np.arange(10)] = 1.0
+        Y1 = oracle.dense_matrix(P[:, perm], P[:, perm[idx]], "EXPONENTIAL", 1.0).T
+
+        def err_of(rr, cc, vv):
+            S = sp.csr_matrix((vv, (rr, cc)), shape=(n, n))
+            S = S + sp.triu(S, 1).T
+            Y2 = fmca_amd.inverseSampletTransform(T, S @ fmca_amd.sampletTransform(T, X))
+            return float(np.linalg.norm(Y2 - Y1) / np.linalg.norm(Y1))
+        res = {"dim": dim, "dtilde": dtilde, "tree_s": t_tree, "compress_s": t_comp,
+               "nnz_per_row": len(v) / n, "nnz_per_row_aposteriori_1e-3": len(va) / n,
+               "compression_error": err_of(r, c, v),
+               "compression_error_aposteriori_1e-3": err_of(ra, ca, va),
+               **{k: st[k] for k in ("pattern_entries", "dense_jobs", "far_jobs", "job_elements",
+                                     "plan_s", "upload_s", "gpu_s", "download_s", "combine_s",
+                                     "triplets_s")}}
+        print(json.dumps({"samplets": res}), flush=True)
+        out.append(res)
+        del comp, r, c, v
+    return {"workload": "samplets: n=1e5 uniform, EXPONENTIAL l=1, eta 0.8, CompareCluster",
+            "cases": out}
+
+
 def side_total_degree(args):
     """n = 1e6 uniform 3-D, MATERN32 l = 0.5, deg 4, eta 0.8 with
     TotalDegreeInterpolator (q = 35, leaves of 35-69 points), 32 rhs N(0,1):
@@ -675,7 +723,8 @@ def main():
             "autograd": side_autograd, "large": side_large,
             "total-degree": side_total_degree, "plan": side_plan,
             "assembled": side_assembled,
-            "clustered-spectrum": side_clustered_spectrum}.get(args.workload)
+            "clustered-spectrum": side_clustered_spectrum,
+            "samplets": side_samplets}.get(args.workload)
     if side is not None:
         print(json.dumps(side(args)), flush=True)
         return

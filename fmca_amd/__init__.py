@@ -363,6 +363,124 @@ class MultipoleFunctionEvaluator:
         return self._h.action(C, precision)
 
 
+class SampletTree:
+    """FMCA's Python SampletTree(P, dtilde) (py/FMCA.cpp:51-118): an
+    H2SampletTree on the ClusterTree with TotalDegree moments of degree
+    2 (dtilde - 1) and samplets with dtilde vanishing moments.  Built on the
+    host; the compression's kernel blocks run on the GPU."""
+
+    def __init__(self, P=None, dtilde: int = 1):
+        self._t = None
+        if P is not None:
+            P = _points(P, "P")
+            self._t = _C.SampletTreeHandle(P, int(dtilde))
+            self.dtilde = max(int(dtilde), 1)
+            self.p = 2 * (self.dtilde - 1)
+            self._perm = self._t.indices()
+            self._map = None
+
+    @property
+    def n(self) -> int:
+        return self._t.n
+
+    def indices(self) -> np.ndarray:
+        """Cluster position -> point index (pySampletTree::indices)."""
+        return self._perm.copy()
+
+    def levels(self) -> np.ndarray:
+        """Level of each samplet coefficient (sampletLevelMapper)."""
+        return self._t.levels()
+
+    def toClusterOrder(self, mat) -> np.ndarray:
+        mat = np.asarray(mat, dtype=np.float64)
+        return np.asfortranarray(mat[self._perm])
+
+    def toNaturalOrder(self, mat) -> np.ndarray:
+        mat = np.asarray(mat, dtype=np.float64)
+        out = np.empty_like(mat)
+        out[self._perm] = mat
+        return np.asfortranarray(out)
+
+    def clusters(self) -> dict:
+        """Per cluster: begin, size, sons, qdim, nscal, start, plus q and mdtilde."""
+        return self._t.clusters()
+
+    def basis(self, cluster: int):
+        """(Q, V) of one cluster: samplet basis and multiscale cluster basis."""
+        return self._t.basis(int(cluster))
+
+    def coeff2indices(self, i: int) -> np.ndarray:
+        """Point indices of the cluster that owns coefficient i
+        (pySampletTree::coeff2indices)."""
+        if self._map is None:
+            cl = self.clusters()
+            owner = np.empty(self.n, dtype=np.int64)
+            for c in range(len(cl["begin"])):
+                s0 = int(cl["start"][c])
+                cnt = int(cl["qdim"][c] - cl["nscal"][c]) + (int(cl["nscal"][c]) if c == 0 else 0)
+                owner[s0:s0 + cnt] = c
+            self._map = (owner, cl["begin"], cl["size"])
+        owner, begin, size = self._map
+        c = owner[int(i)]
+        return self._perm[begin[c]:begin[c] + size[c]].astype(np.int64)
+
+
+def sampletTransform(tree: SampletTree, data) -> np.ndarray:
+    """Samplet coefficients of data given in cluster order (n x k)."""
+    return tree._t.transform(np.asfortranarray(np.asarray(data, dtype=np.float64)), False)
+
+
+def inverseSampletTransform(tree: SampletTree, data) -> np.ndarray:
+    """Data in cluster order from samplet coefficients (n x k)."""
+    return tree._t.transform(np.asfortranarray(np.asarray(data, dtype=np.float64)), True)
+
+
+class SampletKernelCompressor:
+    """FMCA's Python SampletKernelCompressor(hst, ker, P, eta=0.8, thres=0)
+    (py/FMCA.cpp:256-330): the kernel matrix in samplet coordinates,
+    compressed with CompareCluster at eta; entries |v| <= thres dropped (the
+    diagonal kept), upper triangle.  P is accepted for the signature's sake;
+    the tree holds the points."""
+
+    def __init__(self, hst: SampletTree = None, ker: CovarianceKernel = None, P=None,
+                 eta: float = 0.8, thres: float = 0.0):
+        self._m = None
+        if hst is not None:
+            self.init(hst, ker, P, eta, thres)
+
+    def init(self, hst: SampletTree, ker: CovarianceKernel, P=None, eta: float = 0.8,
+             thres: float = 0.0):
+        if P is not None and _points(P, "P").shape[1] != hst.n:
+            raise ValueError("P does not match the tree")
+        self.n = hst.n
+        self.eta = float(eta)
+        self.thres = float(thres)
+        self._m = hst._t.compress(ker.kernelType(), ker.l, ker.c, ker.nu, self.eta, self.thres)
+        return self
+
+    def triplets(self):
+        """(rows, cols, values) in samplet coordinates, rows <= cols."""
+        return self._m.triplets()
+
+    def aposteriori_triplets_fast(self, thres: float):
+        """aposteriori_triplets_fast: the triplets kept at relative threshold thres."""
+        r, c, v = self._m.triplets()
+        keep = self._m.aposteriori(float(thres))
+        return r[keep], c[keep], v[keep]
+
+    def statistics(self) -> dict:
+        keys = ["pattern_entries", "dense_jobs", "far_jobs", "job_elements", "program_nodes",
+                "plan_s", "upload_s", "gpu_s", "download_s", "combine_s", "triplets_s",
+                "triplets"]
+        return dict(zip(keys, self._m.stats()))
+
+    def matrix(self):
+        """The compressed matrix as scipy.sparse.csc_matrix (upper triangle)."""
+        import scipy.sparse as sp
+        r, c, v = self._m.triplets()
+        return sp.csc_matrix((v, (r, c)), shape=(self.n, self.n))
+
+
 def _as_device_matrix(t, rows):
     import torch
     if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
@@ -386,4 +504,5 @@ def _check_out(out, rows, nrhs):
 
 
 __all__ = ["CovarianceKernel", "H2Matrix", "H2kernelActor", "FP64", "FP32", "FP32_BF16", "ROWS", "COLS",
-           "KERNEL_TYPES", "INTERPOLATORS", "MultipoleFunctionEvaluator"]
+           "KERNEL_TYPES", "INTERPOLATORS", "MultipoleFunctionEvaluator", "SampletTree",
+           "SampletKernelCompressor", "sampletTransform", "inverseSampletTransform"]

@@ -36,8 +36,9 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 SOURCES_HIP = ["kernels.hip", "kernels_mfma.hip", "kernels_bf16.hip", "kernels_grad.hip",
-               "blas1.hip", "tree_device.hip", "assembled.hip", "pattern_device.hip"]
-SOURCES_CPP = ["plan.cpp", "actor.cpp", "projector.cpp"]
+               "blas1.hip", "tree_device.hip", "assembled.hip", "pattern_device.hip",
+               "samplets.hip"]
+SOURCES_CPP = ["plan.cpp", "actor.cpp", "projector.cpp", "samplets.cpp", "samplets_api.cpp"]
 
 
 def _run(cmd, verbose):

@@ -19,6 +19,7 @@
 #include "device.hpp"
 #include "plan.hpp"
 #include "projector.hpp"
+#include "samplets.hpp"
 
 using namespace fmca_amd;
 
@@ -39,6 +40,12 @@ fmca_status from_hip(hipError_t e, const char *what) {
                              : FMCA_ERR_DEVICE;
   return fail(st, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+}  // namespace
+
+void fmca_amd::set_last_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
 
 #define HIP_TRY(expr)                                            \
   do {                                                           \
@@ -431,24 +438,7 @@ KernelParams<T> kernel_params(const fmca_h2_actor *a, const VariantGroups &g) {
   k.l = (T)a->l;
   k.c = (T)a->c;
   k.nu = (T)a->nu;
-  double sc = 1.0 / a->l;
-  switch (a->kt) {
-    case KT_MATERN32:
-      sc = std::sqrt(3.0) / a->l;
-      break;
-    case KT_MATERN52:
-      sc = std::sqrt(5.0) / a->l;
-      break;
-    case KT_MATERN72:
-      sc = std::sqrt(7.0) / a->l;
-      break;
-    case KT_MATERN92:
-      sc = 3.0 / a->l;
-      break;
-    case KT_GAUSSIAN:
-      sc = -0.5 / (a->l * a->l);
-      break;
-  }
+  const double sc = kernel_radial_scale(a->kt, a->l);
   k.s = (T)sc;
   k.sl2 = (T)(sc * 1.4426950408889634074);
   k.inv_l = (T)(1.0 / a->l);

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 #include "plan.hpp"
@@ -102,6 +103,24 @@ struct KernelParams {
 };
 
 enum KernelVariant : int { KV_VALUE = 0, KV_DL = 1, KV_PHI = 2 };
+
+// The radial scale s of KernelParams for kernel type kt and lengthscale l.
+inline double kernel_radial_scale(int kt, double l) {
+  switch (kt) {
+    case KT_MATERN32:
+      return std::sqrt(3.0) / l;
+    case KT_MATERN52:
+      return std::sqrt(5.0) / l;
+    case KT_MATERN72:
+      return std::sqrt(7.0) / l;
+    case KT_MATERN92:
+      return 3.0 / l;
+    case KT_GAUSSIAN:
+      return -0.5 / (l * l);
+    default:
+      return 1.0 / l;
+  }
+}
 
 // Level structure of a tree on the host (drives the per-level launches).
 struct HostLevels {

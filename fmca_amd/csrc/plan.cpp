@@ -415,6 +415,10 @@ void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)> &f) 
   parallel_chunks(n, nw, [&](int, int64_t lo, int64_t hi) { f(lo, hi); });
 }
 
+void parallel_dynamic(int64_t n, int64_t grain, const std::function<void(int64_t)> &f) {
+  parallel_for(n, grain, f);
+}
+
 void point_box(int dim, int64_t n, const double *P, double *mn, double *mx) {
   const int nw = (int)std::max<int64_t>(1, std::min<int64_t>(num_workers(), n / 65536 + 1));
   for (int i = 0; i < dim; ++i) {

@@ -127,6 +127,10 @@ struct TreeSplitter {
 // (FMCA_AMD_THREADS, else OMP_NUM_THREADS, else the hardware concurrency).
 void parallel_ranges(int64_t n, const std::function<void(int64_t, int64_t)> &f);
 
+// f(i) for i in [0, n) on the planner's worker threads, dynamically
+// scheduled in chunks of `grain`.
+void parallel_dynamic(int64_t n, int64_t grain, const std::function<void(int64_t)> &f);
+
 // Min / max of each coordinate over n points.
 void point_box(int dim, int64_t n, const double *P, double *mn, double *mx);
 

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
+#include <algorithm>
 
 #include "fmca_amd.h"
 
@@ -124,6 +126,123 @@ py::array_t<int32_t> launch_order(const Actor &a, int which) {
   check(fmca_h2_launch_order(a.h, which, nullptr, &len));
   py::array_t<int32_t> out({(py::ssize_t)len, (py::ssize_t)4});
   check(fmca_h2_launch_order(a.h, which, out.mutable_data(), &len));
+  return out;
+}
+
+struct SampletTreeH {
+  fmca_samplet_tree *h = nullptr;
+  int dim = 0;
+  int64_t n = 0;
+  SampletTreeH() = default;
+  SampletTreeH(const SampletTreeH &) = delete;
+  ~SampletTreeH() {
+    if (h) fmca_samplet_tree_destroy(h);
+  }
+};
+
+struct SampletMatrixH {
+  fmca_samplet_matrix *h = nullptr;
+  int64_t n = 0;
+  SampletMatrixH() = default;
+  SampletMatrixH(const SampletMatrixH &) = delete;
+  ~SampletMatrixH() {
+    if (h) fmca_samplet_matrix_destroy(h);
+  }
+};
+
+std::unique_ptr<SampletTreeH> samplet_tree(const py::array &P_in, int dtilde) {
+  farr P = as_f64(P_in, "P");
+  if (P.ndim() != 2) throw py::value_error("points must be a (dim, n) array");
+  auto t = std::make_unique<SampletTreeH>();
+  t->dim = (int)P.shape(0);
+  t->n = P.shape(1);
+  {
+    py::gil_scoped_release nogil;
+    check(fmca_samplet_tree_create(t->dim, t->n, P.data(), dtilde, &t->h));
+  }
+  return t;
+}
+
+py::array samplet_transform(const SampletTreeH &t, const py::array &in_, bool inverse) {
+  farr in = as_f64(in_, "data");
+  if (in.ndim() < 1 || in.ndim() > 2 || in.shape(0) != t.n)
+    throw py::value_error("data must have n rows");
+  const int64_t k = in.ndim() == 2 ? in.shape(1) : 1;
+  farr out({(py::ssize_t)t.n, (py::ssize_t)k});
+  double *po = out.mutable_data();
+  const double *pi = in.data();
+  {
+    py::gil_scoped_release nogil;
+    check(fmca_samplet_transform(t.h, inverse ? 1 : 0, pi, t.n, k, po, t.n));
+  }
+  return out;
+}
+
+py::dict samplet_clusters(const SampletTreeH &t) {
+  int64_t info[7];
+  check(fmca_samplet_tree_info(t.h, info));
+  const py::ssize_t ncl = info[3];
+  py::array_t<int64_t> begin(ncl), size(ncl), start(ncl);
+  py::array_t<int32_t> sons({ncl, (py::ssize_t)2}), qdim(ncl), nscal(ncl);
+  check(fmca_samplet_tree_clusters(t.h, begin.mutable_data(), size.mutable_data(),
+                                   sons.mutable_data(), qdim.mutable_data(),
+                                   nscal.mutable_data(), start.mutable_data()));
+  py::dict d;
+  d["begin"] = begin;
+  d["size"] = size;
+  d["sons"] = sons;
+  d["qdim"] = qdim;
+  d["nscal"] = nscal;
+  d["start"] = start;
+  d["q"] = info[4];
+  d["mdtilde"] = info[5];
+  d["depth"] = info[6];
+  return d;
+}
+
+py::tuple samplet_basis(const SampletTreeH &t, int c) {
+  int64_t info[7];
+  check(fmca_samplet_tree_info(t.h, info));
+  if (c < 0 || c >= info[3]) throw py::value_error("cluster out of range");
+  int32_t qd = 0;
+  std::vector<int32_t> qdim(info[3]);
+  check(fmca_samplet_tree_clusters(t.h, nullptr, nullptr, nullptr, qdim.data(), nullptr, nullptr));
+  qd = qdim[c];
+  farr Q({(py::ssize_t)qd, (py::ssize_t)qd}), V({(py::ssize_t)info[4], (py::ssize_t)qd});
+  check(fmca_samplet_tree_basis(t.h, c, Q.mutable_data(), V.mutable_data()));
+  return py::make_tuple(Q, V);
+}
+
+std::unique_ptr<SampletMatrixH> samplet_compress(const SampletTreeH &t, const std::string &ktype,
+                                                 double l, double c, double nu, double eta,
+                                                 double thres) {
+  auto M = std::make_unique<SampletMatrixH>();
+  M->n = t.n;
+  fmca_kernel_desc k{ktype.c_str(), l, c, nu};
+  {
+    py::gil_scoped_release nogil;
+    check(fmca_samplet_compress(t.h, &k, eta, thres, &M->h));
+  }
+  return M;
+}
+
+py::tuple samplet_triplets(const SampletMatrixH &M) {
+  int64_t nnz = 0;
+  check(fmca_samplet_matrix_nnz(M.h, &nnz));
+  py::array_t<int64_t> rows(nnz), cols(nnz);
+  py::array_t<double> vals(nnz);
+  check(fmca_samplet_matrix_triplets(M.h, rows.mutable_data(), cols.mutable_data(),
+                                     vals.mutable_data()));
+  return py::make_tuple(rows, cols, vals);
+}
+
+py::array_t<int64_t> samplet_aposteriori(const SampletMatrixH &M, double thres) {
+  int64_t nnz = 0, count = 0;
+  check(fmca_samplet_matrix_nnz(M.h, &nnz));
+  std::vector<int64_t> idx(nnz);
+  check(fmca_samplet_matrix_aposteriori(M.h, thres, &count, idx.data()));
+  py::array_t<int64_t> out(count);
+  std::copy(idx.begin(), idx.begin() + count, out.mutable_data());
   return out;
 }
 
@@ -357,4 +476,34 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
   }, py::arg("reset_peak") = false,
         "(live, peak) device bytes held by the library's plans and workspaces");
   m.def("status_string", [](int s) { return std::string(fmca_status_string((fmca_status)s)); });
+  py::class_<SampletTreeH>(m, "SampletTreeHandle")
+      .def(py::init(&samplet_tree), py::arg("P"), py::arg("dtilde"))
+      .def_readonly("n", &SampletTreeH::n)
+      .def_readonly("dim", &SampletTreeH::dim)
+      .def("indices",
+           [](const SampletTreeH &t) {
+             py::array_t<uint32_t> out(t.n);
+             check(fmca_samplet_tree_indices(t.h, out.mutable_data()));
+             return out;
+           })
+      .def("levels",
+           [](const SampletTreeH &t) {
+             py::array_t<int32_t> out(t.n);
+             check(fmca_samplet_tree_levels(t.h, out.mutable_data()));
+             return out;
+           })
+      .def("transform", &samplet_transform, py::arg("data"), py::arg("inverse") = false)
+      .def("clusters", &samplet_clusters)
+      .def("basis", &samplet_basis, py::arg("cluster"))
+      .def("compress", &samplet_compress, py::arg("ktype"), py::arg("l"), py::arg("c"),
+           py::arg("nu"), py::arg("eta"), py::arg("threshold"));
+  py::class_<SampletMatrixH>(m, "SampletMatrixHandle")
+      .def_readonly("n", &SampletMatrixH::n)
+      .def("triplets", &samplet_triplets)
+      .def("aposteriori", &samplet_aposteriori, py::arg("thres"))
+      .def("stats", [](const SampletMatrixH &M) {
+        std::vector<double> st(FMCA_SAMPLET_STATS);
+        check(fmca_samplet_matrix_stats(M.h, st.data()));
+        return st;
+      });
 }

@@ -425,6 +425,93 @@ const char *fmca_status_string(fmca_status status);
 /* Last error message of the calling thread ("" if none). */
 const char *fmca_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Samplets: the second caller of the evaluator interface (interpolate_kernel
+ * and compute_dense_block) besides the H2 action.
+ * ------------------------------------------------------------------------ */
+
+typedef struct fmca_samplet_tree fmca_samplet_tree;     /* opaque */
+typedef struct fmca_samplet_matrix fmca_samplet_matrix; /* opaque */
+
+/* Replaces pySampletTree(P, dtilde) (py/FMCA.cpp:51-70): H2SampletTree
+ * <ClusterTree> (FMCA/src/Samplets/H2SampletTree.h:75-97) on
+ * NystromMoments<TotalDegreeInterpolator>(P, 2 (dtilde - 1)) and
+ * MinNystromSampletMoments<MonomialInterpolator>(P, dtilde - 1): the
+ * ClusterTree with minimum leaf size q = binom(dim + 2 (dtilde - 1), dim),
+ * samplets with dtilde vanishing moments from a Householder QR per cluster
+ * (H2SampletTree.h:101-133), sampletMapper's coefficient order
+ * (SampletMapper.h:19-31) and the multiscale cluster bases
+ * (H2SampletTreeBase.h:68-86).  dtilde < 1 is taken as 1 (as FMCA's binding
+ * does); dtilde <= 5.  Host only.  P: dim x n column-major. */
+fmca_status fmca_samplet_tree_create(int dim, int64_t n, const double *P, int dtilde,
+                                     fmca_samplet_tree **tree);
+fmca_status fmca_samplet_tree_destroy(fmca_samplet_tree *tree);
+
+/* info[7] = {n, dim, dtilde, clusters, q, mdtilde, depth}.  No direct FMCA
+ * counterpart (FMCA exposes these as tree and moment members). */
+fmca_status fmca_samplet_tree_info(const fmca_samplet_tree *tree, int64_t *info);
+
+/* pySampletTree::indices (py/FMCA.cpp:72-74): the cluster-order permutation,
+ * perm[i] = the point at cluster position i. */
+fmca_status fmca_samplet_tree_indices(const fmca_samplet_tree *tree, uint32_t *perm);
+
+/* pySampletTree::levels (py/FMCA.cpp:75-78, sampletLevelMapper,
+ * SampletMapper.h:34-50): the level of each of the n coefficients. */
+fmca_status fmca_samplet_tree_levels(const fmca_samplet_tree *tree, int32_t *levels);
+
+/* Per cluster (level order; null outputs skipped): point range begin/size,
+ * sons (2 per cluster, -1 for none), the order of its basis Q (qdim), its
+ * scaling functions (nscal) and its first coefficient (start).  Node fields
+ * of SampletTreeBase (SampletTreeBase.h:24-30, 192-197). */
+fmca_status fmca_samplet_tree_clusters(const fmca_samplet_tree *tree, int64_t *begin,
+                                       int64_t *size, int32_t *sons, int32_t *qdim,
+                                       int32_t *nscal, int64_t *start);
+
+/* SampletTreeBase::Q() and H2SampletTreeBase::V() of one cluster: Q is
+ * qdim x qdim, V is q x qdim, both column-major (null outputs skipped). */
+fmca_status fmca_samplet_tree_basis(const fmca_samplet_tree *tree, int cluster, double *Q,
+                                    double *V);
+
+/* sampletTransform / inverseSampletTransform (SampletTreeBase.h:68-85,
+ * py/FMCA.cpp:396-420): in and out are n x k column-major (leading
+ * dimensions ld_in, ld_out >= n); the forward transform maps data in
+ * cluster order to samplet coefficients.  Host only. */
+fmca_status fmca_samplet_transform(const fmca_samplet_tree *tree, int inverse, const double *in,
+                                   int64_t ld_in, int64_t k, double *out, int64_t ld_out);
+
+/* Replaces pySampletKernelCompressor(hst, ker, P, eta, thres)
+ * (py/FMCA.cpp:256-300): SampletMatrixCompressor<H2SampletTree> with
+ * CompareCluster -- init (samplet_matrix_compressor.h:37-71), compress
+ * (:73-186, with recursivelyComputeBlock :330-388) over NystromEvaluator
+ * (FMCA/src/MatrixEvaluator/NystromEvaluator.h:39-67) -- and triplets()
+ * (:222-256): the upper triangle of the kernel matrix in samplet
+ * coordinates, entries with |v| > threshold and the whole diagonal.  The
+ * kernel blocks of the compression (dense leaf blocks and interpolated far
+ * blocks, each with its cluster-basis products) are evaluated on the current
+ * HIP device; their device memory is transient and counted against
+ * fmca_set_device_memory_limit.  FMCA_ERR_NO_DEVICE without a device. */
+fmca_status fmca_samplet_compress(const fmca_samplet_tree *tree, const fmca_kernel_desc *kernel,
+                                  double eta, double threshold, fmca_samplet_matrix **out);
+fmca_status fmca_samplet_matrix_destroy(fmca_samplet_matrix *mat);
+
+/* Number of triplets, and the triplets (row, column, value), rows <= columns,
+ * in samplet coordinates (null outputs skipped). */
+fmca_status fmca_samplet_matrix_nnz(const fmca_samplet_matrix *mat, int64_t *nnz);
+fmca_status fmca_samplet_matrix_triplets(const fmca_samplet_matrix *mat, int64_t *rows,
+                                         int64_t *cols, double *vals);
+
+/* aposteriori_triplets_fast(thres) (samplet_matrix_compressor.h:258-290):
+ * *count kept triplets; index (nullable, nnz entries of room) receives their
+ * positions in the triplet list. */
+fmca_status fmca_samplet_matrix_aposteriori(const fmca_samplet_matrix *mat, double thres,
+                                            int64_t *count, int64_t *index);
+
+/* Work and time of the compression: {pattern entries, dense jobs, far jobs,
+ * job block elements, program nodes, plan s, upload s, GPU s, download s,
+ * combine s, triplets s, triplets}.  No FMCA counterpart. */
+enum { FMCA_SAMPLET_STATS = 12 };
+fmca_status fmca_samplet_matrix_stats(const fmca_samplet_matrix *mat, double *stats);
+
 #ifdef __cplusplus
 }
 #endif
