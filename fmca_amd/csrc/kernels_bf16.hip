@@ -65,13 +65,40 @@ __device__ __forceinline__ void split_bf16(float v, u16 &hi, u16 &lo) {
   lo = __builtin_bit_cast(u16, l);
 }
 
+// Pairs: one packed conversion gives both high parts, whose FP32 values are
+// the two 16-bit halves shifted into place (no second conversion).
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const __bf16 h = (__bf16)v[i];
-    hi[i] = h;
-    lo[i] = (__bf16)(v[i] - (float)h);
+  for (int i = 0; i < 8; i += 2) {
+    const __bf16 h0 = (__bf16)v[i], h1 = (__bf16)v[i + 1];
+    hi[i] = h0;
+    hi[i + 1] = h1;
+    const uint32_t u = (uint32_t)__builtin_bit_cast(u16, h0) |
+                       ((uint32_t)__builtin_bit_cast(u16, h1) << 16);
+    const float f0 = __builtin_bit_cast(float, u << 16);
+    const float f1 = __builtin_bit_cast(float, u & 0xffff0000u);
+    lo[i] = (__bf16)(v[i] - f0);
+    lo[i + 1] = (__bf16)(v[i + 1] - f1);
   }
+}
+
+// Kernels smooth at r = 0 take their squared distances from the matrix
+// cores: for 16 sources and 16 targets one v_mfma_f32_16x16x4_f32 of the
+// source rows [-2 y', |y'|^2] (k-slots 0..D-1 and 3) with the target columns
+// [x', 1] gives |y'|^2 - 2 x'.y', and one add of |x'|^2 per entry completes
+// r^2 (x', y' centred on the target leaf).  In FP32 the cancellation leaves
+// an absolute error of a few eps |y'|^2 in r^2, which moves k by at most
+// |dk/d(r^2)| times that -- bounded for these kernels, unbounded at r -> 0
+// for kernels with a cusp there (EXPONENTIAL and the run-time kernels keep
+// the direct differences).
+template <int KT>
+constexpr bool kGramNear = KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN;
+
+// Lane l of the distance MFMA's output holds sources 4 (l / 16) + v of each
+// 16-source half of a K-step; the BF16 contraction reads k-slots
+// 8 (l / 16) .. + 7, so source s of a 32-block is staged at this position.
+__device__ __forceinline__ int gram_pos(int jj) {
+  return (jj & ~31) | (((jj & 15) >> 2) << 3) | ((jj & 16) >> 2) | (jj & 3);
 }
 
 // the 8 bf16 values at p (16-byte aligned) as one operand
@@ -84,13 +111,20 @@ __device__ __forceinline__ bf16x8 load8(const u16 *p) {
 // tiles {wave, wave + 4, wave + 8, wave + 12} of the current 256-row pass and
 // 16 NT rhs columns.  Far field: A = V_t^T (rows x q) on FP32 MFMA as in the
 // FP32 form (a small share of the work); near field on BF16 MFMA.
+// waves per SIMD the leaf output is compiled for: 3 fit the 16-column form
+// of the compile-time kernels without spilling; the 32-column form and the
+// run-time kernel switch keep 2
+template <int NT, int KT>
+constexpr int kLeafWaves = (NT == 1 && KT >= 0) ? 3 : 2;
+
 template <int D, int KT, int NT>
-__global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
+__global__ __launch_bounds__(256, (kLeafWaves<NT, KT>)) void k_leaf_output_bf16(
     DevTreeView<float> rows, DevTreeView<float> cols, DevPatternView pat,
     InterpParams<float> ip, KernelParams<float> kp, int nrhs, const float *__restrict__ xs,
     const float *__restrict__ tlhs, double *__restrict__ out, int64_t ldo, int natural) {
   constexpr int NC = 16 * NT;
   constexpr int NCP = NC + 16;  // padded stride of the staged far moments
+  constexpr bool kGram = kGramNear<KT>;
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
@@ -98,8 +132,10 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
   int *aidx = reinterpret_cast<int *>(F + D * P1 * kRowsP);     // [qpad] packed indices
   float *Un = reinterpret_cast<float *>(aidx + ((qpad + 3) & ~3));  // union region
   float *Ms = Un;                                  // far: [kFarChunk][NCP] moments
-  float *Ys = Un;                                  // near: [D][kChunk] source points
-  u16 *Xh = reinterpret_cast<u16 *>(Ys + D * kChunk);  // near: [NC][kChunkP] rhs, hi
+  // near: [D][kChunk] source points, or with kGram [4][kChunk] rows of the
+  // distance MFMA's source operand
+  float *Ys = Un;
+  u16 *Xh = reinterpret_cast<u16 *>(Ys + 4 * kChunk);  // near: [NC][kChunkP] rhs, hi
   u16 *Xl = Xh + NC * kChunkP;                         // near: [NC][kChunkP] rhs, lo
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -137,6 +173,24 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
 #pragma unroll
       for (int d = 0; d < D; ++d)
         x[rt][d] = i < sz ? rows.pts[(int64_t)d * rows.npts + b0 + i] : 0.f;
+    }
+    // distance MFMA operands of the targets: xb = component lk of [x', 1]
+    // (x' = x - centre of the leaf), xx = |x'|^2
+    float ctr[D], xb[kRT], xx[kRT];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      ctr[d] = rows.bbmin[(int64_t)c * D + d] + 0.5f * rows.bbsize[(int64_t)c * D + d];
+#pragma unroll
+    for (int rt = 0; rt < kRT; ++rt) {
+      float s2 = 0.f, b = lk == 3 ? 1.f : 0.f;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const float v = x[rt][d] - ctr[d];
+        s2 = fmat(v, v, s2);
+        if (d == lk) b = v;
+      }
+      xx[rt] = s2;
+      xb[rt] = b;
     }
     for (int rc = 0; rc < nrhs; rc += NC) {
       f32x4 acc[kRT][NT];
@@ -179,8 +233,11 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
       // near field: chunks of kChunk sources fetched one chunk ahead into
       // registers, staged as coordinates Ys and the BF16 split of the rhs
       constexpr int KX = kChunk * NC / 256;          // rhs values per thread
-      constexpr int KY = (D * kChunk + 255) / 256;  // coordinates per thread
+      // coordinates per thread: with kGram thread jj < kChunk loads all D of
+      // source jj (its row of the distance operand needs |y'|^2)
+      constexpr int KY = kGram ? D : (D * kChunk + 255) / 256;
       static_assert(kChunk * NC % 256 == 0, "staging layout");
+      static_assert(kChunk <= 256, "one source per thread in the distance staging");
       float py[KY], px[KX];
       int nb = n0, j0 = 0, pnj = 0;
       auto fetch = [&]() {
@@ -188,13 +245,21 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
         const int s = pat.near_src[nb], sl = pat.near_src[nb + run - 1];
         const int sb = cols.begin[s], ss = cols.begin[sl] + cols.size[sl] - sb;
         pnj = min(kChunk, ss - j0);
+        if constexpr (kGram) {
+          const int jj = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          const int d = e / kChunk, jj = e % kChunk;
-          py[k] = (e < D * kChunk && jj < pnj)
-                      ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj]
-                      : 0.f;
+          for (int d = 0; d < D; ++d)  // padding sources sit at the centre
+            py[d] = (jj < kChunk && jj < pnj) ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj]
+                                              : ctr[d];
+        } else {
+#pragma unroll
+          for (int k = 0; k < KY; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            const int d = e / kChunk, jj = e % kChunk;
+            py[k] = (e < D * kChunk && jj < pnj)
+                        ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj]
+                        : 0.f;
+          }
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
@@ -213,25 +278,88 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
       while (more) {
         const int nj = pnj;
         __syncthreads();
+        if constexpr (kGram) {
+          const int jj = threadIdx.x;
+          if (jj < kChunk) {
+            float s2 = 0.f;
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          if (e < D * kChunk) Ys[e] = py[k];
+            for (int d = 0; d < D; ++d) {
+              const float v = py[d] - ctr[d];
+              Ys[d * kChunk + jj] = -2.f * v;
+              s2 = fmat(v, v, s2);
+            }
+#pragma unroll
+            for (int d = D; d < 3; ++d) Ys[d * kChunk + jj] = 0.f;
+            Ys[3 * kChunk + jj] = s2;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < KY; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < D * kChunk) Ys[e] = py[k];
+          }
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
           const int e = threadIdx.x + 256 * k;
           const int jj = e / NC, r = e % NC;
+          const int pos = kGram ? gram_pos(jj) : jj;
           u16 h, l;
           split_bf16(px[k], h, l);
-          Xh[r * kChunkP + jj] = h;
-          Xl[r * kChunkP + jj] = l;
+          Xh[r * kChunkP + pos] = h;
+          Xl[r * kChunkP + pos] = l;
         }
         __syncthreads();
         more = nb < n1;
         if (more) fetch();
         const int nsteps = (nj + kKB - 1) & ~(kKB - 1);
-        for (int ks = 0; ks < nsteps; ks += kKB) {
+        for (int ks = 0; kGram && ks < nsteps; ks += kKB) {
+          const int j8 = ks + 8 * lk;  // this lane group's eight k-slots
+          // squared distances of the two 16-source halves for every row tile
+          const float ya = Ys[lk * kChunk + ks + lr], yb = Ys[lk * kChunk + ks + 16 + lr];
+          // the distances of row tile rt + 1 are issued before the entries
+          // of row tile rt are formed, so the matrix core works under the VALU
+          f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
+          if (tile_on[0]) {
+            g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
+            g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
+          }
+          // right operands of the K-step, shared by the row tiles
+          bf16x8 bh[NT], bl[NT];
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) {
+            bh[nt] = load8(Xh + (nt * 16 + lr) * kChunkP + j8);
+            bl[nt] = load8(Xl + (nt * 16 + lr) * kChunkP + j8);
+          }
+#pragma unroll
+          for (int rt = 0; rt < kRT; ++rt) {
+            f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
+            if (rt + 1 < kRT && tile_on[rt + 1]) {
+              n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
+              n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
+            }
+            const f32x4 c0 = g0, c1 = g1;
+            g0 = n0;
+            g1 = n1;
+            if (!tile_on[rt]) continue;
+            float ev[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              ev[v] = kernel_r2<KT, float>(fmaxf(c0[v] + xx[rt], 0.f), kp);
+              ev[4 + v] = kernel_r2<KT, float>(fmaxf(c1[v] + xx[rt], 0.f), kp);
+            }
+            bf16x8 ah, al;
+            split8(ev, ah, al);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+              if (nt >= ntl) continue;
+              acc[rt][nt] = mma_bf16(ah, bh[nt], acc[rt][nt]);
+              acc[rt][nt] = mma_bf16(al, bh[nt], acc[rt][nt]);
+              acc[rt][nt] = mma_bf16(ah, bl[nt], acc[rt][nt]);
+            }
+          }
+        }
+        for (int ks = 0; !kGram && ks < nsteps; ks += kKB) {
           const int j8 = ks + 8 * lk;  // this lane group's eight sources
           float y[D][8];
 #pragma unroll
@@ -309,8 +437,12 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
   const int q = ip.q;
   const int qpad = (q + 3) & ~3;
   float *U = reinterpret_cast<float *>(smem_raw);  // [D][qpad] unit node coordinates
-  float *Yc = U + D * qpad;                        // [D][kChunk] node coordinates
-  u16 *Mh = reinterpret_cast<u16 *>(Yc + D * kChunk);  // [NC][kChunkP] moments, hi
+  // [4][kChunk] rows of the distance MFMA's source operand: -2 y' (k-slots
+  // 0..D-1), |y'|^2 (k-slot 3), y' = node - centre of the target cluster.
+  // Far blocks are separated (dist >= radius / eta), so the cancellation in
+  // |y'|^2 - 2 x'.y' + |x'|^2 stays at the rounding of the direct form.
+  float *Yc = U + D * qpad;
+  u16 *Mh = reinterpret_cast<u16 *>(Yc + 4 * kChunk);  // [NC][kChunkP] moments, hi
   u16 *Ml = Mh + NC * kChunkP;                         // [NC][kChunkP] moments, lo
   float *Red = reinterpret_cast<float *>(Ml + NC * kChunkP);  // [2 kRT 16][NC] K-split sums
   const int lane = threadIdx.x & 63;
@@ -331,19 +463,26 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
     const int d = e / qpad, b = e - d * qpad, bb = b < q ? b : 0;
     U[e] = ip.td ? ip.unodes[d * q + bb] : ip.xi[unpack(basis_index<float, D>(bb, ip), d)];
   }
-  float x[kRT][D];
+  float ctr[D], xb[kRT], xx[kRT];
   bool tile_on[kRT];
+#pragma unroll
+  for (int d = 0; d < D; ++d) ctr[d] = rows.bbmin[t * D + d] + 0.5f * rows.bbsize[t * D + d];
 #pragma unroll
   for (int rt = 0; rt < kRT; ++rt) {
     const int tile_row0 = (rg + wrows * rt) * 16;
     tile_on[rt] = tile_row0 < q;
     const int row = tile_row0 + lr < q ? tile_row0 + lr : 0;
     const int pk = basis_index<float, D>(row, ip);
+    float s2 = 0.f, b = lk == 3 ? 1.f : 0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const float u = ip.td ? ip.unodes[d * q + row] : ip.xi[unpack(pk, d)];
-      x[rt][d] = u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d];
+      const float v = u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d] - ctr[d];
+      s2 = fmat(v, v, s2);
+      if (d == lk) b = v;
     }
+    xx[rt] = s2;  // distance MFMA operands of the target nodes
+    xb[rt] = b;
   }
   __syncthreads();  // U
   // source slot and node of row bl of a chunk of nsrc whole sources (G >= 2);
@@ -363,44 +502,53 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
     const int ntl = min(NT, (nrhs - rc + 15) >> 4);
     auto compute_chunk = [&](int nrows) {
       for (int ks = ks0; ks < nrows; ks += kstep) {
-        const int j8 = ks + 8 * lk;
-        float y[D][8];
+        const int j8 = ks + 8 * lk;  // this lane group's eight k-slots (gram_pos order)
+        const float ya = Yc[lk * kChunk + ks + lr], yb = Yc[lk * kChunk + ks + 16 + lr];
+        // distances of row tile rt + 1 issued before the entries of rt
+        f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
+        if (tile_on[0]) {
+          g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
+          g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
+        }
+        bf16x8 bh[NT], bl[NT];
 #pragma unroll
-        for (int d = 0; d < D; ++d)
-#pragma unroll
-          for (int i = 0; i < 8; ++i) y[d][i] = Yc[d * kChunk + j8 + i];
+        for (int nt = 0; nt < NT; ++nt) {
+          bh[nt] = load8(Mh + (nt * 16 + lr) * kChunkP + j8);
+          bl[nt] = load8(Ml + (nt * 16 + lr) * kChunkP + j8);
+        }
 #pragma unroll
         for (int rt = 0; rt < kRT; ++rt) {
+          f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
+          if (rt + 1 < kRT && tile_on[rt + 1]) {
+            n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
+            n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
+          }
+          const f32x4 c0 = g0, c1 = g1;
+          g0 = n0;
+          g1 = n1;
           if (!tile_on[rt]) continue;
           float ev[8];
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            float r2 = 0.f;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              const float dd = x[rt][d] - y[d][i];
-              r2 = fmat(dd, dd, r2);
-            }
-            ev[i] = kernel_r2<KT, float>(r2, kp);
+          for (int v = 0; v < 4; ++v) {
+            ev[v] = kernel_r2<KT, float>(fmaxf(c0[v] + xx[rt], 0.f), kp);
+            ev[4 + v] = kernel_r2<KT, float>(fmaxf(c1[v] + xx[rt], 0.f), kp);
           }
           bf16x8 ah, al;
           split8(ev, ah, al);
 #pragma unroll
           for (int nt = 0; nt < NT; ++nt) {
             if (nt >= ntl) continue;
-            const bf16x8 bh = load8(Mh + (nt * 16 + lr) * kChunkP + j8);
-            const bf16x8 bl = load8(Ml + (nt * 16 + lr) * kChunkP + j8);
-            acc[rt][nt] = mma_bf16(ah, bh, acc[rt][nt]);
-            acc[rt][nt] = mma_bf16(al, bh, acc[rt][nt]);
-            acc[rt][nt] = mma_bf16(ah, bl, acc[rt][nt]);
+            acc[rt][nt] = mma_bf16(ah, bh[nt], acc[rt][nt]);
+            acc[rt][nt] = mma_bf16(al, bh[nt], acc[rt][nt]);
+            acc[rt][nt] = mma_bf16(ah, bl[nt], acc[rt][nt]);
           }
         }
       }
     };
-    constexpr int KM = kChunk * NC / 256;          // staged moments per thread
-    constexpr int KY = (D * kChunk + 255) / 256;  // staged coordinates per thread
+    constexpr int KM = kChunk * NC / 256;  // staged moments per thread
     static_assert(kChunk * NC % 256 == 0, "staging layout");
-    float py[KY], pm[KM];
+    static_assert(kChunk <= 256, "one node per thread in the distance staging");
+    float py[D], pm[KM];  // py: the D coordinates of node threadIdx.x of the chunk
     int f = f0, nbe = 0;
     int pn = 0;
     auto fetch = [&]() {
@@ -408,17 +556,14 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
       const int be = nbe;
       pn = G > 1 ? nsrc * qpad : min(kChunk, qpad - be);
       if (G > 1) {
+        if (threadIdx.x < kChunk) {
+          int slot, node;
+          chunk_row(threadIdx.x, nsrc, slot, node);
+          const int s = pat.far_src[f + slot];
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          const int d = e / kChunk, bl = e % kChunk;
-          if (e < D * kChunk) {
-            int slot, node;
-            chunk_row(bl, nsrc, slot, node);
-            const int s = pat.far_src[f + slot];
-            py[k] = U[d * qpad + (node < 0 ? 0 : node)] * cols.bbsize[s * D + d] +
+          for (int d = 0; d < D; ++d)
+            py[d] = U[d * qpad + (node < 0 ? 0 : node)] * cols.bbsize[s * D + d] +
                     cols.bbmin[s * D + d];
-          }
         }
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
@@ -432,14 +577,11 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
       } else {
         const int s = pat.far_src[f];
         const float *ms = mom + (int64_t)s * q * nrhs;
+        if (threadIdx.x < kChunk) {
+          const int node = be + (int)threadIdx.x < q ? be + (int)threadIdx.x : 0;
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          const int d = e / kChunk, bl = e % kChunk;
-          if (e < D * kChunk) {
-            const int node = be + bl < q ? be + bl : 0;
-            py[k] = U[d * qpad + node] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
-          }
+          for (int d = 0; d < D; ++d)
+            py[d] = U[d * qpad + node] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
         }
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
@@ -460,10 +602,18 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
       }
     };
     auto stage = [&]() {
+      if (threadIdx.x < kChunk) {
+        const int bl = threadIdx.x;
+        float s2 = 0.f;
 #pragma unroll
-      for (int k = 0; k < KY; ++k) {
-        const int e = threadIdx.x + 256 * k;
-        if (e < D * kChunk) Yc[e] = py[k];
+        for (int d = 0; d < D; ++d) {
+          const float v = py[d] - ctr[d];
+          Yc[d * kChunk + bl] = -2.f * v;
+          s2 = fmat(v, v, s2);
+        }
+#pragma unroll
+        for (int d = D; d < 3; ++d) Yc[d * kChunk + bl] = 0.f;
+        Yc[3 * kChunk + bl] = s2;
       }
 #pragma unroll
       for (int k = 0; k < KM; ++k) {
@@ -471,8 +621,8 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
         const int bl = e / NC, r = e % NC;
         u16 h, l;
         split_bf16(pm[k], h, l);
-        Mh[r * kChunkP + bl] = h;
-        Ml[r * kChunkP + bl] = l;
+        Mh[r * kChunkP + gram_pos(bl)] = h;
+        Ml[r * kChunkP + gram_pos(bl)] = l;
       }
     };
     // no register lookahead of the next chunk here: its 18 values per thread
@@ -545,7 +695,7 @@ size_t leaf_output_bf16_lds(int D, int P1, int q) {
   constexpr int NC = 16 * NT;
   const int qpad = (q + 3) & ~3;
   const size_t far = (size_t)kFarChunk * (NC + 16) * sizeof(float);
-  const size_t near = (size_t)D * kChunk * sizeof(float) + (size_t)2 * NC * kChunkP * sizeof(u16);
+  const size_t near = (size_t)4 * kChunk * sizeof(float) + (size_t)2 * NC * kChunkP * sizeof(u16);
   return (size_t)D * P1 * kRowsP * sizeof(float) + (size_t)((qpad + 3) & ~3) * sizeof(int) +
          (far > near ? far : near);
 }
@@ -555,7 +705,7 @@ size_t coupling_bf16_lds(int D, int q) {
   constexpr int NC = 16 * NT;
   const int qpad = (q + 3) & ~3;
   const size_t red = q <= 2 * kRT * 16 ? (size_t)2 * kRT * 16 * NC * sizeof(float) : 0;
-  return (size_t)D * qpad * sizeof(float) + (size_t)D * kChunk * sizeof(float) +
+  return (size_t)D * qpad * sizeof(float) + (size_t)4 * kChunk * sizeof(float) +
          (size_t)2 * NC * kChunkP * sizeof(u16) + red;
 }
 

@@ -171,7 +171,11 @@ def test_action_replays_in_a_hip_graph(precision):
 BF16_TOL = 2e-5  # FP32 entries (~4e-6 here) plus the dropped 2^-16 terms of the split
 
 
-@pytest.mark.parametrize("case", FP32_CASES, ids=[c[0] for c in FP32_CASES])
+# every case where the BF16 matrix-core forms run (>= 8 rhs), each kernel type
+BF16_CASES = [c for c in CASES if c[8] >= 8]
+
+
+@pytest.mark.parametrize("case", BF16_CASES, ids=[c[0] for c in BF16_CASES])
 def test_fp32_bf16_action_close_to_fp64(case):
     """The FP32 form with BF16 three-term matrix-core contractions (and its
     FP32 fallbacks below 8 rhs) against the FP64 action."""
