@@ -252,10 +252,13 @@ bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vma
 // BF16 three-term forms of the FP32 leaf output / coupling (kernels_bf16.hip):
 // kernel values only, >= kMfmaMinRhs rhs, where the LDS staging fits.
 bool bf16_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask);
+// The right operand the launch reads (xs for the leaf output, mom for the
+// coupling) is split into BF16 halves in place first, so it holds hi | lo << 16
+// words afterwards.
 hipError_t launch_bf16(bool output, const DevTreeView<float> &rows,
                        const DevTreeView<float> &cols, const DevPatternView &pat,
                        const InterpParams<float> &ip, const KernelParams<float> &kp, int nrhs,
-                       const float *xs, const float *mom, float *tlhs, double *out, int64_t ldo,
+                       float *xs, float *mom, float *tlhs, double *out, int64_t ldo,
                        bool natural, hipStream_t stream);
 
 template <typename T>

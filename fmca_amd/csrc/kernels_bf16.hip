@@ -23,7 +23,9 @@
 // C[4 (l / 16) + v][l % 16].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "device.hpp"
 #include "device_math.hpp"
@@ -106,19 +108,25 @@ __device__ __forceinline__ bf16x8 load8(const u16 *p) {
   return *reinterpret_cast<const bf16x8 *>(p);
 }
 
+// In-place BF16 split of the right operands of one action (the permuted rhs
+// before the leaf output, the source moments before the coupling): x[i]
+// becomes hi | lo << 16, so the staging loops copy the two halves instead
+// of splitting every value again for every target that reads it.
+__global__ __launch_bounds__(256) void k_split_inplace(float *__restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    u16 h, l;
+    split_bf16(x[i], h, l);
+    x[i] = __builtin_bit_cast(float, (uint32_t)h | ((uint32_t)l << 16));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Leaf output: one workgroup per target-leaf task, 4 waves, each wave owns row
 // tiles {wave, wave + 4, wave + 8, wave + 12} of the current 256-row pass and
 // 16 NT rhs columns.  Far field: A = V_t^T (rows x q) on FP32 MFMA as in the
 // FP32 form (a small share of the work); near field on BF16 MFMA.
-// waves per SIMD the leaf output is compiled for: 3 fit the 16-column form
-// of the compile-time kernels without spilling; the 32-column form and the
-// run-time kernel switch keep 2
-template <int NT, int KT>
-constexpr int kLeafWaves = (NT == 1 && KT >= 0) ? 3 : 2;
-
 template <int D, int KT, int NT>
-__global__ __launch_bounds__(256, (kLeafWaves<NT, KT>)) void k_leaf_output_bf16(
+__global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
     DevTreeView<float> rows, DevTreeView<float> cols, DevPatternView pat,
     InterpParams<float> ip, KernelParams<float> kp, int nrhs, const float *__restrict__ xs,
     const float *__restrict__ tlhs, double *__restrict__ out, int64_t ldo, int natural) {
@@ -304,60 +312,72 @@ __global__ __launch_bounds__(256, (kLeafWaves<NT, KT>)) void k_leaf_output_bf16(
           const int e = threadIdx.x + 256 * k;
           const int jj = e / NC, r = e % NC;
           const int pos = kGram ? gram_pos(jj) : jj;
-          u16 h, l;
-          split_bf16(px[k], h, l);
-          Xh[r * kChunkP + pos] = h;
-          Xl[r * kChunkP + pos] = l;
+          const uint32_t w = __builtin_bit_cast(uint32_t, px[k]);  // hi | lo << 16
+          Xh[r * kChunkP + pos] = (u16)(w & 0xffffu);
+          Xl[r * kChunkP + pos] = (u16)(w >> 16);
         }
         __syncthreads();
         more = nb < n1;
         if (more) fetch();
         const int nsteps = (nj + kKB - 1) & ~(kKB - 1);
-        for (int ks = 0; kGram && ks < nsteps; ks += kKB) {
-          const int j8 = ks + 8 * lk;  // this lane group's eight k-slots
-          // squared distances of the two 16-source halves for every row tile
-          const float ya = Ys[lk * kChunk + ks + lr], yb = Ys[lk * kChunk + ks + 16 + lr];
-          // the distances of row tile rt + 1 are issued before the entries
-          // of row tile rt are formed, so the matrix core works under the VALU
-          f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
-          if (tile_on[0]) {
-            g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
-            g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
-          }
-          // right operands of the K-step, shared by the row tiles
-          bf16x8 bh[NT], bl[NT];
+        // all four row tiles valid (the common case): no per-tile branches,
+        // so the four entry chains of a K-step can be interleaved
+        bool all_on = true;
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) {
-            bh[nt] = load8(Xh + (nt * 16 + lr) * kChunkP + j8);
-            bl[nt] = load8(Xl + (nt * 16 + lr) * kChunkP + j8);
-          }
-#pragma unroll
-          for (int rt = 0; rt < kRT; ++rt) {
-            f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
-            if (rt + 1 < kRT && tile_on[rt + 1]) {
-              n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
-              n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
+        for (int rt = 0; rt < kRT; ++rt) all_on = all_on && tile_on[rt];
+        auto gram_steps = [&](auto all_t) {
+          constexpr bool ALL = decltype(all_t)::value;
+          for (int ks = 0; ks < nsteps; ks += kKB) {
+            const int j8 = ks + 8 * lk;  // this lane group's eight k-slots
+            // the distances of row tile rt + 1 are issued before the entries
+            // of row tile rt are formed, so the matrix core works under the VALU
+            const float ya = Ys[lk * kChunk + ks + lr], yb = Ys[lk * kChunk + ks + 16 + lr];
+            f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
+            if (ALL || tile_on[0]) {
+              g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
+              g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
             }
-            const f32x4 c0 = g0, c1 = g1;
-            g0 = n0;
-            g1 = n1;
-            if (!tile_on[rt]) continue;
-            float ev[8];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              ev[v] = kernel_r2<KT, float>(fmaxf(c0[v] + xx[rt], 0.f), kp);
-              ev[4 + v] = kernel_r2<KT, float>(fmaxf(c1[v] + xx[rt], 0.f), kp);
-            }
-            bf16x8 ah, al;
-            split8(ev, ah, al);
+            // right operands of the K-step, shared by the row tiles
+            bf16x8 bh[NT], bl[NT];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-              if (nt >= ntl) continue;
-              acc[rt][nt] = mma_bf16(ah, bh[nt], acc[rt][nt]);
-              acc[rt][nt] = mma_bf16(al, bh[nt], acc[rt][nt]);
-              acc[rt][nt] = mma_bf16(ah, bl[nt], acc[rt][nt]);
+              bh[nt] = load8(Xh + (nt * 16 + lr) * kChunkP + j8);
+              bl[nt] = load8(Xl + (nt * 16 + lr) * kChunkP + j8);
+            }
+#pragma unroll
+            for (int rt = 0; rt < kRT; ++rt) {
+              f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
+              if (rt + 1 < kRT && (ALL || tile_on[rt + 1])) {
+                n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
+                n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
+              }
+              const f32x4 c0 = g0, c1 = g1;
+              g0 = n0;
+              g1 = n1;
+              if (!ALL && !tile_on[rt]) continue;
+              float ev[8];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                ev[v] = kernel_r2<KT, float>(fmaxf(c0[v] + xx[rt], 0.f), kp);
+                ev[4 + v] = kernel_r2<KT, float>(fmaxf(c1[v] + xx[rt], 0.f), kp);
+              }
+              bf16x8 ah, al;
+              split8(ev, ah, al);
+#pragma unroll
+              for (int nt = 0; nt < NT; ++nt) {
+                if (nt >= ntl) continue;
+                acc[rt][nt] = mma_bf16(ah, bh[nt], acc[rt][nt]);
+                acc[rt][nt] = mma_bf16(al, bh[nt], acc[rt][nt]);
+                acc[rt][nt] = mma_bf16(ah, bl[nt], acc[rt][nt]);
+              }
             }
           }
+        };
+        if constexpr (kGram) {
+          if (all_on)
+            gram_steps(std::true_type{});
+          else
+            gram_steps(std::false_type{});
         }
         for (int ks = 0; !kGram && ks < nsteps; ks += kKB) {
           const int j8 = ks + 8 * lk;  // this lane group's eight sources
@@ -500,13 +520,19 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = f32x4{0, 0, 0, 0};
     const int ntl = min(NT, (nrhs - rc + 15) >> 4);
-    auto compute_chunk = [&](int nrows) {
+    bool all_on = true;
+#pragma unroll
+    for (int rt = 0; rt < kRT; ++rt) all_on = all_on && tile_on[rt];
+    // all four row tiles valid (the common case): no per-tile branches, so
+    // the four entry chains of a K-step can be interleaved
+    auto chunk_steps = [&](int nrows, auto all_t) {
+      constexpr bool ALL = decltype(all_t)::value;
       for (int ks = ks0; ks < nrows; ks += kstep) {
         const int j8 = ks + 8 * lk;  // this lane group's eight k-slots (gram_pos order)
         const float ya = Yc[lk * kChunk + ks + lr], yb = Yc[lk * kChunk + ks + 16 + lr];
         // distances of row tile rt + 1 issued before the entries of rt
         f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
-        if (tile_on[0]) {
+        if (ALL || tile_on[0]) {
           g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
           g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
         }
@@ -519,14 +545,14 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
 #pragma unroll
         for (int rt = 0; rt < kRT; ++rt) {
           f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
-          if (rt + 1 < kRT && tile_on[rt + 1]) {
+          if (rt + 1 < kRT && (ALL || tile_on[rt + 1])) {
             n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
             n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
           }
           const f32x4 c0 = g0, c1 = g1;
           g0 = n0;
           g1 = n1;
-          if (!tile_on[rt]) continue;
+          if (!ALL && !tile_on[rt]) continue;
           float ev[8];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
@@ -544,6 +570,12 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
           }
         }
       }
+    };
+    auto compute_chunk = [&](int nrows) {
+      if (all_on)
+        chunk_steps(nrows, std::true_type{});
+      else
+        chunk_steps(nrows, std::false_type{});
     };
     constexpr int KM = kChunk * NC / 256;  // staged moments per thread
     static_assert(kChunk * NC % 256 == 0, "staging layout");
@@ -619,10 +651,9 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
       for (int k = 0; k < KM; ++k) {
         const int e = threadIdx.x + 256 * k;
         const int bl = e / NC, r = e % NC;
-        u16 h, l;
-        split_bf16(pm[k], h, l);
-        Mh[r * kChunkP + gram_pos(bl)] = h;
-        Ml[r * kChunkP + gram_pos(bl)] = l;
+        const uint32_t w = __builtin_bit_cast(uint32_t, pm[k]);  // hi | lo << 16
+        Mh[r * kChunkP + gram_pos(bl)] = (u16)(w & 0xffffu);
+        Ml[r * kChunkP + gram_pos(bl)] = (u16)(w >> 16);
       }
     };
     // no register lookahead of the next chunk here: its 18 values per thread
@@ -775,8 +806,18 @@ bool bf16_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vma
 hipError_t launch_bf16(bool output, const DevTreeView<float> &rows,
                        const DevTreeView<float> &cols, const DevPatternView &pat,
                        const InterpParams<float> &ip, const KernelParams<float> &kp, int nrhs,
-                       const float *xs, const float *mom, float *tlhs, double *out, int64_t ldo,
+                       float *xs, float *mom, float *tlhs, double *out, int64_t ldo,
                        bool natural, hipStream_t stream) {
+  {
+    // the right operand this launch stages, split once here (rewritten by
+    // the next action's permutation / upward pass)
+    float *x = output ? xs : mom;
+    const int64_t n = (output ? cols.npts : (int64_t)cols.ncl * ip.q) * nrhs;
+    if (n > 0) {
+      const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+      hipLaunchKernelGGL(k_split_inplace, dim3(grid), dim3(256), 0, stream, x, n);
+    }
+  }
   switch (rows.dim) {
     case 1:
       return launch_bf16_d<1>(output, rows, cols, pat, ip, kp, nrhs, xs, mom, tlhs, out, ldo,
