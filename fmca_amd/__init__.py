@@ -75,12 +75,28 @@ class CovarianceKernel:
         self.l = float(l)
         self.c = float(c)
         self.nu = float(nu)
+        self._dist = "EUCLIDEAN"
 
     def kernelType(self) -> str:
         return self._ktype
 
+    def setDistanceType(self, dist_type: str):
+        """CovarianceKernel::setDistanceType (CovarianceKernel.h:123-137):
+        "EUCLIDEAN" (the default) or "GEODESIC", acos(clamp(x . y, -1, 1)) as in
+        SphereClusterTree::geodesicDistance (SphereClusterTree.h:65-69).  Actors
+        built from this kernel (or given it by set_kernel) take it over; their
+        geodesic entries run in FP64."""
+        d = str(dist_type).upper()
+        if d not in ("EUCLIDEAN", "GEODESIC"):
+            raise ValueError(f"desired distance not implemented: {dist_type!r}")
+        self._dist = d
+
+    def distanceType(self) -> str:
+        return self._dist
+
     def __repr__(self):
-        return f"CovarianceKernel({self._ktype!r}, l={self.l}, c={self.c}, nu={self.nu})"
+        geo = ", GEODESIC" if self._dist == "GEODESIC" else ""
+        return f"CovarianceKernel({self._ktype!r}, l={self.l}, c={self.c}, nu={self.nu}{geo})"
 
 
 def _points(P, name):
@@ -144,6 +160,7 @@ class H2kernelActor:
         self._a = _C.Actor(kernel.kernelType(), kernel.l, kernel.c, kernel.nu, Peval, P,
                            self.deg, self.eta, INTERPOLATORS[interpolator],
                            planner_id)
+        self._a.set_distance_type(kernel.distanceType())
         self.m = self._a.m
         self.n = self._a.n
         self.dim = self._a.dim
@@ -220,6 +237,7 @@ class H2kernelActor:
     def set_kernel(self, kernel: CovarianceKernel):
         """Swap the kernel (type and parameters); the plan is kept."""
         self._a.set_kernel(kernel.kernelType(), kernel.l, kernel.c, kernel.nu)
+        self._a.set_distance_type(kernel.distanceType())
         self.kernel = kernel
 
     # -- solver ----------------------------------------------------------------
@@ -455,6 +473,8 @@ class SampletKernelCompressor:
         self.n = hst.n
         self.eta = float(eta)
         self.thres = float(thres)
+        if ker.distanceType() != "EUCLIDEAN":
+            raise ValueError("the samplet compressor takes the Euclidean distance")
         self._m = hst._t.compress(ker.kernelType(), ker.l, ker.c, ker.nu, self.eta, self.thres)
         return self
 

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -187,6 +188,7 @@ struct fmca_h2_actor {
   double eta = 0;
   int kt = 0;
   double l = 1, c = 1, nu = 1;
+  int geodesic = 0;  // distance type: 0 Euclidean, 1 geodesic (fmca_h2_set_distance_type)
   Interpolation ip;
   Tree rows_tree, cols_tree;
   bool shared = false;  // Peval and P are the same point set: one tree
@@ -443,6 +445,7 @@ KernelParams<T> kernel_params(const fmca_h2_actor *a, const VariantGroups &g) {
   k.sl2 = (T)(sc * 1.4426950408889634074);
   k.inv_l = (T)(1.0 / a->l);
   k.c2 = (T)(a->c * a->c);
+  k.geodesic = a->geodesic;
   return k;
 }
 
@@ -643,6 +646,19 @@ fmca_status enqueue_any_unordered(fmca_h2_actor *a, const double *d_rhs, int64_t
                                   bool natural, bool transpose, hipStream_t stream,
                                   const VariantGroups &groups, const Region *fixed,
                                   const GradJob *grad) {
+  if (a->geodesic) {
+    // near the diagonal acos turns the rounding of an FP32 dot product into
+    // an error of about 1e-7 / r in r, and the derivative in the points of a
+    // geodesic kernel is not the Euclidean one of the gradient kernels
+    if (precision != FMCA_PRECISION_FP64)
+      return fail(FMCA_ERR_NOT_SUPPORTED, "the geodesic distance is evaluated in FP64 only");
+    bool phi = grad != nullptr;
+    for (int i = 0; i < groups.n; ++i) phi = phi || groups.v[i] == KV_PHI;
+    if (phi)
+      return fail(FMCA_ERR_NOT_SUPPORTED,
+                  "derivatives in the points are Euclidean only; the geodesic distance "
+                  "supports kernel values and d/dl");
+  }
   if (transpose && !a->shared && !a->have_pat_t) {
     compute_pattern(a->pat_t, a->cols(), a->rows(), a->eta, a->ip.q);
     HIP_TRY(a->d_pat_t.upload(a->pat_t));
@@ -996,12 +1012,37 @@ fmca_status fmca_h2_set_kernel(fmca_h2_actor *actor, const fmca_kernel_desc *ker
   return FMCA_OK;
 }
 
+fmca_status fmca_h2_set_distance_type(fmca_h2_actor *actor, const char *dist_type) {
+  if (!actor || !dist_type) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  std::string d(dist_type);
+  for (auto &ch : d) ch = (char)std::toupper((unsigned char)ch);
+  int geo;
+  if (d == "EUCLIDEAN")
+    geo = 0;
+  else if (d == "GEODESIC")
+    geo = 1;
+  else
+    return fail(FMCA_ERR_INVALID_ARGUMENT, "desired distance not implemented: " + d);
+  std::lock_guard<std::mutex> lock(actor->mtx);
+  if (geo != actor->geodesic) actor->release_stored();
+  actor->geodesic = geo;
+  return FMCA_OK;
+}
+
+fmca_status fmca_h2_distance_type(const fmca_h2_actor *actor, int *geodesic) {
+  if (!actor || !geodesic) return fail(FMCA_ERR_INVALID_ARGUMENT, "null pointer");
+  *geodesic = actor->geodesic;
+  return FMCA_OK;
+}
+
 fmca_status fmca_h2_assemble(fmca_h2_actor *actor, int precision, int64_t *bytes) {
   if (!actor) return fail(FMCA_ERR_INVALID_ARGUMENT, "actor is null");
   if (precision != FMCA_PRECISION_FP64 && precision != FMCA_PRECISION_FP32)
     return fail(FMCA_ERR_INVALID_ARGUMENT, "unknown precision");
   if (actor->ip.q > kStoredMaxRows / 2)
     return fail(FMCA_ERR_NOT_SUPPORTED, "assembled blocks need q <= 128");
+  if (actor->geodesic && precision != FMCA_PRECISION_FP64)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the geodesic distance is evaluated in FP64 only");
   std::lock_guard<std::mutex> lock(actor->mtx);
   fmca_status st = ensure_device(actor);
   if (st != FMCA_OK) return st;
@@ -1106,6 +1147,8 @@ fmca_status fmca_h2_gradient_device(fmca_h2_actor *a, const double *d_alpha, int
     return fail(FMCA_ERR_NOT_SUPPORTED, "the gradient form needs one point set (targets = sources)");
   if (a->ip.scheme == IS_TOTAL_DEGREE)
     return fail(FMCA_ERR_NOT_SUPPORTED, "the gradient form needs the tensor-product interpolator");
+  if (a->geodesic)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the gradient form takes the Euclidean distance");
   if (ld_alpha < a->n || ld_g < a->n || ld_dalpha < a->n || ld_points < a->n)
     return fail(FMCA_ERR_INVALID_ARGUMENT, "leading dimension");
   if (precision == FMCA_PRECISION_FP32_BF16) precision = FMCA_PRECISION_FP32;
@@ -1266,6 +1309,8 @@ fmca_status fmca_h2_project(fmca_h2_actor *a, const double *M, int64_t ld, int p
     return fail(FMCA_ERR_NOT_SUPPORTED, "the projector takes the tensor-product interpolator");
   if (a->ip.q > kStoredMaxRows / 2)
     return fail(FMCA_ERR_NOT_SUPPORTED, "projected blocks need q <= 128");
+  if (a->geodesic && precision != FMCA_PRECISION_FP64)
+    return fail(FMCA_ERR_NOT_SUPPORTED, "the geodesic distance is evaluated in FP64 only");
   std::lock_guard<std::mutex> lock(a->mtx);
   ProjectedMatrix pm;
   std::string err;

@@ -90,14 +90,15 @@ __global__ __launch_bounds__(kTB) void k_assemble_far(DevTreeView<T> rows, DevTr
     tile_pos<T>(e, Kq, i, j);
     T v = T(0);
     if (i < q && j < q) {
-      T r2 = T(0);
+      T r2 = T(0), x[D], y[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const T x = fmat(unit_node<T, D>(i, d, ip), rows.bbsize[t * D + d], rows.bbmin[t * D + d]);
-        const T y = fmat(unit_node<T, D>(j, d, ip), cols.bbsize[s * D + d], cols.bbmin[s * D + d]);
-        const T dd = x - y;
+        x[d] = fmat(unit_node<T, D>(i, d, ip), rows.bbsize[t * D + d], rows.bbmin[t * D + d]);
+        y[d] = fmat(unit_node<T, D>(j, d, ip), cols.bbsize[s * D + d], cols.bbmin[s * D + d]);
+        const T dd = x[d] - y[d];
         r2 = fmat(dd, dd, r2);
       }
+      if (kp.geodesic) r2 = geodesic_r2<T, D>(x, y, 1);
       v = kernel_r2<-1, T>(r2, kp);
     }
     dst[e] = v;
@@ -149,12 +150,15 @@ __global__ __launch_bounds__(kTB) void k_assemble_near(DevTreeView<T> rows, DevT
     tile_pos<T>(e, Ks, i, j);
     T v = T(0);
     if (i < st && j < ss) {
-      T r2 = T(0);
+      T r2 = T(0), x[D], y[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const T dd = rows.pts[(int64_t)d * rows.npts + bt + i] - cols.pts[(int64_t)d * cols.npts + bs + j];
+        x[d] = rows.pts[(int64_t)d * rows.npts + bt + i];
+        y[d] = cols.pts[(int64_t)d * cols.npts + bs + j];
+        const T dd = x[d] - y[d];
         r2 = fmat(dd, dd, r2);
       }
+      if (kp.geodesic) r2 = geodesic_r2<T, D>(x, y, 1);
       v = kernel_r2<-1, T>(r2, kp);
     }
     dst[e] = v;

@@ -100,6 +100,10 @@ struct KernelParams {
   int vmask = 1;
   int tv[kMaxVariantGroups] = {0, 0, 0, 0, 0, 0, 0, 0};
   int split_bf16 = 0;  // FP32 form: kernel-value contractions on BF16 matrix cores
+  // distance of the kernel entries: 0 Euclidean, 1 geodesic on the unit
+  // sphere, acos(clamp(x . y, -1, 1)) (CovarianceKernel.h:123-137 with
+  // SphereClusterTree.h:65-69); geodesic entries run in FP64 on the VALU forms
+  int geodesic = 0;
 };
 
 enum KernelVariant : int { KV_VALUE = 0, KV_DL = 1, KV_PHI = 2 };

@@ -90,6 +90,23 @@ __device__ __forceinline__ double fmat(double a, double b, double c) {
 }
 __device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Squared geodesic distance of FMCA's "GEODESIC" distance type
+// (CovarianceKernel.h:130-133, SphereClusterTree.h:65-69): r = acos of the
+// dot product clamped to [-1, 1], returned as r * r for kernel_r2.  The dot
+// product is x[0] y[0] followed by one fma per further axis, in axis order,
+// with `ys` the stride between the axes of y: near the diagonal acos
+// amplifies a rounding of the dot product by 1 / r, so every evaluation of
+// the product (and the oracle) forms it in this one order.
+template <typename T, int D>
+__device__ __forceinline__ T geodesic_r2(const T *x, const T *y, int ys) {
+  T dot = x[0] * y[0];
+#pragma unroll
+  for (int d = 1; d < D; ++d) dot = fmat(x[d], y[d * ys], dot);
+  const T c = dot > T(1) ? T(1) : (dot < T(-1) ? T(-1) : dot);
+  const T r = acos(c);
+  return r * r;
+}
+
 // exp(-u * s): in FP32 one hardware exp2 of u * (s log2 e), the product
 // formed on the host (kp.sl2); in FP64 the exp above.
 template <typename T>

@@ -248,7 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_upward_transfer(DevTreeView<T> tr, I
 // 4. coupling: tlhs[t] = sum_{s in far(t)} K(xi_t, xi_s) mom[s]
 // (unsymmetricNystromEvaluator.h:42-58 with invV = I, H2kernelActor.h:64-66).
 // One workgroup per target cluster; thread <-> target interpolation node.
-template <typename T, int D, int KT, int NR>
+// GEO: geodesic entries (KernelParams::geodesic), one source node at a time.
+template <typename T, int D, int KT, int NR, bool GEO = false>
 __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTreeView<T> cols,
                                                      DevPatternView pat, InterpParams<T> ip,
                                                      KernelParams<T> kp, int nrhs,
@@ -301,7 +302,26 @@ __global__ __launch_bounds__(kBlock) void k_coupling(DevTreeView<T> rows, DevTre
           Mv[e] = r < nr ? mom[((int64_t)s * q + beta) * nrhs + rc + r] : T(0);
         }
         __syncthreads();
-        if (alpha < q && ip.td) {
+        if constexpr (GEO) {
+          // source node beta: axis i at Y[i * ys + b_i], b_0 fastest in the
+          // tensor case
+          int b[D];
+#pragma unroll
+          for (int i = 0; i < D; ++i) b[i] = 0;
+          for (int beta = 0; alpha < q && beta < q; ++beta) {
+            T y[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) y[i] = Y[i * ys + (ip.td ? beta : b[i])];
+            const T kv = kernel_r2<KT, T>(geodesic_r2<T, D>(x, y, 1), kp);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += kv * Mv[beta * NR + r];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+              if (++b[i] < P1) break;
+              b[i] = 0;
+            }
+          }
+        } else if (alpha < q && ip.td) {
           // total degree: nodes without tensor structure
           for (int beta = 0; beta < q; ++beta) {
             T r2 = 0;
@@ -551,7 +571,7 @@ __host__ __device__ constexpr size_t leaf_output_union_elems(int D) {
              : (size_t)D * kSrcTile + (size_t)kSrcTile * NR;
 }
 
-template <typename T, int D, int KT, int NR>
+template <typename T, int D, int KT, int NR, bool GEO = false>
 __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, DevTreeView<T> cols,
                                                         DevPatternView pat, InterpParams<T> ip,
                                                         KernelParams<T> kp, int nrhs,
@@ -647,10 +667,14 @@ __global__ __launch_bounds__(kBlock) void k_leaf_output(DevTreeView<T> rows, Dev
           if (act) {
             for (int jj = 0; jj < nj; ++jj) {
               T r2 = 0;
+              if constexpr (GEO) {
+                r2 = geodesic_r2<T, D>(x, Ys + jj, kSrcTile);
+              } else {
 #pragma unroll
-              for (int d = 0; d < D; ++d) {
-                const T dd = x[d] - Ys[d * kSrcTile + jj];
-                r2 += dd * dd;
+                for (int d = 0; d < D; ++d) {
+                  const T dd = x[d] - Ys[d * kSrcTile + jj];
+                  r2 += dd * dd;
+                }
               }
               const T kv = kernel_r2<KT, T>(r2, kp);
 #pragma unroll
@@ -686,7 +710,7 @@ size_t coupling_lds(int D, int q, bool td) {
   return ((size_t)D * (td ? q : kMaxP1) + (size_t)q * NR) * sizeof(T);
 }
 
-template <typename T, int D, int KT, int NR>
+template <typename T, int D, int KT, int NR, bool GEO = false>
 hipError_t launch_far_near(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                            const DevPatternView &pat, const InterpParams<T> &ip,
                            const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf,
@@ -694,18 +718,18 @@ hipError_t launch_far_near(const DevTreeView<T> &rows, const DevTreeView<T> &col
   const int q = ip.q;
   const int threads = q <= 64 ? 64 : (q <= 128 ? 128 : kBlock);
   const size_t lds = coupling_lds<T, NR>(D, q, ip.td != 0);
-  hipLaunchKernelGGL((k_coupling<T, D, KT, NR>), dim3(pat.nfar_order), dim3(threads), lds, stream, rows,
+  hipLaunchKernelGGL((k_coupling<T, D, KT, NR, GEO>), dim3(pat.nfar_order), dim3(threads), lds, stream, rows,
                      cols, pat, ip, kp, nrhs, buf.mom, buf.tlhs);
   return hipGetLastError();
 }
 
-template <typename T, int D, int KT, int NR>
+template <typename T, int D, int KT, int NR, bool GEO = false>
 hipError_t launch_output(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
                          const DevPatternView &pat, const InterpParams<T> &ip,
                          const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
                          int64_t ldo, bool natural, hipStream_t stream) {
   const size_t lds = output_lds<T, NR>(D, ip.P1);
-  hipLaunchKernelGGL((k_leaf_output<T, D, KT, NR>), dim3(pat.nnear_order), dim3(kBlock), lds,
+  hipLaunchKernelGGL((k_leaf_output<T, D, KT, NR, GEO>), dim3(pat.nnear_order), dim3(kBlock), lds,
                      stream, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.tlhs, out, ldo,
                      natural ? 1 : 0);
   return hipGetLastError();
@@ -722,6 +746,15 @@ hipError_t dispatch_kt(bool output, const DevTreeView<T> &rows, const DevTreeVie
                                                ldo, natural, stream)                    \
                   : launch_far_near<T, D, K, NR>(rows, cols, pat, ip, kp, nrhs, buf, out, \
                                                  ldo, natural, stream);
+  if (kp.geodesic) {
+    // geodesic entries: FP64 only, the kernel type switched at run time
+    if constexpr (sizeof(T) == 8)
+      return output ? launch_output<T, D, -1, NR, true>(rows, cols, pat, ip, kp, nrhs, buf, out,
+                                                        ldo, natural, stream)
+                    : launch_far_near<T, D, -1, NR, true>(rows, cols, pat, ip, kp, nrhs, buf,
+                                                          out, ldo, natural, stream);
+    return hipErrorInvalidValue;
+  }
   switch (kp.kt) {
     FMCA_KT_CASE(KT_EXPONENTIAL)
     FMCA_KT_CASE(KT_MATERN32)
@@ -761,11 +794,12 @@ hipError_t dispatch_nr(bool output, const DevTreeView<T> &rows, const DevTreeVie
                        const KernelParams<T> &kp, int nrhs, ActionBuffers<T> buf, double *out,
                        int64_t ldo, bool natural, hipStream_t stream) {
   if constexpr (sizeof(T) == 4) {
+    if (kp.geodesic) return hipErrorInvalidValue;
     if (kp.split_bf16 && bf16_supported(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask))
       return launch_bf16(output, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.mom, buf.tlhs, out,
                          ldo, natural, stream);
   }
-  if (mfma_supported<T>(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask, ip.td))
+  if (!kp.geodesic && mfma_supported<T>(output, D, ip.P1, ip.q, nrhs, kp.kt, kp.vmask, ip.td))
     return launch_mfma<T>(output, rows, cols, pat, ip, kp, nrhs, buf.xs, buf.mom, buf.tlhs, out,
                           ldo, natural, stream);
   // the VALU forms evaluate one variant per launch (kp.kt)

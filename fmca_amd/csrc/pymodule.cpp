@@ -359,6 +359,16 @@ PYBIND11_MODULE(_fmca_amd_py, m) {
              fmca_kernel_desc k{ktype.c_str(), l, c, nu};
              check(fmca_h2_set_kernel(a.h, &k));
            })
+      .def("set_distance_type",
+           [](Actor &a, const std::string &dist) {
+             check(fmca_h2_set_distance_type(a.h, dist.c_str()));
+           })
+      .def("distance_type",
+           [](const Actor &a) {
+             int g = 0;
+             check(fmca_h2_distance_type(a.h, &g));
+             return std::string(g ? "GEODESIC" : "EUCLIDEAN");
+           })
       .def(
           "action_variants_device",
           [](Actor &a, std::vector<int> variants, bool transpose, uintptr_t rhs, int64_t ld_rhs,

@@ -291,6 +291,25 @@ fmca_status fmca_h2_cg_device(fmca_h2_actor *actor, const double *d_b, int64_t l
  * (FMCA rebuilds the evaluator per action, py/FMCA.cpp:233-235). */
 fmca_status fmca_h2_set_kernel(fmca_h2_actor *actor, const fmca_kernel_desc *kernel);
 
+/* Replaces CovarianceKernel::setDistanceType(dist_type)
+ * (FMCA/src/CovarianceKernel/CovarianceKernel.h:123-137) for the actor's
+ * kernel: "EUCLIDEAN" (the default), |x - y|, or "GEODESIC",
+ * acos(clamp(x . y, -1, 1)) (SphereClusterTree::geodesicDistance,
+ * FMCA/src/Clustering/SphereClusterTree.h:65-69), the great-circle distance
+ * of points on the unit sphere.  As in FMCA only the kernel entries change:
+ * the trees and the admissibility stay Euclidean.  Geodesic entries are
+ * formed in FP64 (x[0] y[0] followed by one fma per axis, then acos): actions
+ * in FMCA_PRECISION_FP64 in both orders, transposed, FMCA_KERNEL_VALUE and
+ * FMCA_KERNEL_DL variants, FP64 solves and FP64 assembled or projected
+ * blocks; the FP32 forms, FMCA_KERNEL_PHI and the gradient form return
+ * FMCA_ERR_NOT_SUPPORTED.  An unknown name is FMCA_ERR_INVALID_ARGUMENT
+ * (FMCA: assert "desired distance not implemented"); names are
+ * case-insensitive.  Assembled blocks are released on a change. */
+fmca_status fmca_h2_set_distance_type(fmca_h2_actor *actor, const char *dist_type);
+
+/* The actor's distance type: *geodesic = 1 for "GEODESIC", 0 for "EUCLIDEAN". */
+fmca_status fmca_h2_distance_type(const fmca_h2_actor *actor, int *geodesic);
+
 /* Kernel variants of an action (no FMCA counterpart; the differentiation
  * form).  The H2 operator is linear in the kernel function, so derivatives
  * of the compressed matrix are H2 actions with derivative kernels on the same

@@ -61,11 +61,25 @@ class CovarianceKernel {
   const double &l() const { return l_; }
   const double &c() const { return c_; }
   const double &nu() const { return nu_; }
+  // CovarianceKernel::setDistanceType (CovarianceKernel.h:123-137):
+  // "EUCLIDEAN" or "GEODESIC"; checked when an actor takes the kernel.
+  void setDistanceType(const std::string &dist_type) { dist_ = dist_type; }
+  const std::string &distanceType() const { return dist_; }
 
  private:
   std::string ktype_;
   double l_, c_, nu_;
+  std::string dist_ = "EUCLIDEAN";
 };
+
+// Whether a kernel class says its distance type (this header's kernel does;
+// FMCA::CovarianceKernel keeps it private, so an actor built from one uses
+// the Euclidean distance).
+template <typename K, typename = void>
+struct has_distance_type : std::false_type {};
+template <typename K>
+struct has_distance_type<K, std::void_t<decltype(std::declval<const K &>().distanceType())>>
+    : std::true_type {};
 
 enum class Precision : int { FP64 = FMCA_PRECISION_FP64, FP32 = FMCA_PRECISION_FP32 };
 
@@ -142,6 +156,9 @@ class H2kernelActor {
                             (int)planner, &h),
           "fmca_h2_create_ex");
     h_.reset(h, [](fmca_h2_actor *p) { fmca_h2_destroy(p); });
+    if constexpr (has_distance_type<Covariance>::value)
+      check(fmca_h2_set_distance_type(h_.get(), std::string(kernel.distanceType()).c_str()),
+            "fmca_h2_set_distance_type");
   }
 
   // H2kernelActor::action (H2kernelActor.h:38-80): K(Peval, P) * rhs, natural order.
@@ -190,6 +207,13 @@ class H2kernelActor {
   }
   void release_assembled() {
     check(fmca_h2_release_assembled(h_.get()), "fmca_h2_release_assembled");
+  }
+
+  // The distance of the kernel entries, "EUCLIDEAN" or "GEODESIC"
+  // (CovarianceKernel::setDistanceType, CovarianceKernel.h:123-137), for a
+  // kernel class that does not say it (FMCA::CovarianceKernel).
+  void setDistanceType(const std::string &dist_type) {
+    check(fmca_h2_set_distance_type(h_.get(), dist_type.c_str()), "fmca_h2_set_distance_type");
   }
 
   int64_t rows() const { return m_; }

@@ -15,6 +15,10 @@ Reference anchors (FMCA repository paths):
 
 Pinning status of each oracle function (what checks it in tests/test_oracle.py):
   kernels          closed forms, scipy's Matern-nu formula, k(0) = 1
+  geodesic         the great-circle distance 2 asin(|x - y| / 2) on the unit
+                   sphere, orthogonal / antipodal / coincident points, the
+                   dense matrix against numpy's arccos(clip(P^T P)), and the H2
+                   action against it, small and falling in degree
   interpolation    closed-form nodes/weights, Lagrange delta property,
                    partition of unity, polynomial reproduction
   tree             bijective permutation, leaf sizes in [q, 2q-1], children
@@ -66,6 +70,8 @@ def lib():
             ctypes.POINTER(ctypes.c_int), ctypes.c_long
         L.orc_parse_kernel.argtypes = [ctypes.c_char_p]
         L.orc_parse_kernel.restype = ctypes.c_int
+        L.orc_geodesic_flag.argtypes = []
+        L.orc_geodesic_flag.restype = ctypes.c_int
         L.orc_kernel_eval.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, vp, lp, vp]
         L.orc_interp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp,
@@ -120,10 +126,17 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def kernel_id(name: str) -> int:
+def kernel_id(name: str, distance: str = "EUCLIDEAN") -> int:
+    """Kernel type id; distance "GEODESIC" sets the oracle's geodesic bit
+    (CovarianceKernel::setDistanceType, CovarianceKernel.h:123-137)."""
     k = lib().orc_parse_kernel(name.encode())
     if k < 0:
         raise ValueError(f"unknown kernel {name!r}")
+    d = distance.upper()
+    if d == "GEODESIC":
+        return k | lib().orc_geodesic_flag()
+    if d != "EUCLIDEAN":
+        raise ValueError(f"unknown distance {distance!r}")
     return k
 
 
@@ -177,23 +190,23 @@ def eval_polynomials(dim: int, deg: int, pts: np.ndarray, scheme: str = "tensor"
     return out
 
 
-def dense_action(P, Peval, kernel, l, rhs, c=1.0, nu=1.0, nthreads=0):
+def dense_action(P, Peval, kernel, l, rhs, c=1.0, nu=1.0, nthreads=0, distance="EUCLIDEAN"):
     P = np.asfortranarray(P, dtype=np.float64)
     Peval = np.asfortranarray(Peval, dtype=np.float64)
     rhs = np.asfortranarray(rhs.reshape(P.shape[1], -1), dtype=np.float64)
     out = np.empty((Peval.shape[1], rhs.shape[1]), order="F")
     lib().orc_dense_action(P.shape[0], P.shape[1], _ptr(P), Peval.shape[1], _ptr(Peval),
-                           kernel_id(kernel), l, c, nu, _ptr(rhs), rhs.shape[1], _ptr(out),
-                           nthreads)
+                           kernel_id(kernel, distance), l, c, nu, _ptr(rhs), rhs.shape[1],
+                           _ptr(out), nthreads)
     return out
 
 
-def dense_matrix(P, Peval, kernel, l, c=1.0, nu=1.0):
+def dense_matrix(P, Peval, kernel, l, c=1.0, nu=1.0, distance="EUCLIDEAN"):
     P = np.asfortranarray(P, dtype=np.float64)
     Peval = np.asfortranarray(Peval, dtype=np.float64)
     out = np.empty((Peval.shape[1], P.shape[1]), order="F")
     lib().orc_dense_matrix(P.shape[0], P.shape[1], _ptr(P), Peval.shape[1], _ptr(Peval),
-                           kernel_id(kernel), l, c, nu, _ptr(out))
+                           kernel_id(kernel, distance), l, c, nu, _ptr(out))
     return out
 
 
@@ -203,7 +216,8 @@ class OracleH2:
     "total_degree" (NystromMoments<TotalDegreeInterpolator>)."""
 
     def __init__(self, P, Peval, kernel: str, l: float, deg: int, eta: float,
-                 c: float = 1.0, nu: float = 1.0, scheme: str = "tensor"):
+                 c: float = 1.0, nu: float = 1.0, scheme: str = "tensor",
+                 distance: str = "EUCLIDEAN"):
         self.P = np.asfortranarray(P, dtype=np.float64)
         self.Peval = np.asfortranarray(Peval, dtype=np.float64)
         self.dim = self.P.shape[0]
@@ -211,7 +225,8 @@ class OracleH2:
         self.m = self.Peval.shape[1]
         L = lib()
         self._h = L.orc_create(self.dim, self.n, _ptr(self.P), self.m, _ptr(self.Peval),
-                               kernel_id(kernel), l, c, nu, deg, eta, SCHEMES[scheme])
+                               kernel_id(kernel, distance), l, c, nu, deg, eta,
+                               SCHEMES[scheme])
         if not self._h:
             raise ValueError("oracle: invalid arguments")
         self.q = L.orc_q(self._h)

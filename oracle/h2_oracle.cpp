@@ -89,9 +89,17 @@ int parse_kernel(const char *name) {
   return -1;
 }
 
+// Bit of the kernel id that selects the geodesic distance.
+constexpr int kGeodesicFlag = 0x100;
+
 struct Kernel {
   int type;
   double l, c, nu;
+  bool geodesic = false;
+  Kernel() = default;
+  // id: kernel type, or'ed with kGeodesicFlag for setDistanceType("GEODESIC")
+  Kernel(int id, double l_, double c_, double nu_)
+      : type(id & ~kGeodesicFlag), l(l_), c(c_), nu(nu_), geodesic((id & kGeodesicFlag) != 0) {}
   double operator()(double r) const {
     switch (type) {
       case K_BIHARMONIC2D:
@@ -126,8 +134,18 @@ struct Kernel {
     }
     return std::numeric_limits<double>::quiet_NaN();
   }
-  // Euclidean distance (CovarianceKernel.h:126-129): sqrt of the sum of squares.
+  // Euclidean distance (CovarianceKernel.h:126-129): sqrt of the sum of
+  // squares.  Geodesic distance (CovarianceKernel.h:130-133 with
+  // SphereClusterTree.h:65-69): acos of the dot product clamped to [-1, 1];
+  // the dot product is x[0] y[0] followed by one fma per further axis (Eigen
+  // leaves the order of its reduction open; this one is fixed so that a
+  // second implementation can round alike where acos amplifies rounding).
   double operator()(const double *x, const double *y, int d) const {
+    if (geodesic) {
+      double dot = x[0] * y[0];
+      for (int i = 1; i < d; ++i) dot = std::fma(x[i], y[i], dot);
+      return (*this)(std::acos(std::min(1., std::max(-1., dot))));
+    }
     double s = 0;
     for (int i = 0; i < d; ++i) s += (x[i] - y[i]) * (x[i] - y[i]);
     return (*this)(std::sqrt(s));
@@ -887,6 +905,7 @@ using namespace orc;
 extern "C" {
 
 int orc_parse_kernel(const char *name) { return parse_kernel(name); }
+int orc_geodesic_flag() { return kGeodesicFlag; }
 
 // Kernel values k(r) for an array of distances.
 void orc_kernel_eval(int type, double l, double c, double nu, const double *r, long n,

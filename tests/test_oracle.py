@@ -380,3 +380,65 @@ def test_action_equals_blockwise_assembly(dim, deg, kernel):
         else:
             y[it] += f(cdist(Xt.T, Xs.T)) @ x[is_]
     assert rel_err(A.action(x), y) < 1e-12
+
+
+def _sphere(n, seed):
+    P = np.random.default_rng(seed).standard_normal((3, n))
+    return P / np.linalg.norm(P, axis=0)
+
+
+def test_geodesic_distance_closed_forms():
+    """setDistanceType("GEODESIC") (CovarianceKernel.h:123-137): the distance
+    is acos(clamp(x . y, -1, 1)) (SphereClusterTree.h:65-69).  BIHARMONIC3D
+    with l = 1 is k(r) = r, so the kernel matrix is the distance itself."""
+    def dist(X, Y):
+        return oracle.dense_matrix(np.asarray(Y, float), np.asarray(X, float), "BIHARMONIC3D",
+                                   1.0, distance="GEODESIC")
+
+    e = np.eye(3)
+    D = dist(e, np.hstack([e, -e]))
+    assert np.allclose(D[:, :3], np.pi / 2 * (1 - np.eye(3)), atol=0, rtol=1e-15)
+    assert np.allclose(np.diag(D[:, 3:]), np.pi, rtol=1e-15)
+    # great-circle distance of unit vectors: 2 asin(chord / 2), where acos is
+    # well conditioned (angles above 1e-3)
+    X, Y = _sphere(300, 30), _sphere(300, 31)
+    G = dist(X, Y)
+    chord = np.linalg.norm(X[:, :, None] - Y[:, None, :], axis=0)
+    ref = 2 * np.arcsin(chord / 2)
+    ok = ref > 1e-3
+    assert ok.mean() > 0.99
+    assert np.abs(G - ref)[ok].max() < 1e-12
+    # off the sphere the dot product is clamped: x . y >= 1 is distance 0,
+    # x . y <= -1 is pi
+    far = np.array([[2.0, 0.0], [0.0, 0.0], [0.0, 0.0]])
+    assert dist(far[:, :1], far[:, :1])[0, 0] == 0.0
+    assert dist(far[:, :1], -far[:, :1])[0, 0] == np.pi
+    # the Euclidean default is the chord
+    assert np.allclose(oracle.dense_matrix(Y, X, "BIHARMONIC3D", 1.0), chord, rtol=1e-13)
+
+
+@pytest.mark.parametrize("kernel,l", [("GAUSSIAN", 0.5), ("MATERN32", 0.5)])
+def test_geodesic_action_error_vs_dense_falls_with_degree(kernel, l):
+    """The H2 action with geodesic entries (near blocks between points, far
+    blocks between interpolation nodes off the sphere) against the dense
+    geodesic product formed in numpy (arccos of clipped P^T P): small, and
+    falling in the degree.  The Euclidean chord in either part would leave an
+    error near the 15 % by which the two dense products differ."""
+    n = 20000
+    P = _sphere(n, 32)
+    x = np.random.default_rng(33).standard_normal((n, 2))
+    f = {"GAUSSIAN": lambda r: np.exp(-0.5 * r * r / (l * l)),
+         "MATERN32": lambda r: (1 + np.sqrt(3) * r / l) * np.exp(-np.sqrt(3) * r / l)}[kernel]
+    yd = np.zeros((n, 2))
+    for b in range(0, n, 2000):
+        yd[b:b + 2000] = f(np.arccos(np.clip(P[:, b:b + 2000].T @ P, -1, 1))) @ x
+    yo = oracle.dense_action(P, P, kernel, l, x, distance="GEODESIC")
+    # the two dense products differ by the rounding of x . y near the diagonal
+    assert rel_err(yo, yd) < 1e-8
+    assert rel_err(oracle.dense_action(P, P, kernel, l, x), yd) > 1e-2
+    errs = []
+    for deg in (2, 3, 4):
+        A = oracle.OracleH2(P, P, kernel, l, deg, 0.8, distance="GEODESIC")
+        assert A.entry_counts()["far_blocks"] > 1000
+        errs.append(rel_err(A.action(x), yd))
+    assert errs[0] < 1e-2 and errs[2] < 2e-4 and errs[1] < errs[0] and errs[2] < errs[1], errs
