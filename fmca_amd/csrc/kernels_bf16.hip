@@ -56,6 +56,8 @@ __device__ __forceinline__ f32x4 mma_f32(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+__device__ __forceinline__ f32x4 splat4(float v) { return f32x4{v, v, v, v}; }
+
 __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0xff; }
 
 // hi = bf16(v) rounded to nearest even, lo = bf16(v - hi)
@@ -87,14 +89,51 @@ __device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &
 // Kernels smooth at r = 0 take their squared distances from the matrix
 // cores: for 16 sources and 16 targets one v_mfma_f32_16x16x4_f32 of the
 // source rows [-2 y', |y'|^2] (k-slots 0..D-1 and 3) with the target columns
-// [x', 1] gives |y'|^2 - 2 x'.y', and one add of |x'|^2 per entry completes
-// r^2 (x', y' centred on the target leaf).  In FP32 the cancellation leaves
-// an absolute error of a few eps |y'|^2 in r^2, which moves k by at most
-// |dk/d(r^2)| times that -- bounded for these kernels, unbounded at r -> 0
-// for kernels with a cusp there (EXPONENTIAL and the run-time kernels keep
-// the direct differences).
+// [x', 1], accumulated onto |x'|^2 (the accumulator input, one value per
+// target column), gives r^2 = |y'|^2 - 2 x'.y' + |x'|^2 (x', y' centred on
+// the target leaf) with no VALU work.  In FP32 the cancellation leaves an
+// absolute error of a few eps |y'|^2 in r^2, of either sign; the entries take
+// |r^2| (a free source modifier), which moves k by at most |dk/d(r^2)| times
+// that error -- bounded for these kernels, unbounded at r -> 0 for kernels
+// with a cusp there (EXPONENTIAL and the run-time kernels keep the direct
+// differences in the near field).
 template <int KT>
 constexpr bool kGramNear = KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN;
+
+// The Gram distances of these kernels are taken in scaled coordinates
+// sigma x, so that the exponential needs no multiply of its own:
+//   MATERN32/52: sigma = s log2(e), r' = sigma r, s r = r' ln 2,
+//                k = p(r' ln 2) exp2(-r')
+//   GAUSSIAN:    sigma^2 = log2(e) / (2 l^2), k = exp2(-r'^2)
+// (the negation is a source modifier).  Other kernels take sigma = 1 and
+// kernel_r2.
+template <int KT>
+__device__ __forceinline__ float gram_sigma(const KernelParams<float> &kp) {
+  if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52)
+    return kp.sl2;
+  else if constexpr (KT == KT_GAUSSIAN)
+    return __builtin_sqrtf(-kp.sl2);
+  else
+    return 1.f;
+}
+
+// Kernel value from the squared distance in the coordinates of gram_sigma.
+template <int KT>
+__device__ __forceinline__ float gram_kernel(float r2s, const KernelParams<float> &kp) {
+  constexpr float kLn2 = 0.693147180559945309f;
+  if constexpr (KT == KT_MATERN32) {
+    const float r = __builtin_amdgcn_sqrtf(r2s);
+    return fmat(r, kLn2, 1.f) * __builtin_amdgcn_exp2f(-r);
+  } else if constexpr (KT == KT_MATERN52) {
+    const float r = __builtin_amdgcn_sqrtf(r2s);
+    // 1 + a + a^2 / 3 with a = r ln 2
+    return fmat(r, fmat(r, kLn2 * kLn2 / 3.f, kLn2), 1.f) * __builtin_amdgcn_exp2f(-r);
+  } else if constexpr (KT == KT_GAUSSIAN) {
+    return __builtin_amdgcn_exp2f(-r2s);
+  } else {
+    return kernel_r2<KT, float>(r2s, kp);
+  }
+}
 
 // Lane l of the distance MFMA's output holds sources 4 (l / 16) + v of each
 // 16-source half of a K-step; the BF16 contraction reads k-slots
@@ -185,6 +224,7 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
     // distance MFMA operands of the targets: xb = component lk of [x', 1]
     // (x' = x - centre of the leaf), xx = |x'|^2
     float ctr[D], xb[kRT], xx[kRT];
+    const float sig = gram_sigma<KT>(kp);
 #pragma unroll
     for (int d = 0; d < D; ++d)
       ctr[d] = rows.bbmin[(int64_t)c * D + d] + 0.5f * rows.bbsize[(int64_t)c * D + d];
@@ -193,7 +233,7 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
       float s2 = 0.f, b = lk == 3 ? 1.f : 0.f;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const float v = x[rt][d] - ctr[d];
+        const float v = (x[rt][d] - ctr[d]) * sig;
         s2 = fmat(v, v, s2);
         if (d == lk) b = v;
       }
@@ -292,7 +332,7 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
             float s2 = 0.f;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-              const float v = py[d] - ctr[d];
+              const float v = (py[d] - ctr[d]) * sig;
               Ys[d * kChunk + jj] = -2.f * v;
               s2 = fmat(v, v, s2);
             }
@@ -334,8 +374,8 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
             const float ya = Ys[lk * kChunk + ks + lr], yb = Ys[lk * kChunk + ks + 16 + lr];
             f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
             if (ALL || tile_on[0]) {
-              g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
-              g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
+              g0 = mma_f32(ya, xb[0], splat4(xx[0]));
+              g1 = mma_f32(yb, xb[0], splat4(xx[0]));
             }
             // right operands of the K-step, shared by the row tiles
             bf16x8 bh[NT], bl[NT];
@@ -348,8 +388,8 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
             for (int rt = 0; rt < kRT; ++rt) {
               f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
               if (rt + 1 < kRT && (ALL || tile_on[rt + 1])) {
-                n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
-                n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
+                n0 = mma_f32(ya, xb[rt + 1], splat4(xx[rt + 1]));
+                n1 = mma_f32(yb, xb[rt + 1], splat4(xx[rt + 1]));
               }
               const f32x4 c0 = g0, c1 = g1;
               g0 = n0;
@@ -358,8 +398,8 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
               float ev[8];
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
-                ev[v] = kernel_r2<KT, float>(fmaxf(c0[v] + xx[rt], 0.f), kp);
-                ev[4 + v] = kernel_r2<KT, float>(fmaxf(c1[v] + xx[rt], 0.f), kp);
+                ev[v] = gram_kernel<KT>(fabsf(c0[v]), kp);
+                ev[4 + v] = gram_kernel<KT>(fabsf(c1[v]), kp);
               }
               bf16x8 ah, al;
               split8(ev, ah, al);
@@ -484,6 +524,7 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
     U[e] = ip.td ? ip.unodes[d * q + bb] : ip.xi[unpack(basis_index<float, D>(bb, ip), d)];
   }
   float ctr[D], xb[kRT], xx[kRT];
+  const float sig = gram_sigma<KT>(kp);
   bool tile_on[kRT];
 #pragma unroll
   for (int d = 0; d < D; ++d) ctr[d] = rows.bbmin[t * D + d] + 0.5f * rows.bbsize[t * D + d];
@@ -497,7 +538,7 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const float u = ip.td ? ip.unodes[d * q + row] : ip.xi[unpack(pk, d)];
-      const float v = u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d] - ctr[d];
+      const float v = (u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d] - ctr[d]) * sig;
       s2 = fmat(v, v, s2);
       if (d == lk) b = v;
     }
@@ -533,8 +574,8 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
         // distances of row tile rt + 1 issued before the entries of rt
         f32x4 g0 = f32x4{0, 0, 0, 0}, g1 = f32x4{0, 0, 0, 0};
         if (ALL || tile_on[0]) {
-          g0 = mma_f32(ya, xb[0], f32x4{0, 0, 0, 0});
-          g1 = mma_f32(yb, xb[0], f32x4{0, 0, 0, 0});
+          g0 = mma_f32(ya, xb[0], splat4(xx[0]));
+          g1 = mma_f32(yb, xb[0], splat4(xx[0]));
         }
         bf16x8 bh[NT], bl[NT];
 #pragma unroll
@@ -546,8 +587,8 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
         for (int rt = 0; rt < kRT; ++rt) {
           f32x4 n0 = f32x4{0, 0, 0, 0}, n1 = f32x4{0, 0, 0, 0};
           if (rt + 1 < kRT && (ALL || tile_on[rt + 1])) {
-            n0 = mma_f32(ya, xb[rt + 1], f32x4{0, 0, 0, 0});
-            n1 = mma_f32(yb, xb[rt + 1], f32x4{0, 0, 0, 0});
+            n0 = mma_f32(ya, xb[rt + 1], splat4(xx[rt + 1]));
+            n1 = mma_f32(yb, xb[rt + 1], splat4(xx[rt + 1]));
           }
           const f32x4 c0 = g0, c1 = g1;
           g0 = n0;
@@ -556,8 +597,8 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
           float ev[8];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            ev[v] = kernel_r2<KT, float>(fmaxf(c0[v] + xx[rt], 0.f), kp);
-            ev[4 + v] = kernel_r2<KT, float>(fmaxf(c1[v] + xx[rt], 0.f), kp);
+            ev[v] = gram_kernel<KT>(fabsf(c0[v]), kp);
+            ev[4 + v] = gram_kernel<KT>(fabsf(c1[v]), kp);
           }
           bf16x8 ah, al;
           split8(ev, ah, al);
@@ -639,7 +680,7 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
         float s2 = 0.f;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          const float v = py[d] - ctr[d];
+          const float v = (py[d] - ctr[d]) * sig;
           Yc[d * kChunk + bl] = -2.f * v;
           s2 = fmat(v, v, s2);
         }
