@@ -137,15 +137,27 @@ void householder_q(int rows, int cols, const double *A, const double *tau, doubl
 // C (m x n) = op(A) (m x k) * B (k x n), column-major, op = transpose if ta.
 void gemm(bool ta, int m, int n, int k, const double *A, int lda, const double *B, int ldb,
           double *C, int ldc) {
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < m; ++i) {
-      double s = 0;
-      if (ta)
-        for (int l = 0; l < k; ++l) s += A[l + (size_t)lda * i] * B[l + (size_t)ldb * j];
-      else
-        for (int l = 0; l < k; ++l) s += A[i + (size_t)lda * l] * B[l + (size_t)ldb * j];
-      C[i + (size_t)ldc * j] = s;
+  for (int j = 0; j < n; ++j) {
+    double *c = C + (size_t)ldc * j;
+    const double *b = B + (size_t)ldb * j;
+    if (ta) {
+      for (int i = 0; i < m; ++i) {
+        const double *a = A + (size_t)lda * i;
+        double s = 0;
+        for (int l = 0; l < k; ++l) s += a[l] * b[l];
+        c[i] = s;
+      }
+    } else {
+      // column of C as a sum of columns of A (unit stride), in the same
+      // order over l as the dot-product form
+      for (int i = 0; i < m; ++i) c[i] = 0;
+      for (int l = 0; l < k; ++l) {
+        const double bl = b[l];
+        const double *a = A + (size_t)lda * l;
+        for (int i = 0; i < m; ++i) c[i] += a[i] * bl;
+      }
     }
+  }
 }
 
 // CompareCluster::compare (Admissibility.h:95-115): 1 = low rank (admissible).
@@ -564,6 +576,8 @@ void plan_samplet_compression(SampletProgram &prog, const SampletTree &S, double
   size_t total = 0;
   for (auto &v : lev) total += v.size();
   B.pattern.reserve(total * 2);
+  B.pat_nodes.reserve(total * 2);
+  B.rec.reserve(total * 4);
   for (auto &v : lev)
     for (auto &e : v) B.pattern.emplace(pair_key(e.first, e.second), 1);
   for (int ll = nlev - 1; ll >= 0; --ll) {
@@ -649,51 +663,77 @@ void combine_samplet_blocks(const SampletProgram &prog, const SampletTree &S,
 }
 
 void samplet_triplets(const SampletProgram &prog, const SampletTree &S, const double *job_blocks,
-                      const std::vector<double> &comb, double threshold,
-                      std::vector<int64_t> &rows, std::vector<int64_t> &cols,
-                      std::vector<double> &vals) {
-  rows.clear();
-  cols.clear();
-  vals.clear();
-  for (size_t e = 0; e < prog.pat_row.size(); ++e) {
+                      const std::vector<double> &comb, double threshold, HostArray<int64_t> &rows,
+                      HostArray<int64_t> &cols, HostArray<double> &vals) {
+  const int64_t np = (int64_t)prog.pat_row.size();
+  // storeBlock's window of pattern entry e: the samplet rows / columns, all
+  // of the root's rows when the row cluster is the root
+  struct Window {
+    const double *B;
+    int dr, nr, nc, r0, c0;
+    int64_t srow, scol;
+  };
+  auto window = [&](int64_t e) {
     const int r = prog.pat_row[e], c = prog.pat_col[e], node = prog.pat_node[e];
-    const double *B = prog.op[node] == 0 ? job_blocks + prog.off[node] : comb.data() + prog.off[node];
-    const int dr = S.qdim[r];
-    // storeBlock's window of the block: the samplet rows / columns, all of
-    // the root's rows when the row cluster is the root
-    int nr, nc, r0, c0;
+    Window w;
+    w.B = prog.op[node] == 0 ? job_blocks + prog.off[node] : comb.data() + prog.off[node];
+    w.dr = S.qdim[r];
     if (r != 0 && c != 0) {
-      nr = S.nsamp[r], nc = S.nsamp[c];
-      r0 = dr - nr;
-      c0 = S.qdim[c] - nc;
+      w.nr = S.nsamp[r], w.nc = S.nsamp[c];
+      w.r0 = w.dr - w.nr;
+      w.c0 = S.qdim[c] - w.nc;
     } else if (c != 0) {
-      nr = dr, nc = S.nsamp[c];
-      r0 = 0;
-      c0 = S.qdim[c] - nc;
+      w.nr = w.dr, w.nc = S.nsamp[c];
+      w.r0 = 0;
+      w.c0 = S.qdim[c] - w.nc;
     } else {
-      nr = dr, nc = S.qdim[c];
-      r0 = c0 = 0;
+      w.nr = w.dr, w.nc = S.qdim[c];
+      w.r0 = w.c0 = 0;
     }
-    const int64_t srow = S.start[r], scol = S.start[c];
-    for (int k = 0; k < nc; ++k)
-      for (int j = 0; j < nr; ++j) {
-        const double v = B[(r0 + j) + (size_t)dr * (c0 + k)];
-        if ((srow + j <= scol + k && std::abs(v) > threshold) || (srow == scol && j == k)) {
-          rows.push_back(srow + j);
-          cols.push_back(scol + k);
-          vals.push_back(v);
+    w.srow = S.start[r];
+    w.scol = S.start[c];
+    return w;
+  };
+  // the entries kept: upper triangle above the threshold, and the diagonal
+  auto keep = [&](const Window &w, int j, int k, double v) {
+    return (w.srow + j <= w.scol + k && std::abs(v) > threshold) || (w.srow == w.scol && j == k);
+  };
+  std::vector<int64_t> off(np + 1, 0);
+  parallel_dynamic(np, 64, [&](int64_t e) {
+    const Window w = window(e);
+    int64_t cnt = 0;
+    for (int k = 0; k < w.nc; ++k)
+      for (int j = 0; j < w.nr; ++j)
+        cnt += keep(w, j, k, w.B[(w.r0 + j) + (size_t)w.dr * (w.c0 + k)]) ? 1 : 0;
+    off[e + 1] = cnt;
+  });
+  for (int64_t e = 0; e < np; ++e) off[e + 1] += off[e];
+  const size_t total = (size_t)off[np];
+  rows.resize_uninit(total);
+  cols.resize_uninit(total);
+  vals.resize_uninit(total);
+  parallel_dynamic(np, 64, [&](int64_t e) {
+    const Window w = window(e);
+    int64_t t = off[e];
+    for (int k = 0; k < w.nc; ++k)
+      for (int j = 0; j < w.nr; ++j) {
+        const double v = w.B[(w.r0 + j) + (size_t)w.dr * (w.c0 + k)];
+        if (keep(w, j, k, v)) {
+          rows.data()[t] = w.srow + j;
+          cols.data()[t] = w.scol + k;
+          vals.data()[t] = v;
+          ++t;
         }
       }
-  }
+  });
 }
 
-std::vector<int64_t> samplet_aposteriori_fast(const std::vector<int64_t> &rows,
-                                              const std::vector<int64_t> &cols,
-                                              const std::vector<double> &vals, double thres) {
+std::vector<int64_t> samplet_aposteriori_fast(const int64_t *rows, const int64_t *cols,
+                                              const double *vals, size_t count, double thres) {
   std::vector<std::vector<int64_t>> buckets(17);
   double norms2[17] = {0};
   const double invlog10 = 1. / std::log(10.);
-  for (size_t i = 0; i < vals.size(); ++i) {
+  for (size_t i = 0; i < count; ++i) {
     const double entry = std::abs(vals[i]);
     const double val = -std::floor(invlog10 * std::log(entry));
     // decade of the entry, the last bucket for 1e-16 and below (zero included)

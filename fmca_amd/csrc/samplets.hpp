@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -100,18 +101,34 @@ void combine_samplet_blocks(const SampletProgram &prog, const SampletTree &S,
 // triplets() (samplet_matrix_compressor.h:222-256): the upper triangle of the
 // compressed matrix in samplet coordinates, entries above threshold and the
 // whole diagonal.
+// A host array whose elements are not initialised on allocation (the
+// triplet lists and job blocks run to several GB; they are written in full
+// by parallel loops, which also spreads the first touch of their pages).
+template <typename T>
+struct HostArray {
+  std::unique_ptr<T[]> p;
+  size_t n = 0;
+  void resize_uninit(size_t k) {
+    p.reset(k ? new T[k] : nullptr);
+    n = k;
+  }
+  T *data() { return p.get(); }
+  const T *data() const { return p.get(); }
+  size_t size() const { return n; }
+};
+
+// Two passes over the pattern on host threads: entry counts, then the fill
+// at their prefix sums, so the order is FMCA's serial one.
 void samplet_triplets(const SampletProgram &prog, const SampletTree &S, const double *job_blocks,
-                      const std::vector<double> &comb, double threshold,
-                      std::vector<int64_t> &rows, std::vector<int64_t> &cols,
-                      std::vector<double> &vals);
+                      const std::vector<double> &comb, double threshold, HostArray<int64_t> &rows,
+                      HostArray<int64_t> &cols, HostArray<double> &vals);
 
 // aposteriori_triplets_fast (samplet_matrix_compressor.h:258-290): keep the
 // magnitude decades, largest first, until the dropped part is below thres of
 // the Frobenius norm; diagonal entries are always kept.  Returns the indices
 // of the kept triplets in the order FMCA emits them.
-std::vector<int64_t> samplet_aposteriori_fast(const std::vector<int64_t> &rows,
-                                              const std::vector<int64_t> &cols,
-                                              const std::vector<double> &vals, double thres);
+std::vector<int64_t> samplet_aposteriori_fast(const int64_t *rows, const int64_t *cols,
+                                              const double *vals, size_t count, double thres);
 
 // The GPU part (samplets.hip): evaluate every job of the program into
 // d_out (device, job_elems doubles).  Device inputs are prepared by the

@@ -21,8 +21,8 @@ struct fmca_samplet_tree {
 
 struct fmca_samplet_matrix {
   int64_t n = 0;
-  std::vector<int64_t> rows, cols;
-  std::vector<double> vals;
+  HostArray<int64_t> rows, cols;
+  HostArray<double> vals;
   double stats[FMCA_SAMPLET_STATS] = {0};
 };
 
@@ -252,12 +252,37 @@ fmca_status fmca_samplet_compress(const fmca_samplet_tree *tree, const fmca_kern
     HIP_TRY(A.upload(prog.jobs, &d.jobs));
     std::vector<double>().swap(W);
     // the job blocks in batches of at most 2 GiB of output
-    std::vector<double> blocks(prog.job_elems);
+    HostArray<double> blocks;  // every element is written by the copies below
+    blocks.resize_uninit((size_t)prog.job_elems);
     const int64_t cap = std::min<int64_t>(prog.job_elems, (int64_t)1 << 28);
     void *dout = nullptr;
     HIP_TRY(A.alloc(sizeof(double) * std::max<int64_t>(cap, 1), &dout));
     hipStream_t stream;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // the blocks come back through a pinned staging buffer, copied on from
+    // there by host threads (which also spreads the first touch of the pages
+    // of `blocks`): several times the rate of a copy into pageable memory
+    constexpr int64_t kStage = (int64_t)1 << 23;  // doubles (64 MiB)
+    double *pinned = nullptr;
+    if (hipHostMalloc((void **)&pinned, sizeof(double) * kStage, hipHostMallocDefault) !=
+        hipSuccess) {
+      (void)hipStreamDestroy(stream);
+      return fail(FMCA_ERR_OUT_OF_MEMORY, "pinned staging buffer");
+    }
+    auto download = [&](double *dst, const double *src, int64_t elems) -> hipError_t {
+      for (int64_t o = 0; o < elems; o += kStage) {
+        const int64_t m = std::min(kStage, elems - o);
+        const hipError_t e =
+            hipMemcpy(pinned, src + o, sizeof(double) * m, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return e;
+        constexpr int64_t kPiece = (int64_t)1 << 17;
+        parallel_dynamic((m + kPiece - 1) / kPiece, 1, [&](int64_t p) {
+          const int64_t a = p * kPiece, b = std::min(m, a + kPiece);
+          std::memcpy(dst + o + a, pinned + a, sizeof(double) * (b - a));
+        });
+      }
+      return hipSuccess;
+    };
     M->stats[6] = seconds_since(t0);
     t0 = std::chrono::steady_clock::now();
     SampletKernel K;
@@ -292,9 +317,10 @@ fmca_status fmca_samplet_compress(const fmca_samplet_tree *tree, const fmca_kern
       if (err == hipSuccess) err = hipStreamSynchronize(stream);
       gpu_s += seconds_since(tg);
       if (err == hipSuccess)
-        err = hipMemcpy(blocks.data() + base, dout, sizeof(double) * elems, hipMemcpyDeviceToHost);
+        err = download(blocks.data() + base, reinterpret_cast<const double *>(dout), elems);
       j0 = j1;
     }
+    (void)hipHostFree(pinned);
     (void)hipStreamDestroy(stream);
     if (err != hipSuccess) return from_hip(err, "samplet kernel blocks");
     M->stats[7] = gpu_s;
@@ -339,7 +365,8 @@ fmca_status fmca_samplet_matrix_aposteriori(const fmca_samplet_matrix *mat, doub
                                             int64_t *count, int64_t *index) {
   if (!mat || !count) return fail(FMCA_ERR_INVALID_ARGUMENT, "null argument");
   try {
-    const std::vector<int64_t> keep = samplet_aposteriori_fast(mat->rows, mat->cols, mat->vals, thres);
+    const std::vector<int64_t> keep = samplet_aposteriori_fast(
+        mat->rows.data(), mat->cols.data(), mat->vals.data(), mat->vals.size(), thres);
     if (index) std::memcpy(index, keep.data(), sizeof(int64_t) * keep.size());
     *count = (int64_t)keep.size();
   } catch (const std::bad_alloc &) {
