@@ -2,9 +2,12 @@
 // FMCA::H2kernelActor: construct from (kernel, Peval, P, deg, eta), call
 // action(rhs) on a column-major matrix.  Reads points and rhs from a binary
 // file, writes the result; "plan" mode only builds the actor.
-//   facade_check plan|action|plan_td|action_td|action_dev|action_td_dev <in.bin> <out.bin>
+//   facade_check plan|action|plan_td|action_td|action_dev|action_td_dev|
+//                action_geo|action_geo_set <in.bin> <out.bin>
 // (the _td modes instantiate the actor with a Moments type shaped like
-// FMCA's NystromMoments<TotalDegreeInterpolator>)
+// FMCA's NystromMoments<TotalDegreeInterpolator>; action_geo takes the
+// facade's CovarianceKernel with setDistanceType("GEODESIC"), action_geo_set
+// a kernel class without a distance and the actor's setDistanceType)
 // in.bin: int64 dim, n, nrhs, deg; double eta, l; char[16] kernel;
 //         double P[dim*n] (column-major); double rhs[n*nrhs]
 #include <cstdio>
@@ -78,6 +81,17 @@ int main(int argc, char **argv) {
   if (!in) return 3;
   try {
     const MyKernel kernel{std::string(kname), par[1]};
+    if (std::strcmp(argv[1], "action_geo") == 0) {
+      fmca_amd::CovarianceKernel k(std::string(kname), par[1]);
+      k.setDistanceType("GEODESIC");
+      fmca_amd::H2kernelActor<fmca_amd::CovarianceKernel> act(k, P, P, (unsigned)hdr[3], par[0]);
+      return run(act, argv[1], rhs, argv[3]);
+    }
+    if (std::strcmp(argv[1], "action_geo_set") == 0) {
+      fmca_amd::H2kernelActor<MyKernel> act(kernel, P, P, (unsigned)hdr[3], par[0]);
+      act.setDistanceType("GEODESIC");
+      return run(act, argv[1], rhs, argv[3]);
+    }
     const bool td = std::strstr(argv[1], "_td") != nullptr;
     // "_dev": plan on the device and stream assembled blocks
     const auto planner = std::strstr(argv[1], "_dev") ? fmca_amd::Planner::Device

@@ -111,6 +111,24 @@ def test_facade_device_planner_and_assembled_blocks(tmp_path, mode, scheme):
     assert rel_err(y, ref) <= 1e-12
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["action_geo", "action_geo_set"])
+def test_facade_geodesic_distance_matches_oracle(tmp_path, mode):
+    """setDistanceType("GEODESIC") through the facade: on its own kernel
+    class, and on the actor for a kernel class that does not carry a distance
+    (FMCA's keeps it private)."""
+    P = np.random.default_rng(28).standard_normal((3, 6000))
+    P /= np.linalg.norm(P, axis=0)
+    rhs = np.asfortranarray(np.random.default_rng(29).standard_normal((6000, 3)))
+    _write(tmp_path / "in.bin", P, rhs, 3, 0.8, "matern32", 0.5)
+    r = subprocess.run([_binary(), mode, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    y = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(3, 6000).T
+    ref = oracle.OracleH2(P, P, "MATERN32", 0.5, 3, 0.8, distance="GEODESIC").action(rhs)
+    assert rel_err(y, ref) <= 1e-12
+
+
 _MOMENTS_SNIPPET = r"""
 #include "fmca_amd/H2kernelActor.hpp"
 namespace FMCA {
