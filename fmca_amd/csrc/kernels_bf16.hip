@@ -360,9 +360,9 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
         more = nb < n1;
         if (more) fetch();
         const int nsteps = (nj + kKB - 1) & ~(kKB - 1);
-        // all four row tiles valid (the common case): no per-tile branches,
-        // so the four entry chains of a K-step can be interleaved
-        bool all_on = true;
+        // all four row tiles valid and both column tiles used (the common
+        // case): no branches in a K-step, one basic block for the scheduler
+        bool all_on = ntl == NT;
 #pragma unroll
         for (int rt = 0; rt < kRT; ++rt) all_on = all_on && tile_on[rt];
         auto gram_steps = [&](auto all_t) {
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256, 2) void k_leaf_output_bf16(
               split8(ev, ah, al);
 #pragma unroll
               for (int nt = 0; nt < NT; ++nt) {
-                if (nt >= ntl) continue;
+                if (!ALL && nt >= ntl) continue;
                 acc[rt][nt] = mma_bf16(ah, bh[nt], acc[rt][nt]);
                 acc[rt][nt] = mma_bf16(al, bh[nt], acc[rt][nt]);
                 acc[rt][nt] = mma_bf16(ah, bl[nt], acc[rt][nt]);
@@ -561,11 +561,11 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = f32x4{0, 0, 0, 0};
     const int ntl = min(NT, (nrhs - rc + 15) >> 4);
-    bool all_on = true;
+    bool all_on = ntl == NT;
 #pragma unroll
     for (int rt = 0; rt < kRT; ++rt) all_on = all_on && tile_on[rt];
-    // all four row tiles valid (the common case): no per-tile branches, so
-    // the four entry chains of a K-step can be interleaved
+    // all four row tiles valid and both column tiles used (the common case):
+    // no branches in a K-step, one basic block for the scheduler
     auto chunk_steps = [&](int nrows, auto all_t) {
       constexpr bool ALL = decltype(all_t)::value;
       for (int ks = ks0; ks < nrows; ks += kstep) {
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256, 2) void k_coupling_bf16(
           split8(ev, ah, al);
 #pragma unroll
           for (int nt = 0; nt < NT; ++nt) {
-            if (nt >= ntl) continue;
+            if (!ALL && nt >= ntl) continue;
             acc[rt][nt] = mma_bf16(ah, bh[nt], acc[rt][nt]);
             acc[rt][nt] = mma_bf16(al, bh[nt], acc[rt][nt]);
             acc[rt][nt] = mma_bf16(ah, bl[nt], acc[rt][nt]);
