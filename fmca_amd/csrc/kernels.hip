@@ -99,30 +99,35 @@ __global__ __launch_bounds__(kBlock) void k_zero_leaf_rows(DevTreeView<T> rows,
 // 2. leaf moments: mom[t][alpha][r] = sum_p V_t[alpha][p] xs[p][r]
 // One workgroup per leaf; the 1-D factors (Lagrange or Legendre) of 64 points
 // at a time are staged in LDS and the basis value, a product of D factors, is
-// formed on the fly.
-template <typename T, int D, int NR>
+// formed on the fly.  G groups of kBlock / G threads split the NR columns, so
+// that q <= kBlock / G basis functions keep every lane busy.
+template <typename T, int D, int NR, int G>
 __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, InterpParams<T> ip,
                                                         const T *__restrict__ xs, int nrhs,
                                                         T *__restrict__ mom) {
+  constexpr int A = kBlock / G;  // basis functions per pass
+  constexpr int NRG = NR / G;    // columns per thread
+  static_assert(NR % G == 0 && A % 64 == 0, "groups of whole waves, whole columns");
   extern __shared__ unsigned char smem_raw[];
   T *F = reinterpret_cast<T *>(smem_raw);  // [D][P1][64]
   T *X = F + D * ip.P1 * 64;               // [64][NR]
   const int c = tr.leaves[blockIdx.x];
   const int b0 = tr.begin[c], sz = tr.size[c];
   const int P1 = ip.P1, q = ip.q;
+  const int al = threadIdx.x % A, g0 = (threadIdx.x / A) * NRG;
   for (int rc = 0; rc < nrhs; rc += NR) {
     const int nr = min(NR, nrhs - rc);
-    for (int abase = 0; abase < q; abase += kBlock) {
-      const int alpha = abase + threadIdx.x;
+    for (int abase = 0; abase < q; abase += A) {
+      const int alpha = abase + al;
       int a[D];
       {
         const int pk = basis_index<T, D>(alpha < q ? alpha : 0, ip);
 #pragma unroll
         for (int i = 0; i < D; ++i) a[i] = (pk >> (8 * i)) & 0xff;
       }
-      T acc[NR];
+      T acc[NRG];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) acc[r] = 0;
+      for (int r = 0; r < NRG; ++r) acc[r] = 0;
       for (int p0 = 0; p0 < sz; p0 += 64) {
         const int np = min(64, sz - p0);
         __syncthreads();
@@ -143,14 +148,14 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaf(DevTreeView<T> tr, Inter
 #pragma unroll
             for (int i = 1; i < D; ++i) v *= F[(i * P1 + a[i]) * 64 + jj];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += v * X[jj * NR + r];
+            for (int r = 0; r < NRG; ++r) acc[r] += v * X[jj * NR + g0 + r];
           }
         }
       }
       if (alpha < q) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
-          if (r < nr) mom[((int64_t)c * q + alpha) * nrhs + rc + r] = acc[r];
+        for (int r = 0; r < NRG; ++r)
+          if (g0 + r < nr) mom[((int64_t)c * q + alpha) * nrhs + rc + g0 + r] = acc[r];
       }
     }
   }
@@ -840,19 +845,31 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   // 2. leaf moments
   mark(PH_UPWARD_LEAF);
   {
+    // FP64: column groups, as many as leave kBlock / G >= q threads per group
+    // (n = 1e6, q = 125, 32 rhs: 0.79 -> 0.64 ms); FP32 is faster with one
+    // group (0.36 against 0.40 ms: the shared basis values then cover twice
+    // the columns)
+    const int G = sizeof(T) == 4 ? 1 : (q <= kBlock / 4 ? 4 : (q <= kBlock / 2 ? 2 : 1));
+#define FMCA_UPWARD_LEAF(NRV, GV)                                                              \
+  hipLaunchKernelGGL((k_upward_leaf<T, D, NRV, GV>), dim3(cols.nleaves), dim3(kBlock),        \
+                     ((size_t)D * P1 * 64 + 64 * NRV) * sizeof(T), stream, cols, ip, buf.xs,  \
+                     nrhs, buf.mom)
     if (nrhs == 1) {
-      const size_t lds = ((size_t)D * P1 * 64 + 64 * 1) * sizeof(T);
-      hipLaunchKernelGGL((k_upward_leaf<T, D, 1>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
-                         cols, ip, buf.xs, nrhs, buf.mom);
+      FMCA_UPWARD_LEAF(1, 1);
     } else if (nrhs <= 8) {
-      const size_t lds = ((size_t)D * P1 * 64 + 64 * 8) * sizeof(T);
-      hipLaunchKernelGGL((k_upward_leaf<T, D, 8>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
-                         cols, ip, buf.xs, nrhs, buf.mom);
+      if (G >= 2)
+        FMCA_UPWARD_LEAF(8, 2);
+      else
+        FMCA_UPWARD_LEAF(8, 1);
     } else {
-      const size_t lds = ((size_t)D * P1 * 64 + 64 * 32) * sizeof(T);
-      hipLaunchKernelGGL((k_upward_leaf<T, D, 32>), dim3(cols.nleaves), dim3(kBlock), lds, stream,
-                         cols, ip, buf.xs, nrhs, buf.mom);
+      if (G == 4)
+        FMCA_UPWARD_LEAF(32, 4);
+      else if (G == 2)
+        FMCA_UPWARD_LEAF(32, 2);
+      else
+        FMCA_UPWARD_LEAF(32, 1);
     }
+#undef FMCA_UPWARD_LEAF
   }
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // 3. upward transfers, deepest inner level first, up to the level of the
