@@ -253,6 +253,12 @@ constexpr int kMfmaMinRhs = 8;
 template <typename T>
 bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask, int td);
 
+// Leaf moments of the FP32 forms on the matrix cores (kernels_mfma.hip):
+// q <= 128 and >= kMfmaMinRhs rhs where the LDS staging fits.
+bool upward_leaf_mfma_supported(int D, int P1, int q, int nrhs);
+hipError_t launch_upward_leaf_mfma(const DevTreeView<float> &tr, const InterpParams<float> &ip,
+                                   const float *xs, int nrhs, float *mom, hipStream_t stream);
+
 // BF16 three-term forms of the FP32 leaf output / coupling (kernels_bf16.hip):
 // kernel values only, >= kMfmaMinRhs rhs, where the LDS staging fits.
 bool bf16_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask);

@@ -854,7 +854,17 @@ hipError_t action_d(const DevTreeView<T> &rows, const DevTreeView<T> &cols,
   hipLaunchKernelGGL((k_upward_leaf<T, D, NRV, GV>), dim3(cols.nleaves), dim3(kBlock),        \
                      ((size_t)D * P1 * 64 + 64 * NRV) * sizeof(T), stream, cols, ip, buf.xs,  \
                      nrhs, buf.mom)
-    if (nrhs == 1) {
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (upward_leaf_mfma_supported(D, P1, q, nrhs)) {
+        if ((err = launch_upward_leaf_mfma(cols, ip, buf.xs, nrhs, buf.mom, stream)) !=
+            hipSuccess)
+          return err;
+        done = true;
+      }
+    }
+    if (done) {
+    } else if (nrhs == 1) {
       FMCA_UPWARD_LEAF(1, 1);
     } else if (nrhs <= 8) {
       if (G >= 2)

@@ -577,6 +577,101 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   }
 }
 
+// ---------------------------------------------------------------------------
+// Leaf moments on FP32 MFMA, q <= 128: mom[t] (q x NC) = V_t (q x |t|) X_t
+// (|t| x NC) per leaf, one workgroup per leaf.  Row tiles are the basis
+// functions (16 each), K-steps 4 points; lane (row lr, k-slot lk) forms its
+// A value V_alpha(x_p) as the product of the D 1-D factors of point p,
+// staged in LDS 64 points at a time with the points' rhs rows.  Wave w owns
+// the row tiles w and w + 4 (q <= 128: at most 8 tiles) and all NT column
+// tiles.
+template <int D, int NT>
+__global__ __launch_bounds__(256) void k_upward_leaf_mfma(DevTreeView<float> tr,
+                                                          InterpParams<float> ip,
+                                                          const float *__restrict__ xs, int nrhs,
+                                                          float *__restrict__ mom) {
+  using M = Mfma<float>;
+  using acc_t = typename M::acc_t;
+  constexpr int NC = 16 * NT;
+  constexpr int kPts = 64;          // points staged per chunk
+  constexpr int kPtsP = kPts + 4;   // padded stride of a factor row
+  constexpr int NCP = NC + 4;       // padded stride of a staged rhs row
+  extern __shared__ unsigned char smem_raw[];
+  float *F = reinterpret_cast<float *>(smem_raw);  // [D][P1][kPtsP] factors
+  float *X = F + D * ip.P1 * kPtsP;                 // [kPts][NCP] rhs rows
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  const int c = tr.leaves[blockIdx.x];
+  const int b0 = tr.begin[c], sz = tr.size[c];
+  const int P1 = ip.P1, q = ip.q;
+  // the two row tiles of this wave and the multi-index of this lane's row
+  bool on[2];
+  int pk[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int row0 = (wave + 4 * t) * 16;
+    on[t] = row0 < q;
+    pk[t] = basis_index<float, D>(row0 + lr < q ? row0 + lr : 0, ip);
+  }
+  for (int rc = 0; rc < nrhs; rc += NC) {
+    acc_t acc[2][NT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[t][nt] = acc_t{0, 0, 0, 0};
+    for (int p0 = 0; p0 < sz; p0 += kPts) {
+      const int np = min(kPts, sz - p0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < D * kPts; e += 256) {
+        const int d = e / kPts, jj = e % kPts;
+        if (jj < np) {
+          basis_1d<float>(tr.unit[(int64_t)d * tr.npts + b0 + p0 + jj], ip,
+                          F + (d * P1) * kPtsP + jj, kPtsP);
+        } else {
+          for (int a = 0; a < P1; ++a) F[(d * P1 + a) * kPtsP + jj] = 0.f;
+        }
+      }
+      for (int e = threadIdx.x; e < kPts * NC; e += 256) {
+        const int jj = e / NC, r = e % NC;
+        X[jj * NCP + r] = (jj < np && rc + r < nrhs) ? xs[(int64_t)(b0 + p0 + jj) * nrhs + rc + r]
+                                                     : 0.f;
+      }
+      __syncthreads();
+      const int nsteps = (np + 3) & ~3;
+      for (int ks = 0; ks < nsteps; ks += 4) {
+        const int jj = ks + lk;
+        float b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) b[nt] = X[jj * NCP + nt * 16 + lr];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          if (!on[t]) continue;
+          float v = F[(0 * P1 + unpack(pk[t], 0)) * kPtsP + jj];
+#pragma unroll
+          for (int d = 1; d < D; ++d) v *= F[(d * P1 + unpack(pk[t], d)) * kPtsP + jj];
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) acc[t][nt] = M::mma(v, b[nt], acc[t][nt]);
+        }
+      }
+    }
+    // C row crow(lk, v) of tile t is basis function (wave + 4 t) 16 + crow
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      if (!on[t]) continue;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int col = rc + nt * 16 + lr;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int alpha = (wave + 4 * t) * 16 + M::crow(lk, v);
+          if (alpha < q && col < nrhs) mom[((int64_t)c * q + alpha) * nrhs + col] = acc[t][nt][v];
+        }
+      }
+    }
+  }
+}
+
 template <typename T, int NT>
 size_t leaf_output_mfma_lds(int D, int P1, int q) {
   constexpr int kChunk = kChunkOf<T, NT>;
@@ -686,6 +781,32 @@ hipError_t launch_mfma_d(bool output, const DevTreeView<T> &rows, const DevTreeV
 }
 
 }  // namespace
+
+// Leaf moments of the FP32 forms on the matrix cores (q <= 128, >= 8 rhs);
+// false: the caller takes the VALU kernel.
+bool upward_leaf_mfma_supported(int D, int P1, int q, int nrhs) {
+  return q <= 128 && nrhs >= kMfmaMinRhs && D >= 1 && D <= 3 &&
+         ((size_t)D * P1 * 68 + 64 * 36) * sizeof(float) <= 64 * 1024;
+}
+
+hipError_t launch_upward_leaf_mfma(const DevTreeView<float> &tr, const InterpParams<float> &ip,
+                                   const float *xs, int nrhs, float *mom, hipStream_t stream) {
+  const int D = tr.dim, P1 = ip.P1;
+  const int NT = nrhs > 16 ? 2 : 1;
+  const size_t lds = ((size_t)D * P1 * 68 + (size_t)64 * (16 * NT + 4)) * sizeof(float);
+#define FMCA_UPL(DV, NTV)                                                                   \
+  hipLaunchKernelGGL((k_upward_leaf_mfma<DV, NTV>), dim3(tr.nleaves), dim3(256), lds, stream, \
+                     tr, ip, xs, nrhs, mom)
+  if (D == 1) {
+    if (NT == 2) FMCA_UPL(1, 2); else FMCA_UPL(1, 1);
+  } else if (D == 2) {
+    if (NT == 2) FMCA_UPL(2, 2); else FMCA_UPL(2, 1);
+  } else {
+    if (NT == 2) FMCA_UPL(3, 2); else FMCA_UPL(3, 1);
+  }
+#undef FMCA_UPL
+  return hipGetLastError();
+}
 
 template <typename T>
 bool mfma_supported(bool output, int D, int P1, int q, int nrhs, int kt, int vmask, int td) {
