@@ -355,6 +355,7 @@ fmca_status fmca_samplet_matrix_triplets(const fmca_samplet_matrix *mat, int64_t
                                          int64_t *cols, double *vals) {
   if (!mat) return fail(FMCA_ERR_INVALID_ARGUMENT, "matrix is null");
   const size_t k = mat->vals.size();
+  if (k == 0) return FMCA_OK;
   if (rows) std::memcpy(rows, mat->rows.data(), sizeof(int64_t) * k);
   if (cols) std::memcpy(cols, mat->cols.data(), sizeof(int64_t) * k);
   if (vals) std::memcpy(vals, mat->vals.data(), sizeof(double) * k);
@@ -367,7 +368,7 @@ fmca_status fmca_samplet_matrix_aposteriori(const fmca_samplet_matrix *mat, doub
   try {
     const std::vector<int64_t> keep = samplet_aposteriori_fast(
         mat->rows.data(), mat->cols.data(), mat->vals.data(), mat->vals.size(), thres);
-    if (index) std::memcpy(index, keep.data(), sizeof(int64_t) * keep.size());
+    if (index && !keep.empty()) std::memcpy(index, keep.data(), sizeof(int64_t) * keep.size());
     *count = (int64_t)keep.size();
   } catch (const std::bad_alloc &) {
     return fail(FMCA_ERR_OUT_OF_MEMORY, "host memory");
