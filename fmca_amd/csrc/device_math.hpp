@@ -387,5 +387,80 @@ __device__ __forceinline__ int basis_index(int alpha, const InterpParams<T> &ip)
   return pk;
 }
 
+// Kernels smooth at r = 0 take their squared distances from the matrix
+// cores: for 16 sources and 16 targets one v_mfma_f32_16x16x4_f32 (BF16 forms) of the
+// source rows [-2 y', |y'|^2] (k-slots 0..D-1 and 3) with the target columns
+// [x', 1], accumulated onto |x'|^2 (the accumulator input, one value per
+// target column), gives r^2 = |y'|^2 - 2 x'.y' + |x'|^2 (x', y' centred on
+// the target leaf) with no VALU work.  In FP32 the cancellation leaves an
+// absolute error of a few eps |y'|^2 in r^2, of either sign; the entries take
+// |r^2| (a free source modifier), which moves k by at most |dk/d(r^2)| times
+// that error -- bounded for these kernels, unbounded at r -> 0 for kernels
+// with a cusp there (EXPONENTIAL and the run-time kernels keep the direct
+// differences in the near field).
+template <int KT>
+constexpr bool kGramNear = KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN;
+
+// The Gram distances of these kernels are taken in scaled coordinates
+// sigma x, so that the exponential needs no multiply of its own:
+//   MATERN32/52: FP32 sigma = s log2(e), r' = sigma r, s r = r' ln 2,
+//                k = p(r' ln 2) exp2(-r');
+//                FP64 sigma = s, k = p(r') exp(-r')
+//   GAUSSIAN:    FP32 sigma^2 = log2(e) / (2 l^2), k = exp2(-r'^2);
+//                FP64 sigma^2 = 1 / (2 l^2), k = exp(-r'^2)
+// (the negation is a source modifier).  Other kernels take sigma = 1 and
+// kernel_r2.
+template <int KT, typename T>
+__device__ __forceinline__ T gram_sigma(const KernelParams<T> &kp) {
+  if constexpr (sizeof(T) == 4) {
+    if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52)
+      return kp.sl2;
+    else if constexpr (KT == KT_GAUSSIAN)
+      return __builtin_sqrtf(-kp.sl2);
+    else
+      return 1.f;
+  } else {
+    if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52)
+      return kp.s;
+    else if constexpr (KT == KT_GAUSSIAN)
+      return __builtin_sqrt(-kp.s);
+    else
+      return 1.0;
+  }
+}
+
+// Kernel value from the squared distance in the coordinates of gram_sigma.
+template <int KT, typename T>
+__device__ __forceinline__ T gram_kernel(T r2s, const KernelParams<T> &kp) {
+  if constexpr (sizeof(T) == 4) {
+    constexpr float kLn2 = 0.693147180559945309f;
+    if constexpr (KT == KT_MATERN32) {
+      const float r = __builtin_amdgcn_sqrtf(r2s);
+      return fmat(r, kLn2, 1.f) * __builtin_amdgcn_exp2f(-r);
+    } else if constexpr (KT == KT_MATERN52) {
+      const float r = __builtin_amdgcn_sqrtf(r2s);
+      // 1 + a + a^2 / 3 with a = r ln 2
+      return fmat(r, fmat(r, kLn2 * kLn2 / 3.f, kLn2), 1.f) * __builtin_amdgcn_exp2f(-r);
+    } else if constexpr (KT == KT_GAUSSIAN) {
+      return __builtin_amdgcn_exp2f(-r2s);
+    } else {
+      return kernel_r2<KT, float>(r2s, kp);
+    }
+  } else {
+    using M = Math<double>;
+    if constexpr (KT == KT_MATERN32) {
+      const double a = M::sqrt_(r2s);
+      return (1.0 + a) * M::exp_(-a);
+    } else if constexpr (KT == KT_MATERN52) {
+      const double a = M::sqrt_(r2s);
+      return fmat(a, fmat(a, 1. / 3., 1.0), 1.0) * M::exp_(-a);
+    } else if constexpr (KT == KT_GAUSSIAN) {
+      return M::exp_(-r2s);
+    } else {
+      return kernel_r2<KT, double>(r2s, kp);
+    }
+  }
+}
+
 }  // namespace dev
 }  // namespace fmca_amd

@@ -86,55 +86,6 @@ __device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &
   }
 }
 
-// Kernels smooth at r = 0 take their squared distances from the matrix
-// cores: for 16 sources and 16 targets one v_mfma_f32_16x16x4_f32 of the
-// source rows [-2 y', |y'|^2] (k-slots 0..D-1 and 3) with the target columns
-// [x', 1], accumulated onto |x'|^2 (the accumulator input, one value per
-// target column), gives r^2 = |y'|^2 - 2 x'.y' + |x'|^2 (x', y' centred on
-// the target leaf) with no VALU work.  In FP32 the cancellation leaves an
-// absolute error of a few eps |y'|^2 in r^2, of either sign; the entries take
-// |r^2| (a free source modifier), which moves k by at most |dk/d(r^2)| times
-// that error -- bounded for these kernels, unbounded at r -> 0 for kernels
-// with a cusp there (EXPONENTIAL and the run-time kernels keep the direct
-// differences in the near field).
-template <int KT>
-constexpr bool kGramNear = KT == KT_MATERN32 || KT == KT_MATERN52 || KT == KT_GAUSSIAN;
-
-// The Gram distances of these kernels are taken in scaled coordinates
-// sigma x, so that the exponential needs no multiply of its own:
-//   MATERN32/52: sigma = s log2(e), r' = sigma r, s r = r' ln 2,
-//                k = p(r' ln 2) exp2(-r')
-//   GAUSSIAN:    sigma^2 = log2(e) / (2 l^2), k = exp2(-r'^2)
-// (the negation is a source modifier).  Other kernels take sigma = 1 and
-// kernel_r2.
-template <int KT>
-__device__ __forceinline__ float gram_sigma(const KernelParams<float> &kp) {
-  if constexpr (KT == KT_MATERN32 || KT == KT_MATERN52)
-    return kp.sl2;
-  else if constexpr (KT == KT_GAUSSIAN)
-    return __builtin_sqrtf(-kp.sl2);
-  else
-    return 1.f;
-}
-
-// Kernel value from the squared distance in the coordinates of gram_sigma.
-template <int KT>
-__device__ __forceinline__ float gram_kernel(float r2s, const KernelParams<float> &kp) {
-  constexpr float kLn2 = 0.693147180559945309f;
-  if constexpr (KT == KT_MATERN32) {
-    const float r = __builtin_amdgcn_sqrtf(r2s);
-    return fmat(r, kLn2, 1.f) * __builtin_amdgcn_exp2f(-r);
-  } else if constexpr (KT == KT_MATERN52) {
-    const float r = __builtin_amdgcn_sqrtf(r2s);
-    // 1 + a + a^2 / 3 with a = r ln 2
-    return fmat(r, fmat(r, kLn2 * kLn2 / 3.f, kLn2), 1.f) * __builtin_amdgcn_exp2f(-r);
-  } else if constexpr (KT == KT_GAUSSIAN) {
-    return __builtin_amdgcn_exp2f(-r2s);
-  } else {
-    return kernel_r2<KT, float>(r2s, kp);
-  }
-}
-
 // Lane l of the distance MFMA's output holds sources 4 (l / 16) + v of each
 // 16-source half of a K-step; the BF16 contraction reads k-slots
 // 8 (l / 16) .. + 7, so source s of a 32-block is staged at this position.

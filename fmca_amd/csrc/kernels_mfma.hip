@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "device.hpp"
 #include "device_math.hpp"
@@ -58,12 +59,22 @@ constexpr int kRowsP = kRows + 16;
 // n = 1e6, 32 rhs), 64 otherwise
 template <typename T, int NT>
 constexpr int kChunkOf = (sizeof(T) == 4 && NT == 2) ? 128 : 64;
+// rows of the leaf output's staged source block: D coordinates, or in FP64
+// the four k-slots of the distance rows
+template <typename T, int D>
+constexpr int kYsRows = (sizeof(T) == 8 && D < 4) ? 4 : D;
 
 // Occupancy target (waves per SIMD) of the MFMA kernels: FP32 fits 4 waves
 // without spilling; FP64 (twice the registers per value) and the
 // three-variant run-time switch keep the default.
 template <typename T, int NT, int KT, bool MULTI>
 constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2 && !(MULTI && KT < 0)) ? 3 : 1;
+// The FP64 leaf output with distances from the matrix cores asks for 2 waves
+// per SIMD: measured 45.2 ms at n = 1e6, 32 rhs (56.7 ms with the default
+// target), although the compiler reports 304 registers for 32 columns
+template <typename T, int NT, int KT, bool MULTI>
+constexpr int kMinWavesOut =
+    (sizeof(T) == 8 && !MULTI && kGramNear<KT>) ? 2 : kMinWaves<T, NT, KT, MULTI>;
 
 // Packed tensor index of every moment: a0 | a1 << 8 | a2 << 16.
 __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0xff; }
@@ -75,7 +86,7 @@ __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0x
 // MULTI: the rhs columns carry different kernel variants (kp.tv per 16-column
 // tile); every entry is evaluated once for all three variants.
 template <typename T, int D, int KT, int NT, bool MULTI>
-__global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_output_mfma(
+__global__ __launch_bounds__(256, (kMinWavesOut<T, NT, KT, MULTI>)) void k_leaf_output_mfma(
     DevTreeView<T> rows, DevTreeView<T> cols, DevPatternView pat, InterpParams<T> ip,
     KernelParams<T> kp, int nrhs, const T *__restrict__ xs, const T *__restrict__ tlhs,
     double *__restrict__ out, int64_t ldo, int natural) {
@@ -84,6 +95,9 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
   constexpr int NC = 16 * NT;
   constexpr int NCP = NC + 16;  // padded LDS row stride
   constexpr int kChunk = kChunkOf<T, NT>;
+  // FP64 near field of the kernels smooth at r = 0: squared distances from
+  // the matrix cores (kGramNear), 16 sources per distance MFMA
+  constexpr bool kGram = sizeof(T) == 8 && !MULTI && kGramNear<KT> && D <= 3;
   extern __shared__ unsigned char smem_raw[];
   const int P1 = ip.P1, q = ip.q;
   const int qpad = (q + 3) & ~3;
@@ -91,8 +105,8 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
   int *aidx = reinterpret_cast<int *>(F + D * P1 * kRowsP);  // [qpad] packed indices
   T *U = reinterpret_cast<T *>(aidx + ((qpad + 3) & ~3));    // union region
   T *Ms = U;                                            // [kChunk][NCP] coefficients
-  T *Ys = U;                                            // [D][kChunk] source points
-  T *Xs = U + D * kChunk;                               // [kChunk][NCP] source rhs
+  T *Ys = U;  // [D][kChunk] source points, or kGram: [kChunk][4] distance rows
+  T *Xs = U + kYsRows<T, D> * kChunk;                   // [kChunk][NCP] source rhs
   const int lane = threadIdx.x & 63;
   // wave index as a scalar: the row-tile guards below are then uniform branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -106,6 +120,14 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
   const int b0 = rows.begin[c], sz = rows.size[c];
   for (int e = threadIdx.x; e < qpad; e += blockDim.x)
     aidx[e] = basis_index<T, D>(e < q ? e : 0, ip);
+  // kGram: coordinates centred on the leaf's bounding box and scaled by sigma
+  T gctr[D] = {}, gsig = T(1);
+  if constexpr (kGram) {
+    gsig = gram_sigma<KT>(kp);
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      gctr[d] = rows.bbmin[c * D + d] + T(0.5) * rows.bbsize[c * D + d];
+  }
   for (int pbase = 0; pbase < sz; pbase += kRows) {
     // 1-D basis factors of the rows of this pass (thread <-> row)
     __syncthreads();
@@ -132,6 +154,24 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
 #pragma unroll
       for (int d = 0; d < D; ++d)
         x[rt][d] = i < sz ? rows.pts[(int64_t)d * rows.npts + b0 + i] : T(0);
+    }
+    // kGram: the distance MFMA's right operand, k-slot lk of target lr of
+    // each row tile ([x', 1], x' = sigma (x - centre of the leaf)), and its
+    // accumulator input |x'|^2
+    T gx[kRT], gxx[kRT];
+    if constexpr (kGram) {
+#pragma unroll
+      for (int rt = 0; rt < kRT; ++rt) {
+        T xx = 0, sel = lk == 3 ? T(1) : T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const T xd = gsig * (x[rt][d] - gctr[d]);
+          xx = fmat(xd, xd, xx);
+          sel = lk == d ? xd : sel;
+        }
+        gx[rt] = sel;
+        gxx[rt] = xx;
+      }
     }
     for (int rc = 0; rc < nrhs; rc += NC) {
       acc_t acc[kRT][NT];
@@ -180,7 +220,9 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
       // The chunks of the near list are staged through registers one chunk
       // ahead: the global loads of chunk c+1 overlap the MFMA work of chunk c.
       constexpr int KX = kChunk * NC / 256;          // staged rhs values per thread
-      constexpr int KY = (D * kChunk + 255) / 256;  // staged coordinates per thread
+      // staged coordinates per thread (kGram: thread j < kChunk holds the D
+      // coordinates of source j)
+      constexpr int KY = kGram ? D : (D * kChunk + 255) / 256;
       static_assert(kChunk * NC % 256 == 0, "staging layout");
       T py[KY], px[KX];
       int nb = n0, j0 = 0, pnj = 0;
@@ -190,13 +232,20 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
         const int s = pat.near_src[nb], sl = pat.near_src[nb + run - 1];
         const int sb = cols.begin[s], ss = cols.begin[sl] + cols.size[sl] - sb;
         pnj = min(kChunk, ss - j0);
+        if constexpr (kGram) {
+          const int jj = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          const int d = e / kChunk, jj = e % kChunk;
-          py[k] = (e < D * kChunk && jj < pnj)
-                      ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj]
-                      : T(0);
+          for (int d = 0; d < D; ++d)
+            py[d] = jj < pnj ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj] : gctr[d];
+        } else {
+#pragma unroll
+          for (int k = 0; k < KY; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            const int d = e / kChunk, jj = e % kChunk;
+            py[k] = (e < D * kChunk && jj < pnj)
+                        ? cols.pts[(int64_t)d * cols.npts + sb + j0 + jj]
+                        : T(0);
+          }
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
@@ -216,10 +265,26 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
       while (more) {
         const int nj = pnj;
         __syncthreads();
+        if constexpr (kGram) {
+          // distance row of source j: [-2 y', |y'|^2] in k-slots 0..D-1 and
+          // 3 (slots D..2 zero), y' = sigma (y - centre)
+          if (threadIdx.x < kChunk) {
+            T g[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
-          const int e = threadIdx.x + 256 * k;
-          if (e < D * kChunk) Ys[e] = py[k];
+            for (int d = 0; d < D; ++d) {
+              const T yd = gsig * (py[d] - gctr[d]);
+              g[d] = T(-2) * yd;
+              g[3] = fmat(yd, yd, g[3]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Ys[threadIdx.x * 4 + k] = g[k];
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < KY; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < D * kChunk) Ys[e] = py[k];
+          }
         }
 #pragma unroll
         for (int k = 0; k < KX; ++k) {
@@ -229,7 +294,37 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_leaf_out
         __syncthreads();
         more = nb < n1;
         if (more) fetch();
-        {
+        if constexpr (kGram) {
+          // 16 sources per group: one distance MFMA per row tile gives lane
+          // (lr, lk) the squared distances of target lr to sources
+          // lk + 4 v (v = 0..3: the FP64 accumulator layout), i.e. exactly
+          // its A value of K-step v.  Rows past nj carry zero rhs and the
+          // finite distance row of the leaf's centre.
+          auto group = [&](int s0, auto full) {
+            const T ga = Ys[(s0 + lr) * 4 + lk];
+            T bv[4][NT];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+              for (int nt = 0; nt < NT; ++nt) bv[v][nt] = Xs[(s0 + 4 * v + lk) * NCP + nt * 16 + lr];
+#pragma unroll
+            for (int rt = 0; rt < kRT; ++rt) {
+              if (!tile_on[rt]) continue;
+              const acc_t r2 = M::mma(ga, gx[rt], acc_t{gxx[rt], gxx[rt], gxx[rt], gxx[rt]});
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                if (!decltype(full)::value && s0 + 4 * v >= nj) break;
+                const T e = gram_kernel<KT>(__builtin_fabs(r2[v]), kp);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                  if (nt < ntl) acc[rt][nt] = M::mma(e, bv[v][nt], acc[rt][nt]);
+              }
+            }
+          };
+          int s0 = 0;
+          for (; s0 + 16 <= nj; s0 += 16) group(s0, std::true_type{});
+          if (s0 < nj) group(s0, std::false_type{});
+        } else {
           const int nsteps = (nj + 3) & ~3;
 #pragma unroll 2
           for (int ks = 0; ks < nsteps; ks += 4) {
@@ -677,7 +772,8 @@ size_t leaf_output_mfma_lds(int D, int P1, int q) {
   constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
   const size_t u1 = (size_t)kChunk * (16 * NT + 16);
-  const size_t u2 = (size_t)D * kChunk + (size_t)kChunk * (16 * NT + 16);
+  const int ys = (sizeof(T) == 8 && D < 4) ? 4 : D;  // kYsRows
+  const size_t u2 = (size_t)ys * kChunk + (size_t)kChunk * (16 * NT + 16);
   return (size_t)D * P1 * kRowsP * sizeof(T) + (size_t)((qpad + 3) & ~3) * sizeof(int) +
          (u1 > u2 ? u1 : u2) * sizeof(T);
 }

@@ -50,6 +50,10 @@ CASES = [
     ("1d-16rhs", 5000, None, 1, 4, "MATERN32", 0.2, 0.8, 16),
     ("rect-33rhs", 4000, 3000, 3, 3, "MATERN52", 0.4, 0.8, 33),
     ("2d-65rhs", 6000, None, 2, 4, "EXPONENTIAL", 0.3, 0.8, 65),
+    # FP64 near field with squared distances from the matrix cores (16-column
+    # and 32-column passes, partial last column tile)
+    ("gauss-3d-12rhs-gram", 6000, None, 3, 3, "GAUSSIAN", 0.3, 0.8, 12),
+    ("matern52-2d-20rhs-gram", 5000, 4000, 2, 4, "MATERN52", 0.3, 0.8, 20),
 ]
 
 
