@@ -69,11 +69,11 @@ constexpr int kYsRows = (sizeof(T) == 8 && D < 4) ? 4 : D;
 // three-variant run-time switch keep the default.
 template <typename T, int NT, int KT, bool MULTI>
 constexpr int kMinWaves = (sizeof(T) == 4 && NT <= 2 && !(MULTI && KT < 0)) ? 3 : 1;
-// The FP64 leaf output with distances from the matrix cores asks for 2 waves
-// per SIMD: measured 45.2 ms at n = 1e6, 32 rhs (56.7 ms with the default
-// target), although the compiler reports 304 registers for 32 columns
+// The FP64 kernels with distances from the matrix cores ask for 2 waves per
+// SIMD (256 registers, a few spills at 32 columns): leaf output 45.3 ms at
+// n = 1e6, 32 rhs, against 56.7 ms with the default target
 template <typename T, int NT, int KT, bool MULTI>
-constexpr int kMinWavesOut =
+constexpr int kMinWavesGram =
     (sizeof(T) == 8 && !MULTI && kGramNear<KT>) ? 2 : kMinWaves<T, NT, KT, MULTI>;
 
 // Packed tensor index of every moment: a0 | a1 << 8 | a2 << 16.
@@ -86,7 +86,7 @@ __device__ __forceinline__ int unpack(int p, int d) { return (p >> (8 * d)) & 0x
 // MULTI: the rhs columns carry different kernel variants (kp.tv per 16-column
 // tile); every entry is evaluated once for all three variants.
 template <typename T, int D, int KT, int NT, bool MULTI>
-__global__ __launch_bounds__(256, (kMinWavesOut<T, NT, KT, MULTI>)) void k_leaf_output_mfma(
+__global__ __launch_bounds__(256, (kMinWavesGram<T, NT, KT, MULTI>)) void k_leaf_output_mfma(
     DevTreeView<T> rows, DevTreeView<T> cols, DevPatternView pat, InterpParams<T> ip,
     KernelParams<T> kp, int nrhs, const T *__restrict__ xs, const T *__restrict__ tlhs,
     double *__restrict__ out, int64_t ldo, int natural) {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(256, (kMinWavesOut<T, NT, KT, MULTI>)) void k_leaf_
 // K-steps, and the four partial sums are added in LDS (in wave order, so the
 // result does not depend on scheduling): all waves stay busy.
 template <typename T, int D, int KT, int NT, bool MULTI>
-__global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
+__global__ __launch_bounds__(256, (kMinWavesGram<T, NT, KT, MULTI>)) void k_coupling_mfma(DevTreeView<T> rows, DevTreeView<T> cols,
                                                        DevPatternView pat, InterpParams<T> ip,
                                                        KernelParams<T> kp, int nrhs,
                                                        const T *__restrict__ mom,
@@ -403,12 +403,16 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
   constexpr int NC = 16 * NT;
   constexpr int NCP = NC + 16;  // padded LDS row stride
   constexpr int kChunk = kChunkOf<T, NT>;
+  // FP64 entries of the kernels smooth at r = 0: squared distances from the
+  // matrix cores, as in the leaf output (16 source nodes per distance MFMA)
+  constexpr bool kGram = sizeof(T) == 8 && !MULTI && kGramNear<KT> && D <= 3;
   extern __shared__ unsigned char smem_raw[];
   const int q = ip.q;
   const int qpad = (q + 3) & ~3;
   T *U = reinterpret_cast<T *>(smem_raw);  // [D][qpad] unit coordinates of the nodes
-  T *Yc = U + D * qpad;                    // [D][kChunk] coordinates of the chunk's rows
-  T *Ms = Yc + D * kChunk;                 // [kChunk][NCP] moments of the chunk's rows
+  // [D][kChunk] coordinates of the chunk's rows, or kGram: [kChunk][4] distance rows
+  T *Yc = U + D * qpad;
+  T *Ms = Yc + kYsRows<T, D> * kChunk;     // [kChunk][NCP] moments of the chunk's rows
   T *Red = Ms + kChunk * NCP;  // [2 kRT * 16][NC] partial sums of the K-split (q <= 128)
   const int lane = threadIdx.x & 63;
   // wave index as a scalar: the row-tile guards below are then uniform branches
@@ -455,6 +459,28 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
       x[rt][d] = u * rows.bbsize[t * D + d] + rows.bbmin[t * D + d];
     }
   }
+  // kGram: coordinates centred on the target cluster's box and scaled by
+  // sigma; the distance MFMA's right operand (k-slot lk of target node lr:
+  // [x', 1]) and accumulator input |x'|^2 per row tile
+  T gctr[D] = {}, gsig = T(1), gx[kRT], gxx[kRT];
+  if constexpr (kGram) {
+    gsig = gram_sigma<KT>(kp);
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      gctr[d] = rows.bbmin[t * D + d] + T(0.5) * rows.bbsize[t * D + d];
+#pragma unroll
+    for (int rt = 0; rt < kRT; ++rt) {
+      T xx = 0, sel = lk == 3 ? T(1) : T(0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const T xd = gsig * (x[rt][d] - gctr[d]);
+        xx = fmat(xd, xd, xx);
+        sel = lk == d ? xd : sel;
+      }
+      gx[rt] = sel;
+      gxx[rt] = xx;
+    }
+  }
   __syncthreads();  // U
   // source slot and node of row bl of a chunk of nsrc whole sources (G >= 2);
   // node -1: a padding row (its coordinates are then those of slot 0, node 0)
@@ -477,6 +503,33 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
     const int ntl = min(NT, (nrhs - rc + 15) >> 4);
     // K-steps of the staged chunk, rows [0, nrows)
     auto compute_chunk = [&](int nrows) {
+      if constexpr (kGram) {
+        // row groups of 16 (wave group sg takes every kspl-th): lane (lr, lk)
+        // gets the squared distances of target lr to rows lk + 4 v of the
+        // group, its A values of the group's K-steps v = 0..3.  Rows past
+        // nrows (at most kChunk) carry zero moments and a real node.
+        for (int g0 = 16 * sg; g0 < nrows; g0 += 16 * kspl) {
+          const T ga = Yc[(g0 + lr) * 4 + lk];
+          T bv[4][NT];
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bv[v][nt] = Ms[(g0 + 4 * v + lk) * NCP + nt * 16 + lr];
+#pragma unroll
+          for (int rt = 0; rt < kRT; ++rt) {
+            if (!tile_on[rt]) continue;
+            const acc_t r2 = M::mma(ga, gx[rt], acc_t{gxx[rt], gxx[rt], gxx[rt], gxx[rt]});
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const T e = gram_kernel<KT>(__builtin_fabs(r2[v]), kp);
+#pragma unroll
+              for (int nt = 0; nt < NT; ++nt)
+                if (nt < ntl) acc[rt][nt] = M::mma(e, bv[v][nt], acc[rt][nt]);
+            }
+          }
+        }
+        return;
+      }
 #pragma unroll 2
       for (int ks = ks0; ks < nrows; ks += kstep) {
         const int bl = ks + lk;
@@ -516,7 +569,9 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
     // current one; FP64 (one wave per SIMD) fetches just before staging, as a
     // lookahead measured slower there.
     constexpr int KM = kChunk * NC / 256;          // staged moments per thread
-    constexpr int KY = (D * kChunk + 255) / 256;  // staged coordinates per thread
+    // staged coordinates per thread (kGram: thread bl < kChunk holds the D
+    // coordinates of row bl)
+    constexpr int KY = kGram ? D : (D * kChunk + 255) / 256;
     static_assert(kChunk * NC % 256 == 0, "staging layout");
     T py[KY], pm[KM];
     int f = f0, nbe = 0;  // next chunk: sources from f, nodes from nbe (G = 1)
@@ -530,8 +585,20 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
         // one wave-wide moment load (64 / NC consecutive rows, starting at a
         // multiple of 64 / NC) never straddle a source since qpad is a
         // multiple of 4, so that load's source is wave-uniform
+        if constexpr (kGram) {
+          const int bl = threadIdx.x;
+          if (bl < kChunk) {
+            int slot, node;
+            chunk_row(bl, nsrc, slot, node);
+            const int s = pat.far_src[f + slot];
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
+            for (int d = 0; d < D; ++d)
+              py[d] = U[d * qpad + (node < 0 ? 0 : node)] * cols.bbsize[s * D + d] +
+                      cols.bbmin[s * D + d];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < (kGram ? 0 : KY); ++k) {
           const int e = threadIdx.x + 256 * k;
           const int d = e / kChunk, bl = e % kChunk;
           if (e < D * kChunk) {
@@ -555,8 +622,17 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
         // one source, nodes [be, be + kChunk)
         const int s = pat.far_src[f];
         const T *ms = mom + (int64_t)s * q * nrhs;
+        if constexpr (kGram) {
+          const int bl = threadIdx.x;
+          if (bl < kChunk) {
+            const int node = be + bl < q ? be + bl : 0;
 #pragma unroll
-        for (int k = 0; k < KY; ++k) {
+            for (int d = 0; d < D; ++d)
+              py[d] = U[d * qpad + node] * cols.bbsize[s * D + d] + cols.bbmin[s * D + d];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < (kGram ? 0 : KY); ++k) {
           const int e = threadIdx.x + 256 * k;
           const int d = e / kChunk, bl = e % kChunk;
           if (e < D * kChunk) {
@@ -583,10 +659,26 @@ __global__ __launch_bounds__(256, (kMinWaves<T, NT, KT, MULTI>)) void k_coupling
       }
     };
     auto stage = [&]() {
+      if constexpr (kGram) {
+        // distance row of node row bl: [-2 y', |y'|^2] in k-slots 0..D-1 and
+        // 3 (slots D..2 zero), y' = sigma (y - centre)
+        if (threadIdx.x < kChunk) {
+          T g[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-      for (int k = 0; k < KY; ++k) {
-        const int e = threadIdx.x + 256 * k;
-        if (e < D * kChunk) Yc[e] = py[k];
+          for (int d = 0; d < D; ++d) {
+            const T yd = gsig * (py[d] - gctr[d]);
+            g[d] = T(-2) * yd;
+            g[3] = fmat(yd, yd, g[3]);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) Yc[threadIdx.x * 4 + k] = g[k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < KY; ++k) {
+          const int e = threadIdx.x + 256 * k;
+          if (e < D * kChunk) Yc[e] = py[k];
+        }
       }
 #pragma unroll
       for (int k = 0; k < KM; ++k) {
@@ -784,7 +876,8 @@ size_t coupling_mfma_lds(int D, int q, bool td) {
   constexpr int kChunk = kChunkOf<T, NT>;
   const int qpad = (q + 3) & ~3;
   const size_t red = q <= 2 * kRT * 16 ? (size_t)2 * kRT * 16 * 16 * NT : 0;  // K-split partials
-  return ((size_t)D * qpad + (size_t)D * kChunk + (size_t)kChunk * (16 * NT + 16) + red) *
+  const int ys = (sizeof(T) == 8 && D < 4) ? 4 : D;  // kYsRows
+  return ((size_t)D * qpad + (size_t)ys * kChunk + (size_t)kChunk * (16 * NT + 16) + red) *
          sizeof(T);
 }
 
